@@ -1,0 +1,210 @@
+/*
+ * prores_hip.h — C ABI of the MI355X ProRes slice engine.
+ *
+ * The drop-in boundary for the rivet-prores crate's API (Rust). Each entry
+ * point names the reference item it replaces (file:line in the reference
+ * tree). Plain pointers and sizes only; no HIP or torch types cross it
+ * except the opaque device pointers of the device-resident entry points,
+ * which are plain `void *` / `uint8_t *` values.
+ *
+ * Errors: every call that can fail returns a prores_status and fills a
+ * prores_error whose kind and message text are the reference's
+ * (src/error.rs:5-21; the message strings of src/header.rs, src/decode.rs,
+ * src/bits.rs, src/vlc.rs, src/frame.rs, src/encode.rs). PRORES_ERR_DEVICE
+ * is this library's own: a HIP runtime failure (no device, out of memory).
+ */
+#ifndef PRORES_HIP_H
+#define PRORES_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error kinds: Invalid, Unsupported, Config (src/error.rs:5-21). */
+typedef enum {
+  PRORES_OK = 0,
+  PRORES_ERR_INVALID = 1,
+  PRORES_ERR_UNSUPPORTED = 2,
+  PRORES_ERR_CONFIG = 3,
+  PRORES_ERR_DEVICE = 4
+} prores_status;
+
+typedef struct {
+  int32_t kind;     /* a prores_status */
+  char msg[256];    /* the reference's message (UTF-8), without the "invalid ProRes data: " prefix */
+} prores_error;
+
+/* ChromaFormat codes (src/frame.rs:7-41): the header's chroma_format. */
+enum { PRORES_CHROMA_422 = 2, PRORES_CHROMA_444 = 3 };
+/* Interlace (src/frame.rs:46-77). */
+enum { PRORES_PROGRESSIVE = 0, PRORES_TOP_FIELD_FIRST = 1, PRORES_BOTTOM_FIELD_FIRST = 2 };
+/* AlphaType (src/header.rs:11-40). */
+enum { PRORES_ALPHA_NONE = 0, PRORES_ALPHA_8 = 1, PRORES_ALPHA_16 = 2 };
+/* Profile (src/encode.rs:30-44), in Profile::ALL order. */
+enum {
+  PRORES_PROFILE_PROXY = 0,
+  PRORES_PROFILE_LT = 1,
+  PRORES_PROFILE_STANDARD = 2,
+  PRORES_PROFILE_HQ = 3,
+  PRORES_PROFILE_4444 = 4,
+  PRORES_PROFILE_4444XQ = 5
+};
+
+/* Metadata (src/frame.rs:83-110). */
+typedef struct {
+  uint8_t aspect_ratio, frame_rate_code, color_primaries, transfer_characteristic,
+      matrix_coefficients;
+} prores_metadata;
+
+/* Plane (src/frame.rs:150-158): offset in samples into `data`. */
+typedef struct {
+  uint64_t offset;
+  uint32_t width, height;
+} prores_plane;
+
+/* Frame (src/frame.rs:180-203). data = Y' || Cb || Cr, u16 native-endian,
+ * each plane tightly packed; alpha = width*height samples or NULL. When the
+ * library fills a frame (prores_decode) it owns the buffers until
+ * prores_frame_free. */
+typedef struct {
+  uint32_t width, height, bit_depth;
+  uint32_t chroma, interlace;
+  prores_metadata metadata;
+  uint16_t *data;
+  size_t data_len;
+  prores_plane planes[3];
+  uint16_t *alpha;
+  size_t alpha_len;
+} prores_frame;
+
+/* FrameHeader (src/header.rs:44-72). */
+typedef struct {
+  uint32_t frame_size;
+  uint16_t header_size;
+  uint8_t bitstream_version;
+  uint8_t encoder_identifier[4];
+  uint16_t width, height;
+  uint32_t chroma, interlace;
+  prores_metadata metadata;
+  uint32_t alpha; /* PRORES_ALPHA_* */
+  uint8_t has_luma_matrix, has_chroma_matrix;
+  uint8_t luma_matrix[64], chroma_matrix[64];
+} prores_frame_header;
+
+/* Config (src/encode.rs:135-159). target_frame_bytes 0 = the profile's. */
+typedef struct {
+  uint32_t profile;
+  uint32_t log2_slice_mbs;
+  uint32_t alpha;
+  uint8_t has_luma_matrix, has_chroma_matrix;
+  uint8_t luma_matrix[64], chroma_matrix[64];
+  uint64_t target_frame_bytes;
+  uint8_t encoder_identifier[4];
+  uint32_t threads;
+} prores_config;
+
+typedef struct prores_decoder prores_decoder;
+
+/* ---- Decoder (src/decode.rs:19-104) ---- */
+
+/* Decoder::new / Decoder::with_bit_depth / with_threads
+ * (src/decode.rs:28-57). bit_depth 0 = natural (10 for 4:2:2, 12 for
+ * 4:4:4); otherwise 8..16 or a Config error. threads is accepted for
+ * drop-in parity and does not change the GPU path. */
+prores_status prores_decoder_create(uint32_t bit_depth, uint32_t threads, prores_decoder **out,
+                                    prores_error *err);
+void prores_decoder_destroy(prores_decoder *d);
+
+/* Decoder::decode (src/decode.rs:61-104): one frame of a MOV sample, bytes
+ * after frame_size ignored. The library allocates out->data / out->alpha. */
+prores_status prores_decode(const prores_decoder *d, const uint8_t *data, size_t len,
+                            prores_frame *out, prores_error *err);
+
+/* The same into caller buffers: out->data must hold the frame's planes
+ * (prores_frame_layout) and out->alpha width*height samples when the frame
+ * has alpha (prores_frame_header_parse says). No counterpart in the
+ * reference beyond Decoder::decode; for callers that reuse buffers. */
+prores_status prores_decode_into(const prores_decoder *d, const uint8_t *data, size_t len,
+                                 prores_frame *out, prores_error *err);
+
+void prores_frame_free(prores_frame *f);
+
+/* FrameHeader::parse (src/header.rs:82-170). */
+prores_status prores_frame_header_parse(const uint8_t *data, size_t len, prores_frame_header *out,
+                                        prores_error *err);
+
+/* Frame::layout (src/frame.rs:247-266): fills planes[3]; returns the total
+ * sample count of data. */
+size_t prores_frame_layout(uint32_t width, uint32_t height, uint32_t chroma, prores_plane *planes);
+
+/* ---- Encoder (src/encode.rs:195-287) ---- */
+
+/* Config::new (src/encode.rs:163-174). */
+void prores_config_default(uint32_t profile, prores_config *cfg);
+
+/* Encoder::encode (src/encode.rs:223-287). *out is malloc'd by the library;
+ * release with prores_buffer_free. */
+prores_status prores_encode(const prores_config *cfg, const prores_frame *frame, uint8_t **out,
+                            size_t *out_len, prores_error *err);
+void prores_buffer_free(uint8_t *p);
+
+/* Profile helpers (src/encode.rs:46-118). */
+uint64_t prores_profile_target_frame_bytes(uint32_t profile, uint32_t width, uint32_t height);
+void prores_profile_fourcc(uint32_t profile, char out[4]);
+int32_t prores_profile_from_fourcc(const char code[4]); /* -1 if none */
+
+/* ---- Device-resident batch entry points ----
+ * No counterpart in the reference (it runs on CPU threads). For callers that
+ * keep bitstreams and planes in GPU memory (the batched path, the serving
+ * loop, the benchmark): packets and outputs are device pointers; the host
+ * copy of each packet is read for its headers only.
+ */
+typedef struct {
+  const uint8_t *host_packet; /* host bytes: at least the frame and picture headers */
+  size_t len;                 /* bytes of the packet (>= frame_size) */
+  const uint8_t *dev_packet;  /* device copy of the packet; 64 readable bytes past len */
+  uint16_t *dev_data;         /* device output planes, Frame::layout at the decoder's depth */
+  uint16_t *dev_alpha;        /* device output alpha (width*height) or NULL if none */
+} prores_dev_decode_job;
+
+/* Decodes n frames with one launch sequence on the library's stream; waits
+ * for completion. errs[i] receives frame i's result (kind 0 = Ok). Returns
+ * PRORES_ERR_DEVICE only for runtime failures, else PRORES_OK. */
+prores_status prores_decode_device_batch(const prores_decoder *d, size_t n,
+                                         const prores_dev_decode_job *jobs, prores_error *errs);
+
+/* One frame to encode from device memory: frame.data / frame.alpha are
+ * DEVICE pointers laid out as frame.planes say; the rest of `frame` is host
+ * metadata. On return dev_out / out_len locate the packet inside the
+ * library's output arena (valid until the next encode batch on the device). */
+typedef struct {
+  const prores_config *cfg;
+  prores_frame frame;
+  const uint8_t *dev_out;
+  size_t out_len;
+} prores_dev_encode_job;
+
+/* Encodes n frames with one launch sequence (Encoder::encode semantics per
+ * frame, src/encode.rs:223-287); errs[i] receives frame i's result. */
+prores_status prores_encode_device_batch(size_t n, prores_dev_encode_job *jobs, prores_error *errs);
+
+/* Microseconds of the last decode batch's kernels by stage, measured with
+ * HIP events on the library's stream: [0] slice table, [1] entropy, [2]
+ * idct, [3] whole batch. No counterpart in the reference. */
+void prores_last_batch_timing(double *us4);
+
+/* Microseconds of the last encode batch by stage: [0] fdct + alpha, [1] rate
+ * control (probes, window, share-out, offsets), [2] pack, [3] whole batch
+ * (host sync for the output sizes included). */
+void prores_last_encode_timing(double *us4);
+
+/* Library build string. */
+const char *prores_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,417 @@
+// Decoder host runtime: Decoder::decode (src/decode.rs:61-104) over the
+// gfx950 kernels. The host parses the frame and picture headers (a few bytes
+// each) and lays out the batch; the slice-table walk, entropy decoding, IDCT
+// and placement run on the device. Errors come back per slice and are
+// reported as the reference does: the first in picture order.
+#include <stdlib.h>
+
+#include <algorithm>
+
+#include "host_internal.h"
+
+struct prores_decoder {
+  uint32_t bit_depth;  // 0: natural
+  uint32_t threads;
+};
+
+namespace prores_hip {
+namespace {
+
+const char *slice_message(uint32_t code) {
+  switch (code) {
+    case E_SLICE_SHORT: return "a slice is shorter than its header";
+    case E_COMP_SIZES: return "slice component sizes exceed the slice";
+    case E_PAST_END: return "a codeword runs past the end of its data";
+    case E_PREFIX: return "a variable-length code's prefix is too long";
+    case E_RANGE: return "a variable-length code's value is out of range";
+    case E_DC_OVERFLOW: return "a DC coefficient overflows";
+    case E_RUN_PAST: return "a run passes the end of the coefficient array";
+    case E_ALPHA_RUN: return "an alpha run passes the end of the slice";
+    default: return "unknown slice error";
+  }
+}
+
+Err slice_error(uint32_t word) {
+  uint32_t code = word & 0xFF, arg = word >> 8;
+  if (code == E_SLICE_HDR_SIZE) return invalid(fmtstr("slice_header_size %u is out of range", arg));
+  if (code == E_QINDEX) return invalid(fmtstr("quantization_index %u is reserved", arg));
+  return invalid(slice_message(code));
+}
+
+// One frame of a batch: where its pictures sit in the batch arrays, and the
+// error the host found in its headers, which counts only when every picture
+// before it decodes cleanly.
+struct FramePlan {
+  Err early;            // header errors before any picture: no device work
+  Err deferred;         // a picture header error after `n_pics` good pictures
+  uint32_t first_pic = 0, n_pics = 0;
+};
+
+struct Batch {
+  std::vector<DecFrame> frames;
+  std::vector<DecPic> pics;
+  std::vector<RowSlot> rows;
+  std::vector<FramePlan> plans;
+  uint64_t n_slices = 0, n_coefs = 0;
+};
+
+struct Workspace {
+  DevBuf frames, pics, rows, slices, status, first_err, slice_err, coefs, wide, packet, planes;
+};
+Workspace &workspace(Device &d) {
+  static Workspace ws[64];
+  return ws[d.id];
+}
+
+double g_last_us[4] = {0, 0, 0, 0};
+
+// Lays out one frame's pictures. `dev_packet` is the device address of the
+// packet; `dev_data` / `dev_alpha` the device output planes.
+void plan_frame(const uint8_t *host, size_t len, uint64_t dev_packet, uint32_t want_depth,
+                uint16_t *dev_data, uint16_t *dev_alpha, Batch &b, const Header *parsed) {
+  FramePlan fp;
+  Header hd;
+  if (parsed) {
+    hd = *parsed;
+  } else {
+    fp.early = parse_frame_header(host, len, hd);
+    if (fp.early.ok()) fp.early = check_plausible_size(hd);
+  }
+  if (!fp.early.ok()) {
+    b.plans.push_back(fp);
+    return;
+  }
+  const prores_frame_header &h = hd.h;
+  DecFrame F;
+  memset(&F, 0, sizeof F);
+  const uint32_t bd = want_depth ? want_depth : (h.chroma == 2 ? 10 : 12);
+  prores_plane planes[3];
+  prores_frame_layout(h.width, h.height, h.chroma, planes);
+  for (int i = 0; i < 3; i++) {
+    F.out_plane[i] = (uint64_t)(dev_data + planes[i].offset);
+    F.plane_w[i] = planes[i].width;
+  }
+  F.out_alpha = (uint64_t)dev_alpha;
+  F.width = h.width;
+  F.height = h.height;
+  F.chroma = h.chroma;
+  F.alpha_bits = hd.alpha_bits();
+  F.bit_depth = bd;
+  F.interlaced = h.interlace != 0;
+  // Output (src/dsp/mod.rs:88-96): scale 2^b/512, bias 2^(b-1) + 0.5, max 2^b - 1.
+  F.out_scale = (float)(1u << bd) / 512.0f;
+  F.out_bias = (float)(1u << (bd - 1)) + 0.5f;
+  F.out_max = (float)((1u << bd) - 1);
+  if (F.alpha_bits) {
+    // Rescale (src/decode.rs:446-469): (a * ceil(max_out 2^40 / max_in) + 2^39) >> 40.
+    const uint64_t max_in = (1ull << F.alpha_bits) - 1, max_out = (1ull << bd) - 1;
+    F.rescale_m = ((max_out << 40) + max_in - 1) / max_in;
+    F.rescale_identity = F.alpha_bits == bd;
+  }
+  hd.luma_weights(F.luma_w);
+  hd.chroma_weights(F.chroma_w);
+  const uint32_t frame_index = (uint32_t)b.frames.size();
+  b.frames.push_back(F);
+
+  fp.first_pic = (uint32_t)b.pics.size();
+  const uint32_t wmb = (h.width + 15u) / 16u, cpm = h.chroma == 2 ? 2u : 4u;
+  size_t pos = 8 + (size_t)h.header_size;
+  for (uint32_t index = 0; index < hd.picture_count(); index++) {
+    if (pos > h.frame_size) {
+      fp.deferred = invalid("a picture starts past the end of the frame");
+      break;
+    }
+    PicHeader ph;
+    Err e = parse_picture_header(host + pos, h.frame_size - pos, ph);
+    if (!e.ok()) {
+      fp.deferred = e;
+      break;
+    }
+    const uint32_t height = hd.picture_height(index), hmb = (height + 15u) / 16u;
+    const std::vector<uint32_t> sizes = slice_sizes(wmb, ph.log2_slice_mbs);
+    const uint64_t n = (uint64_t)sizes.size() * hmb;
+    if ((uint64_t)ph.header_size + 2 * n > ph.picture_size) {
+      fp.deferred = invalid("the slice table runs past picture_size");
+      break;
+    }
+    DecPic P;
+    memset(&P, 0, sizeof P);
+    P.pic_off = dev_packet + pos;
+    P.header_size = ph.header_size;
+    P.picture_size = ph.picture_size;
+    P.n_slices = (uint32_t)n;
+    P.row_slices = (uint32_t)sizes.size();
+    P.slice_base = (uint32_t)b.n_slices;
+    P.layout_base = (uint32_t)b.rows.size();
+    P.frame = frame_index;
+    P.height = height;
+    P.height_in_mb = hmb;
+    hd.picture_rows(index, P.first_row, P.row_step);
+    uint32_t mb_x = 0, prefix = 0;
+    for (uint32_t m : sizes) {
+      b.rows.push_back(RowSlot{(uint16_t)mb_x, (uint16_t)m, prefix});
+      mb_x += m;
+      prefix += m * (4 + 2 * cpm) * 64;
+    }
+    P.row_coefs = prefix;
+    P.coef_base = b.n_coefs;
+    b.n_coefs += (uint64_t)prefix * hmb;
+    b.n_slices += n;
+    b.pics.push_back(P);
+    fp.n_pics++;
+    pos += ph.picture_size;
+  }
+  b.plans.push_back(fp);
+}
+
+template <class T>
+Err upload(Device &d, DevBuf &buf, const std::vector<T> &v) {
+  Err e = buf.reserve(v.size() * sizeof(T) + 16);
+  if (!e.ok() || v.empty()) return e;
+  return check(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, d.stream),
+               "hipMemcpyAsync(plan)");
+}
+
+// Runs the three kernels over a planned batch and turns the device's error
+// records into one Result per frame (`results`, in frame order).
+Err run_batch(Device &d, Batch &b, std::vector<Err> &results) {
+  Workspace &ws = workspace(d);
+  Err e;
+  const uint64_t G = b.n_slices;
+  if (!(e = upload(d, ws.frames, b.frames)).ok()) return e;
+  if (!(e = upload(d, ws.pics, b.pics)).ok()) return e;
+  if (!(e = upload(d, ws.rows, b.rows)).ok()) return e;
+  const size_t np = b.pics.size();
+  if (!(e = ws.slices.reserve(G * sizeof(SliceRec) + 64)).ok()) return e;
+  if (!(e = ws.slice_err.reserve(G * 16 + 64)).ok()) return e;
+  if (!(e = ws.status.reserve(np * 4 + 64)).ok()) return e;
+  if (!(e = ws.first_err.reserve(np * 4 + 64)).ok()) return e;
+  if (!(e = ws.coefs.reserve(b.n_coefs * 2 + 64)).ok()) return e;
+  if (!(e = ws.wide.reserve(b.n_coefs * 4 + 64)).ok()) return e;
+  if (G) {
+    if (!(e = check(hipMemsetAsync(ws.slice_err.p, 0, G * 16, d.stream), "memset")).ok()) return e;
+  }
+  if (np) {
+    if (!(e = check(hipMemsetAsync(ws.first_err.p, 0xFF, np * 4, d.stream), "memset")).ok()) return e;
+  }
+  (void)hipEventRecord(d.ev[0], d.stream);
+  void *pics = ws.pics.p, *rows = ws.rows.p, *slices = ws.slices.p, *status = ws.status.p;
+  void *frames = ws.frames.p, *tables = d.tables, *coefs = ws.coefs.p, *wide = ws.wide.p;
+  void *serr = ws.slice_err.p, *ferr = ws.first_err.p;
+  uint32_t g32 = (uint32_t)G;
+  {
+    void *args[] = {&pics, &rows, &slices, &status};
+    if (!(e = launch(d, d.k_slice_table, (unsigned)np, 1, 256, args)).ok()) return e;
+  }
+  (void)hipEventRecord(d.ev[1], d.stream);
+  {
+    void *args[] = {&tables, &frames, &pics, &slices, &status, &g32, &coefs, &wide, &serr, &ferr};
+    if (!(e = launch(d, d.k_entropy, (unsigned)((G + 255) / 256), 4, 256, args)).ok()) return e;
+  }
+  (void)hipEventRecord(d.ev[2], d.stream);
+  {
+    void *args[] = {&tables, &frames, &pics, &slices, &status, &serr, &coefs, &wide};
+    if (!(e = launch(d, d.k_idct, (unsigned)G, 1, 256, args)).ok()) return e;
+  }
+  (void)hipEventRecord(d.ev[3], d.stream);
+  std::vector<uint32_t> st(np), fe(np);
+  if (np) {
+    if (!(e = check(hipMemcpyAsync(st.data(), ws.status.p, np * 4, hipMemcpyDeviceToHost, d.stream),
+                    "status")).ok())
+      return e;
+    if (!(e = check(hipMemcpyAsync(fe.data(), ws.first_err.p, np * 4, hipMemcpyDeviceToHost, d.stream),
+                    "first_err")).ok())
+      return e;
+  }
+  if (!(e = check(hipStreamSynchronize(d.stream), "hipStreamSynchronize")).ok()) return e;
+  float ms[3] = {0, 0, 0}, all = 0;
+  (void)hipEventElapsedTime(&ms[0], d.ev[0], d.ev[1]);
+  (void)hipEventElapsedTime(&ms[1], d.ev[1], d.ev[2]);
+  (void)hipEventElapsedTime(&ms[2], d.ev[2], d.ev[3]);
+  (void)hipEventElapsedTime(&all, d.ev[0], d.ev[3]);
+  for (int i = 0; i < 3; i++) g_last_us[i] = ms[i] * 1000.0;
+  g_last_us[3] = all * 1000.0;
+
+  results.assign(b.plans.size(), Err{});
+  for (size_t f = 0; f < b.plans.size(); f++) {
+    const FramePlan &fp = b.plans[f];
+    if (!fp.early.ok()) {
+      results[f] = fp.early;
+      continue;
+    }
+    Err r;
+    for (uint32_t p = fp.first_pic; p < fp.first_pic + fp.n_pics && r.ok(); p++) {
+      if (st[p] == PIC_TABLE_ERR) {
+        r = invalid("a slice runs past picture_size");
+      } else if (fe[p] != 0xFFFFFFFFu) {
+        uint32_t words[4];
+        const uint64_t g = b.pics[p].slice_base + (uint64_t)fe[p];
+        if (!(e = check(hipMemcpy(words, (uint8_t *)ws.slice_err.p + 16 * g, 16, hipMemcpyDeviceToHost),
+                        "slice_err")).ok())
+          return e;
+        for (int c = 0; c < 4; c++)
+          if (words[c]) {
+            r = slice_error(words[c]);
+            break;
+          }
+      }
+    }
+    if (r.ok()) r = fp.deferred;
+    results[f] = r;
+  }
+  return Err{};
+}
+
+}  // namespace
+}  // namespace prores_hip
+
+using namespace prores_hip;
+
+extern "C" {
+
+size_t prores_frame_layout(uint32_t width, uint32_t height, uint32_t chroma, prores_plane *p) {
+  // Frame::layout (src/frame.rs:247-266): chroma width ceil(w/2) for 4:2:2.
+  const uint32_t cw = chroma == PRORES_CHROMA_422 ? (width + 1) / 2 : width;
+  p[0] = prores_plane{0, width, height};
+  p[1] = prores_plane{(uint64_t)width * height, cw, height};
+  p[2] = prores_plane{p[1].offset + (uint64_t)cw * height, cw, height};
+  return (size_t)(p[2].offset + (uint64_t)cw * height);
+}
+
+prores_status prores_decoder_create(uint32_t bit_depth, uint32_t threads, prores_decoder **out,
+                                    prores_error *err) {
+  *out = nullptr;
+  if (bit_depth != 0 && (bit_depth < 8 || bit_depth > 16)) {
+    // Decoder::with_bit_depth (src/decode.rs:37-47)
+    fill_error(err, config_error(fmtstr("output bit depth %u is outside 8%s16", bit_depth, EN_DASH)));
+    return PRORES_ERR_CONFIG;
+  }
+  *out = new prores_decoder{bit_depth, threads};
+  fill_error(err, Err{});
+  return PRORES_OK;
+}
+
+void prores_decoder_destroy(prores_decoder *d) { delete d; }
+
+prores_status prores_frame_header_parse(const uint8_t *data, size_t len, prores_frame_header *out,
+                                        prores_error *err) {
+  Header h;
+  Err e = parse_frame_header(data, len, h);
+  if (e.ok()) *out = h.h;
+  fill_error(err, e);
+  return (prores_status)e.kind;
+}
+
+void prores_frame_free(prores_frame *f) {
+  if (!f) return;
+  free(f->data);
+  free(f->alpha);
+  f->data = nullptr;
+  f->alpha = nullptr;
+}
+
+static prores_status decode_common(const prores_decoder *d, const uint8_t *data, size_t len,
+                                   prores_frame *out, prores_error *err, bool allocate) {
+  Header hd;
+  Err e = parse_frame_header(data, len, hd);
+  if (e.ok()) e = check_plausible_size(hd);
+  if (!e.ok()) {
+    fill_error(err, e);
+    return (prores_status)e.kind;
+  }
+  const prores_frame_header &h = hd.h;
+  const uint32_t bd = d->bit_depth ? d->bit_depth : (h.chroma == 2 ? 10 : 12);
+  prores_plane planes[3];
+  const size_t total = prores_frame_layout(h.width, h.height, h.chroma, planes);
+  const size_t alpha_n = h.alpha ? (size_t)h.width * h.height : 0;
+  if (allocate) {
+    memset(out, 0, sizeof *out);
+    out->data = (uint16_t *)malloc(total * 2 + 2);
+    out->alpha = alpha_n ? (uint16_t *)malloc(alpha_n * 2 + 2) : nullptr;
+    if (!out->data || (alpha_n && !out->alpha)) {
+      prores_frame_free(out);
+      fill_error(err, Err{PRORES_ERR_DEVICE, "host allocation failed"});
+      return PRORES_ERR_DEVICE;
+    }
+  } else if (!out->data || out->data_len < total || (alpha_n && (!out->alpha || out->alpha_len < alpha_n))) {
+    fill_error(err, config_error("the output buffers are smaller than the frame"));
+    return PRORES_ERR_CONFIG;
+  }
+  out->width = h.width;
+  out->height = h.height;
+  out->bit_depth = bd;
+  out->chroma = h.chroma;
+  out->interlace = h.interlace;
+  out->metadata = h.metadata;
+  out->data_len = total;
+  out->alpha_len = alpha_n;
+  for (int i = 0; i < 3; i++) out->planes[i] = planes[i];
+
+  Device *dev = nullptr;
+  if (!(e = device(dev)).ok()) {
+    fill_error(err, e);
+    return (prores_status)e.kind;
+  }
+  std::lock_guard<std::mutex> g(dev->mu);
+  Workspace &ws = workspace(*dev);
+  const size_t fsz = h.frame_size;
+  const size_t plane_bytes = (total * 2 + 255) & ~(size_t)255;
+  if ((e = ws.packet.reserve(fsz + 128)).ok() && (e = ws.planes.reserve(plane_bytes + alpha_n * 2 + 64)).ok() &&
+      (e = check(hipMemcpyAsync(ws.packet.p, data, fsz, hipMemcpyHostToDevice, dev->stream), "H2D packet")).ok()) {
+    Batch b;
+    uint16_t *dd = ws.planes.as<uint16_t>();
+    uint16_t *da = alpha_n ? (uint16_t *)((uint8_t *)ws.planes.p + plane_bytes) : nullptr;
+    plan_frame(data, len, (uint64_t)ws.packet.p, d->bit_depth, dd, da, b, &hd);
+    std::vector<Err> results;
+    e = run_batch(*dev, b, results);
+    if (e.ok()) e = results[0];
+    if (e.ok()) {
+      e = check(hipMemcpy(out->data, dd, total * 2, hipMemcpyDeviceToHost), "D2H planes");
+      if (e.ok() && alpha_n) e = check(hipMemcpy(out->alpha, da, alpha_n * 2, hipMemcpyDeviceToHost), "D2H alpha");
+    }
+  }
+  if (!e.ok() && allocate) prores_frame_free(out);
+  fill_error(err, e);
+  return (prores_status)e.kind;
+}
+
+prores_status prores_decode(const prores_decoder *d, const uint8_t *data, size_t len, prores_frame *out,
+                            prores_error *err) {
+  return decode_common(d, data, len, out, err, true);
+}
+
+prores_status prores_decode_into(const prores_decoder *d, const uint8_t *data, size_t len,
+                                 prores_frame *out, prores_error *err) {
+  return decode_common(d, data, len, out, err, false);
+}
+
+prores_status prores_decode_device_batch(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,
+                                         prores_error *errs) {
+  Device *dev = nullptr;
+  Err e = device(dev);
+  if (!e.ok()) {
+    for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
+    return (prores_status)e.kind;
+  }
+  std::lock_guard<std::mutex> g(dev->mu);
+  Batch b;
+  for (size_t i = 0; i < n; i++)
+    plan_frame(jobs[i].host_packet, jobs[i].len, (uint64_t)jobs[i].dev_packet, d->bit_depth, jobs[i].dev_data,
+               jobs[i].dev_alpha, b, nullptr);
+  std::vector<Err> results;
+  e = run_batch(*dev, b, results);
+  if (!e.ok()) {
+    for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
+    return (prores_status)e.kind;
+  }
+  for (size_t i = 0; i < n; i++) fill_error(&errs[i], results[i]);
+  return PRORES_OK;
+}
+
+void prores_last_batch_timing(double *us4) {
+  for (int i = 0; i < 4; i++) us4[i] = g_last_us[i];
+}
+
+const char *prores_version(void) { return "prores-hip 0.1 gfx950"; }
+
+}  // extern "C"

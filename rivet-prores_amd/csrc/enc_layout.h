@@ -1,0 +1,55 @@
+// Encoder records shared by the host runtime and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+namespace prores_hip {
+
+constexpr uint32_t ENC_MAXK = 8;         // window columns of the size table
+constexpr uint32_t ALPHA_SLICE_CAP = 4608;  // bytes: 2048 values x 18 bits, the worst case
+
+struct EncFrame {
+  uint64_t in_plane[3];   // u16* Y', Cb, Cr (tightly packed)
+  uint64_t in_alpha;      // u16* or 0 (alpha coded only when alpha_bits != 0)
+  uint64_t out;           // device address of the frame's output bytes
+  uint32_t plane_w[3];
+  uint32_t width, height, chroma, bit_depth, interlaced;
+  uint32_t alpha_bits;    // 0, 8, 16: as coded
+  uint32_t rescale_identity, max_in;
+  uint64_t rescale_m;     // frame depth -> alpha bits
+  uint32_t first_pic, n_pics;
+  uint32_t hdr_len, hdr_off;  // frame header bytes in the header table
+  float in_scale;         // 512 / 2^bit_depth
+  uint32_t first_slice;   // global index of the frame's first slice
+};
+
+struct EncPic {
+  uint64_t budget;        // per-picture budget before alpha and overhead
+  uint64_t out_off;       // picture offset inside the frame (pe_offsets)
+  uint32_t frame, n_slices, slice_base, height;
+  uint32_t first_row, row_step, header_bytes, log2_slice_mbs;
+  uint32_t qtab;          // index of the 225 x 2 x 64 inverse-quantiser table
+  uint32_t interlaced;
+};
+
+struct EncSlice {
+  uint64_t coef;          // element offset of the slice's f32 coefficients
+  uint32_t pic;
+  uint16_t mb_x, mb_y;
+  uint32_t mbs, pad;
+};
+
+// The picture-level rate-control state (src/encode.rs:685-794).
+struct EncState {
+  unsigned long long sum;  // probe accumulator: total bytes at the probed qi
+  uint32_t notfit;         // probe accumulator: slices that do not fit
+  uint32_t alpha_total;    // alpha bytes of the picture
+  uint64_t budget;         // budget after alpha and overhead
+  uint32_t phase;          // 0: probing 224, 1: bisecting, 2: done
+  uint32_t probe;          // qi of the pending probe, 0 when none
+  uint32_t lo, hi;
+  uint32_t fits_at_all, uniform;
+  uint32_t window_k;       // columns computed: uniform, uniform-1, ...
+  uint32_t fallback;       // sizes the share-out computed itself (diagnostic)
+};
+
+}  // namespace prores_hip

@@ -1,0 +1,466 @@
+// Encoder host runtime: Encoder::encode (src/encode.rs:223-287) over the
+// gfx950 kernels. The host checks the configuration and the frame, writes
+// the frame header and lays out pictures and slices; the transform, the rate
+// control (probe passes, binary search, share-out) and the entropy coding
+// run on the device. One launch sequence covers a whole batch of frames.
+#include <stdlib.h>
+
+#include <map>
+
+#include "enc_layout.h"
+#include "host_internal.h"
+
+namespace prores_hip {
+namespace {
+
+const char *PROFILE_NAMES[6] = {"Proxy", "Lt", "Standard", "Hq", "P4444", "P4444Xq"};
+const uint64_t BITRATES[6] = {45000000, 102000000, 147000000, 220000000, 330000000, 500000000};
+const char *FOURCC[6] = {"apco", "apcs", "apcn", "apch", "ap4h", "ap4x"};
+const uint8_t SCAN_P[64] = {0,  1,  4,  5,  16, 17, 21, 22, 2,  3,  6,  7,  18, 20, 23, 28,
+                            8,  9,  12, 13, 19, 24, 27, 29, 10, 11, 14, 15, 25, 26, 30, 31,
+                            32, 33, 37, 38, 45, 46, 53, 54, 34, 36, 39, 44, 47, 52, 55, 60,
+                            35, 40, 43, 48, 51, 56, 59, 61, 41, 42, 49, 50, 57, 58, 62, 63};
+constexpr uint32_t WINDOW = 6;  // size-table columns below the uniform qi
+
+uint32_t profile_chroma(uint32_t p) { return p >= PRORES_PROFILE_4444 ? 3u : 2u; }
+const char *chroma_name(uint32_t c) { return c == 2 ? "Yuv422" : "Yuv444"; }
+
+uint64_t default_headers(uint32_t w, uint32_t h) {
+  // Frame header, picture header and per slice a table entry and a 6-byte
+  // slice header, at 8-macroblock slices (src/encode.rs:128-131).
+  const uint64_t slices = slice_sizes((w + 15) / 16, 3).size() * (uint64_t)((h + 15) / 16);
+  return 8 + 20 + 8 + 8 * slices;
+}
+
+uint64_t target_frame_bytes(uint32_t p, uint32_t w, uint32_t h) {
+  // The reference rate's bytes per macroblock at 1920x1080 times the frame's
+  // macroblocks, plus its headers; f64 as in src/encode.rs:112-117.
+  const double per_frame = (double)BITRATES[p] * 1001.0 / 30000.0 / 8.0;
+  const double per_mb = (per_frame - (double)default_headers(1920, 1080)) / (120.0 * 68.0);
+  const double t = per_mb * (double)((uint64_t)((w + 15) / 16) * ((h + 15) / 16));
+  return (uint64_t)(t < 0 ? 0.0 : t) + default_headers(w, h);
+}
+
+Err validate_config(const prores_config &c) {  // src/encode.rs:176-192
+  if (c.profile > 5) return config_error(fmtstr("profile %u is not a ProRes profile", c.profile));
+  if (c.log2_slice_mbs > 3)
+    return config_error(fmtstr("log2_slice_mbs %u is outside 0%s3", c.log2_slice_mbs, EN_DASH));
+  for (int m = 0; m < 2; m++) {
+    if (!(m == 0 ? c.has_luma_matrix : c.has_chroma_matrix)) continue;
+    const uint8_t *w = m == 0 ? c.luma_matrix : c.chroma_matrix;
+    for (int k = 0; k < 64; k++)
+      if (w[k] < 2 || w[k] > 63) return config_error(fmtstr("quantisation weights must be 2%s63", EN_DASH));
+  }
+  return Err{};
+}
+
+Err validate_frame(const prores_frame &f) {  // Frame::validate, src/frame.rs:299-340
+  if (f.width == 0 || f.height == 0 || f.width > 65535 || f.height > 65535)
+    return config_error(fmtstr("a frame of %u%s%u cannot be coded", f.width, TIMES, f.height));
+  if (f.bit_depth < 8 || f.bit_depth > 16)
+    return config_error(fmtstr("bit depth %u is outside 8%s16", f.bit_depth, EN_DASH));
+  if (f.chroma != 2 && f.chroma != 3) return config_error(fmtstr("chroma format %u is not 4:2:2 or 4:4:4", f.chroma));
+  prores_plane want[3];
+  prores_frame_layout(f.width, f.height, f.chroma, want);
+  for (int i = 0; i < 3; i++) {
+    const prores_plane &p = f.planes[i];
+    if (p.width != want[i].width || p.height != want[i].height)
+      return config_error(fmtstr("plane %d is %u%s%u; a %s frame of %u%s%u needs %u%s%u", i, p.width, TIMES,
+                                 p.height, chroma_name(f.chroma), f.width, TIMES, f.height, want[i].width,
+                                 TIMES, want[i].height));
+    const uint64_t end = p.offset + (uint64_t)p.width * p.height;
+    if (end < p.offset || end > f.data_len)
+      return config_error(fmtstr("plane %d runs past the end of the frame's data", i));
+  }
+  if (f.alpha && f.alpha_len != (size_t)f.width * f.height)
+    return config_error(fmtstr("the alpha plane has %zu samples, not %u%s%u", f.alpha_len, f.width, TIMES, f.height));
+  return Err{};
+}
+
+void write_frame_header(const Header &hd, std::vector<uint8_t> &out) {  // src/header.rs:174-199
+  const prores_frame_header &h = hd.h;
+  const uint32_t size = 20 + 64 * ((uint32_t)h.has_luma_matrix + h.has_chroma_matrix);
+  const uint8_t fixed[28] = {0, 0, 0, 0, 'i', 'c', 'p', 'f', (uint8_t)(size >> 8), (uint8_t)size, 0,
+                             h.bitstream_version, h.encoder_identifier[0], h.encoder_identifier[1],
+                             h.encoder_identifier[2], h.encoder_identifier[3], (uint8_t)(h.width >> 8),
+                             (uint8_t)h.width, (uint8_t)(h.height >> 8), (uint8_t)h.height,
+                             (uint8_t)((h.chroma << 6) | (h.interlace << 2)),
+                             (uint8_t)((h.metadata.aspect_ratio << 4) | (h.metadata.frame_rate_code & 15)),
+                             h.metadata.color_primaries, h.metadata.transfer_characteristic,
+                             h.metadata.matrix_coefficients, (uint8_t)h.alpha, 0,
+                             (uint8_t)((h.has_luma_matrix << 1) | h.has_chroma_matrix)};
+  out.insert(out.end(), fixed, fixed + 28);
+  if (h.has_luma_matrix) out.insert(out.end(), h.luma_matrix, h.luma_matrix + 64);
+  if (h.has_chroma_matrix) out.insert(out.end(), h.chroma_matrix, h.chroma_matrix + 64);
+}
+
+struct EncBatch {
+  std::vector<EncFrame> frames;
+  std::vector<EncPic> pics;
+  std::vector<EncSlice> slices;
+  std::vector<EncState> states;
+  std::vector<float> qtabs;
+  std::vector<uint8_t> hdrs;
+  std::map<std::vector<uint8_t>, uint32_t> qtab_index;
+  std::vector<int> job_frame;  // batch frame index per job, -1 when the job failed its checks
+  uint64_t n_coefs = 0;
+};
+
+uint32_t quantiser_table(EncBatch &b, const Header &hd) {
+  // 8 / (W qScale) at scanned positions for luma and chroma and every
+  // quantization_index (index 0 as 1), IEEE single division (src/encode.rs:303-324).
+  std::vector<uint8_t> key(129);
+  hd.luma_weights(key.data());
+  hd.chroma_weights(key.data() + 64);
+  key[128] = hd.h.interlace != 0;
+  auto it = b.qtab_index.find(key);
+  if (it != b.qtab_index.end()) return it->second;
+  const uint32_t index = (uint32_t)b.qtab_index.size();
+  b.qtab_index[key] = index;
+  size_t base = b.qtabs.size();
+  b.qtabs.resize(base + 225 * 128);
+  for (uint32_t qi = 0; qi <= 224; qi++) {
+    const uint32_t q = qscale(qi ? qi : 1);
+    for (int k = 0; k < 64; k++) {
+      const uint32_t s = key[128] ? SCAN_P[8 * (k % 8) + k / 8] : SCAN_P[k];
+      b.qtabs[base + qi * 128 + s] = 8.0f / (float)((uint32_t)key[k] * q);
+      b.qtabs[base + qi * 128 + 64 + s] = 8.0f / (float)((uint32_t)key[64 + k] * q);
+    }
+  }
+  return index;
+}
+
+// Checks one job and lays out its pictures and slices.
+Err plan_encode(const prores_config &cfg, const prores_frame &f, uint64_t dev_data, uint64_t dev_alpha,
+                EncBatch &b) {
+  Err e = validate_config(cfg);
+  if (e.ok()) e = validate_frame(f);
+  if (!e.ok()) return e;
+  const uint32_t pc = profile_chroma(cfg.profile);
+  if (f.chroma != pc)
+    return config_error(fmtstr("%s codes %s, and the frame is %s", PROFILE_NAMES[cfg.profile], chroma_name(pc),
+                               chroma_name(f.chroma)));
+  uint32_t alpha = PRORES_ALPHA_NONE;
+  if (f.alpha) {
+    if (pc == 2) return config_error(fmtstr("%s carries no alpha channel", PROFILE_NAMES[cfg.profile]));
+    alpha = cfg.alpha;
+  }
+  Header hd;
+  memset(&hd.h, 0, sizeof hd.h);
+  prores_frame_header &h = hd.h;
+  h.bitstream_version = (f.chroma == 2 && alpha == 0) ? 0 : 1;
+  memcpy(h.encoder_identifier, cfg.encoder_identifier, 4);
+  h.width = (uint16_t)f.width;
+  h.height = (uint16_t)f.height;
+  h.chroma = f.chroma;
+  h.interlace = f.interlace;
+  h.metadata = f.metadata;
+  h.alpha = alpha;
+  h.has_luma_matrix = cfg.has_luma_matrix;
+  h.has_chroma_matrix = cfg.has_chroma_matrix;
+  memcpy(h.luma_matrix, cfg.luma_matrix, 64);
+  memcpy(h.chroma_matrix, cfg.chroma_matrix, 64);
+
+  EncFrame F;
+  memset(&F, 0, sizeof F);
+  for (int i = 0; i < 3; i++) {
+    F.in_plane[i] = dev_data + 2 * f.planes[i].offset;
+    F.plane_w[i] = f.planes[i].width;
+  }
+  F.in_alpha = alpha ? dev_alpha : 0;
+  F.width = f.width;
+  F.height = f.height;
+  F.chroma = f.chroma;
+  F.bit_depth = f.bit_depth;
+  F.interlaced = f.interlace != 0;
+  F.alpha_bits = hd.alpha_bits();
+  if (F.alpha_bits) {
+    const uint64_t max_in = (1ull << f.bit_depth) - 1, max_out = (1ull << F.alpha_bits) - 1;
+    F.rescale_m = ((max_out << 40) + max_in - 1) / max_in;
+    F.rescale_identity = f.bit_depth == F.alpha_bits;
+  }
+  F.max_in = 0xFFFFu >> (16 - f.bit_depth);
+  F.in_scale = 512.0f / (float)(1u << f.bit_depth);
+  F.hdr_off = (uint32_t)b.hdrs.size();
+  write_frame_header(hd, b.hdrs);
+  F.hdr_len = (uint32_t)(b.hdrs.size() - F.hdr_off);
+  const uint64_t target = cfg.target_frame_bytes ? cfg.target_frame_bytes : target_frame_bytes(cfg.profile, f.width, f.height);
+  const uint32_t npics = hd.picture_count();
+  const uint64_t per_picture = (target > F.hdr_len ? target - F.hdr_len : 0) / npics;
+  const uint32_t qtab = quantiser_table(b, hd);
+  F.first_pic = (uint32_t)b.pics.size();
+  F.n_pics = npics;
+  F.first_slice = (uint32_t)b.slices.size();
+  const uint32_t fi = (uint32_t)b.frames.size();
+  const uint32_t wmb = (f.width + 15) / 16, cpm = f.chroma == 2 ? 2u : 4u;
+  const std::vector<uint32_t> sizes = slice_sizes(wmb, cfg.log2_slice_mbs);
+  for (uint32_t index = 0; index < npics; index++) {
+    EncPic P;
+    memset(&P, 0, sizeof P);
+    P.budget = per_picture;
+    P.frame = fi;
+    P.height = hd.picture_height(index);
+    hd.picture_rows(index, P.first_row, P.row_step);
+    P.header_bytes = alpha ? 8 : 6;
+    P.log2_slice_mbs = cfg.log2_slice_mbs;
+    P.qtab = qtab;
+    P.interlaced = F.interlaced;
+    P.slice_base = (uint32_t)b.slices.size();
+    const uint32_t hmb = (P.height + 15) / 16;
+    const uint32_t pic = (uint32_t)b.pics.size();
+    for (uint32_t my = 0; my < hmb; my++) {
+      uint32_t mx = 0;
+      for (uint32_t m : sizes) {
+        EncSlice s;
+        memset(&s, 0, sizeof s);
+        s.coef = b.n_coefs;
+        s.pic = pic;
+        s.mb_x = (uint16_t)mx;
+        s.mb_y = (uint16_t)my;
+        s.mbs = m;
+        b.slices.push_back(s);
+        b.n_coefs += 64ull * (4 + 2 * cpm) * m;
+        mx += m;
+      }
+    }
+    P.n_slices = (uint32_t)(b.slices.size() - P.slice_base);
+    b.pics.push_back(P);
+    EncState st;
+    memset(&st, 0, sizeof st);
+    st.probe = 224;
+    b.states.push_back(st);
+  }
+  b.frames.push_back(F);
+  return Err{};
+}
+
+struct EncWorkspace {
+  DevBuf frames, pics, slices, states, qtabs, hdrs, coefs, alpha_scratch, alpha_len, size_tab, used, qidx,
+      slice_out, frame_size, out, planes, alpha_in;
+};
+EncWorkspace &enc_workspace(Device &d) {
+  static EncWorkspace ws[64];
+  return ws[d.id];
+}
+
+double g_enc_us[4] = {0, 0, 0, 0};
+
+template <class T>
+Err upload(Device &d, DevBuf &buf, const std::vector<T> &v) {
+  Err e = buf.reserve(v.size() * sizeof(T) + 16);
+  if (!e.ok() || v.empty()) return e;
+  return check(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, d.stream),
+               "hipMemcpyAsync(encode plan)");
+}
+
+// Runs the kernels over a planned batch; on return the packets are in the
+// output arena at `offsets[i]` with `sizes[i]` bytes (batch frame order).
+Err run_encode(Device &d, EncBatch &b, std::vector<uint64_t> &offsets, std::vector<uint64_t> &sizes) {
+  EncWorkspace &ws = enc_workspace(d);
+  Err e;
+  const uint64_t G = b.slices.size();
+  const uint32_t NP = (uint32_t)b.pics.size(), NF = (uint32_t)b.frames.size();
+  if (!(e = upload(d, ws.frames, b.frames)).ok()) return e;
+  if (!(e = upload(d, ws.pics, b.pics)).ok()) return e;
+  if (!(e = upload(d, ws.slices, b.slices)).ok()) return e;
+  if (!(e = upload(d, ws.states, b.states)).ok()) return e;
+  if (!(e = upload(d, ws.qtabs, b.qtabs)).ok()) return e;
+  if (!(e = upload(d, ws.hdrs, b.hdrs)).ok()) return e;
+  if (!(e = ws.coefs.reserve(b.n_coefs * 4 + 64)).ok()) return e;
+  if (!(e = ws.alpha_scratch.reserve(G * ALPHA_SLICE_CAP + 64)).ok()) return e;
+  if (!(e = ws.alpha_len.reserve(G * 4 + 64)).ok()) return e;
+  if (!(e = ws.size_tab.reserve(G * ENC_MAXK * 4 + 64)).ok()) return e;
+  if (!(e = ws.used.reserve(G * 4 + 64)).ok()) return e;
+  if (!(e = ws.qidx.reserve(G * 4 + 64)).ok()) return e;
+  if (!(e = ws.slice_out.reserve(G * 8 + 64)).ok()) return e;
+  if (!(e = ws.frame_size.reserve(NF * 8 + 64)).ok()) return e;
+  void *frames = ws.frames.p, *pics = ws.pics.p, *slices = ws.slices.p, *states = ws.states.p;
+  void *qtabs = ws.qtabs.p, *hdrs = ws.hdrs.p, *coefs = ws.coefs.p, *ascr = ws.alpha_scratch.p;
+  void *alen = ws.alpha_len.p, *stab = ws.size_tab.p, *used = ws.used.p, *qidx = ws.qidx.p;
+  void *sout = ws.slice_out.p, *fsize = ws.frame_size.p, *tables = d.tables;
+  uint32_t g32 = (uint32_t)G, np32 = NP, window = WINDOW;
+  (void)hipEventRecord(d.ev[0], d.stream);
+  {
+    void *args[] = {&tables, &frames, &pics, &slices, &coefs};
+    if (!(e = launch(d, d.k_fdct, g32, 1, 256, args)).ok()) return e;
+  }
+  {
+    void *args[] = {&frames, &pics, &slices, &g32, &ascr, &alen, &states};
+    if (!(e = launch(d, d.k_alpha_enc, (g32 + 255) / 256, 1, 256, args)).ok()) return e;
+  }
+  (void)hipEventRecord(d.ev[1], d.stream);
+  // qi = 224, then at most 8 bisection steps (224 < 2^8).
+  for (int probe = 0; probe < 9; probe++) {
+    void *a1[] = {&frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states};
+    if (!(e = launch(d, d.k_size, (g32 + 3) / 4, 1, 256, a1)).ok()) return e;
+    void *a2[] = {&pics, &states, &np32, &window};
+    if (!(e = launch(d, d.k_search, (np32 + 63) / 64, 1, 64, a2)).ok()) return e;
+  }
+  {
+    void *args[] = {&frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &stab};
+    if (!(e = launch(d, d.k_window, (g32 + 3) / 4, 1, 256, args)).ok()) return e;
+  }
+  {
+    void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &states, &stab, &used, &qidx};
+    if (!(e = launch(d, d.k_shareout, np32, 1, 64, args)).ok()) return e;
+  }
+  {
+    void *args[] = {&frames, &pics, &used, &sout, &fsize};
+    if (!(e = launch(d, d.k_offsets, NF, 1, 256, args)).ok()) return e;
+  }
+  (void)hipEventRecord(d.ev[2], d.stream);
+  sizes.assign(NF, 0);
+  if (NF && !(e = check(hipMemcpyAsync(sizes.data(), ws.frame_size.p, NF * 8, hipMemcpyDeviceToHost, d.stream),
+                        "frame sizes")).ok())
+    return e;
+  if (!(e = check(hipStreamSynchronize(d.stream), "hipStreamSynchronize")).ok()) return e;
+  offsets.assign(NF, 0);
+  uint64_t total = 0;
+  for (uint32_t f = 0; f < NF; f++) {
+    if (sizes[f] > 0xFFFFFFFFull) return invalid("an encoded frame exceeds 4 GiB");
+    offsets[f] = total;
+    total += (sizes[f] + 255) & ~255ull;
+  }
+  if (!(e = ws.out.reserve(total + 256)).ok()) return e;
+  for (uint32_t f = 0; f < NF; f++) b.frames[f].out = (uint64_t)ws.out.p + offsets[f];
+  if (!(e = upload(d, ws.frames, b.frames)).ok()) return e;
+  (void)hipEventRecord(d.ev[3], d.stream);
+  {
+    void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &ascr, &used, &qidx, &sout, &fsize, &hdrs};
+    if (!(e = launch(d, d.k_pack, g32, 1, 64, args)).ok()) return e;
+  }
+  hipEvent_t done;
+  (void)hipEventCreate(&done);
+  (void)hipEventRecord(done, d.stream);
+  if (!(e = check(hipStreamSynchronize(d.stream), "hipStreamSynchronize")).ok()) return e;
+  float a = 0, r = 0, p = 0, all = 0;
+  (void)hipEventElapsedTime(&a, d.ev[0], d.ev[1]);
+  (void)hipEventElapsedTime(&r, d.ev[1], d.ev[2]);
+  (void)hipEventElapsedTime(&p, d.ev[3], done);
+  (void)hipEventElapsedTime(&all, d.ev[0], done);
+  (void)hipEventDestroy(done);
+  g_enc_us[0] = a * 1000.0;
+  g_enc_us[1] = r * 1000.0;
+  g_enc_us[2] = p * 1000.0;
+  g_enc_us[3] = all * 1000.0;
+  return Err{};
+}
+
+}  // namespace
+}  // namespace prores_hip
+
+using namespace prores_hip;
+
+extern "C" {
+
+void prores_config_default(uint32_t profile, prores_config *c) {  // Config::new, src/encode.rs:163-174
+  memset(c, 0, sizeof *c);
+  c->profile = profile;
+  c->log2_slice_mbs = 3;
+  c->alpha = PRORES_ALPHA_16;
+  memcpy(c->encoder_identifier, "rivt", 4);
+}
+
+uint64_t prores_profile_target_frame_bytes(uint32_t profile, uint32_t width, uint32_t height) {
+  return profile > 5 ? 0 : target_frame_bytes(profile, width, height);
+}
+
+void prores_profile_fourcc(uint32_t profile, char out[4]) {
+  memcpy(out, FOURCC[profile > 5 ? 0 : profile], 4);
+}
+
+int32_t prores_profile_from_fourcc(const char code[4]) {
+  for (int p = 0; p < 6; p++)
+    if (memcmp(code, FOURCC[p], 4) == 0) return p;
+  return -1;
+}
+
+void prores_buffer_free(uint8_t *p) { free(p); }
+
+prores_status prores_encode(const prores_config *cfg, const prores_frame *frame, uint8_t **out, size_t *out_len,
+                            prores_error *err) {
+  *out = nullptr;
+  *out_len = 0;
+  Device *dev = nullptr;
+  Err e = device(dev);
+  if (!e.ok()) {
+    fill_error(err, e);
+    return (prores_status)e.kind;
+  }
+  std::lock_guard<std::mutex> g(dev->mu);
+  if (!(e = ensure_encoder(*dev)).ok()) {
+    fill_error(err, e);
+    return (prores_status)e.kind;
+  }
+  EncWorkspace &ws = enc_workspace(*dev);
+  // Checks first, so that a refused frame costs no copies.
+  EncBatch probe;
+  e = plan_encode(*cfg, *frame, 0, 0, probe);
+  if (!e.ok()) {
+    fill_error(err, e);
+    return (prores_status)e.kind;
+  }
+  const size_t nsamp = frame->data_len, nalpha = frame->alpha ? frame->alpha_len : 0;
+  if ((e = ws.planes.reserve(nsamp * 2 + 64)).ok() && (e = ws.alpha_in.reserve(nalpha * 2 + 64)).ok() &&
+      (e = check(hipMemcpyAsync(ws.planes.p, frame->data, nsamp * 2, hipMemcpyHostToDevice, dev->stream), "H2D frame")).ok() &&
+      (!nalpha || (e = check(hipMemcpyAsync(ws.alpha_in.p, frame->alpha, nalpha * 2, hipMemcpyHostToDevice, dev->stream),
+                             "H2D alpha")).ok())) {
+    EncBatch b;
+    e = plan_encode(*cfg, *frame, (uint64_t)ws.planes.p, (uint64_t)ws.alpha_in.p, b);
+    std::vector<uint64_t> offsets, sizes;
+    if (e.ok()) e = run_encode(*dev, b, offsets, sizes);
+    if (e.ok()) {
+      *out = (uint8_t *)malloc(sizes[0] ? sizes[0] : 1);
+      if (!*out) e = Err{PRORES_ERR_DEVICE, "host allocation failed"};
+    }
+    if (e.ok()) e = check(hipMemcpy(*out, (uint8_t *)ws.out.p + offsets[0], sizes[0], hipMemcpyDeviceToHost), "D2H packet");
+    if (e.ok()) {
+      *out_len = sizes[0];
+    } else {
+      free(*out);
+      *out = nullptr;
+    }
+  }
+  fill_error(err, e);
+  return (prores_status)e.kind;
+}
+
+prores_status prores_encode_device_batch(size_t n, prores_dev_encode_job *jobs, prores_error *errs) {
+  Device *dev = nullptr;
+  Err e = device(dev);
+  if (e.ok()) {
+    std::lock_guard<std::mutex> g(dev->mu);
+    e = ensure_encoder(*dev);
+    if (e.ok()) {
+      EncBatch b;
+      std::vector<int> frame_of(n, -1);
+      for (size_t i = 0; i < n; i++) {
+        jobs[i].dev_out = nullptr;
+        jobs[i].out_len = 0;
+        const uint32_t before = (uint32_t)b.frames.size();
+        Err je = plan_encode(*jobs[i].cfg, jobs[i].frame, (uint64_t)jobs[i].frame.data, (uint64_t)jobs[i].frame.alpha, b);
+        fill_error(&errs[i], je);
+        if (je.ok()) frame_of[i] = (int)before;
+      }
+      std::vector<uint64_t> offsets, sizes;
+      if (!b.frames.empty()) e = run_encode(*dev, b, offsets, sizes);
+      if (e.ok()) {
+        EncWorkspace &ws = enc_workspace(*dev);
+        for (size_t i = 0; i < n; i++)
+          if (frame_of[i] >= 0) {
+            jobs[i].dev_out = (const uint8_t *)ws.out.p + offsets[frame_of[i]];
+            jobs[i].out_len = sizes[frame_of[i]];
+          }
+        return PRORES_OK;
+      }
+    }
+  }
+  for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
+  return (prores_status)e.kind;
+}
+
+void prores_last_encode_timing(double *us4) {
+  for (int i = 0; i < 4; i++) us4[i] = g_enc_us[i];
+}
+
+}  // extern "C"

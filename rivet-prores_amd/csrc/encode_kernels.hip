@@ -1,0 +1,861 @@
+// gfx950 encode kernels for the ProRes slice engine (src/encode.rs).
+//
+//   pe_fdct      one workgroup per slice: edge-replicated 8x8 gathers, the
+//                f32 forward DCT in the reference's order (src/dct.rs:70-93,
+//                src/dsp/scalar.rs:14-17), scattered to scanned-interleaved
+//                order per component (src/encode.rs:518-555).
+//   pe_alpha     one lane per slice: alpha rescaled and run/difference coded
+//                (src/encode.rs:838-868, 457-470).
+//   pe_size      one wavefront per slice: bytes at the picture's pending probe
+//                qi (quantise + exact bit count, src/encode.rs:671-681,
+//                398-409), summed per picture.
+//   pe_search    one lane per picture: one step of the binary search over qi
+//                (src/encode.rs:685-713).
+//   pe_window    one wavefront per slice: sizes at uniform, uniform-1, ... in
+//                one pass over the coefficients.
+//   pe_shareout  one wavefront per picture: the sequential slice-by-slice
+//                share-out (src/encode.rs:769-793) solved 64 slices at a time
+//                by fixed-point iteration of its triangular recurrence.
+//   pe_offsets   one workgroup per frame: slice and picture offsets.
+//   pe_pack      one wavefront per slice: quantise at the chosen qi and place
+//                every codeword with a wave prefix sum over their lengths.
+//
+// Built with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "enc_layout.h"
+#include "layout.h"
+
+using namespace prores_hip;
+
+namespace {
+
+__device__ __forceinline__ uint32_t clz64(uint64_t v) { return v ? __builtin_clzll(v) : 64; }
+__device__ __forceinline__ uint32_t bitlen64(uint64_t v) { return 64 - clz64(v); }
+
+// Codebooks as combination codes (lastRiceQ, kRice, kExp), packed 4 bits per
+// context as lastRiceQ | kRice << 2; kExp is kRice + 1 except for level
+// context 0, RICE_EXP_COMBO(2, 0, 2). Tables 9-11 (src/vlc.rs:30-68).
+constexpr uint64_t RUN_P = 0x8444444555501122ull;
+constexpr uint64_t LVL_P = 0x844440212ull;
+constexpr uint32_t DC_P = 0xC940u;
+
+struct Book {
+  uint32_t lq, kr, ke;
+};
+__device__ __forceinline__ Book run_book(uint32_t ctx) {
+  uint32_t nib = (uint32_t)(RUN_P >> (4 * ctx)) & 15;
+  return Book{nib & 3, nib >> 2, (nib >> 2) + 1};
+}
+__device__ __forceinline__ Book lvl_book(uint32_t ctx) {
+  uint32_t nib = (uint32_t)(LVL_P >> (4 * ctx)) & 15;
+  return Book{nib & 3, nib >> 2, (nib >> 2) + 1 + (ctx == 0 ? 1u : 0u)};
+}
+__device__ __forceinline__ Book dc_book(uint32_t ctx) {
+  uint32_t nib = (DC_P >> (4 * ctx)) & 15;
+  return Book{nib & 3, nib >> 2, (nib >> 2) + 1};
+}
+constexpr Book FIRST_DC = {0, 5, 6};  // EG(5)
+
+// Codebook::len (src/vlc.rs:155-172).
+__device__ __forceinline__ uint32_t book_len(Book b, uint32_t n) {
+  uint32_t th = (b.lq + 1) << b.kr;
+  if (n < th) return (n >> b.kr) + 1 + b.kr;
+  uint64_t v = (uint64_t)(n - th) + (1ull << b.ke);
+  return b.lq + 1 + 2 * bitlen64(v) - b.ke - 1;
+}
+// Codebook::code (src/vlc.rs:131-151): the low `len` bits are the codeword.
+__device__ __forceinline__ uint64_t book_code(Book b, uint32_t n, uint32_t &len) {
+  uint32_t th = (b.lq + 1) << b.kr;
+  if (n < th) {
+    len = (n >> b.kr) + 1 + b.kr;
+    return (1ull << b.kr) | (n & ((1u << b.kr) - 1));
+  }
+  uint64_t v = (uint64_t)(n - th) + (1ull << b.ke);
+  len = b.lq + 1 + 2 * bitlen64(v) - b.ke - 1;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t signed_to_symbol(int32_t n) {
+  return n >= 0 ? 2u * (uint32_t)n : 2u * (0u - (uint32_t)n) - 1u;
+}
+
+// The scalar quantiser (src/dsp/scalar.rs:19-39): QF = sign(x) floor(|x| + r),
+// x = F inv, with a saturating float-to-int conversion.
+__device__ __forceinline__ int32_t quant(float f, float inv, float r) {
+  float x = f * inv;
+  float t = floorf(fabsf(x) + r);
+  int32_t m = t >= 2147483648.0f ? 0x7FFFFFFF : (int32_t)t;
+  return x < 0.0f ? -m : m;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t lo = __shfl_xor((uint32_t)v, o, 64), hi = __shfl_xor((uint32_t)(v >> 32), o, 64);
+    v += ((uint64_t)hi << 32) | lo;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t excl_scan(uint32_t v, uint32_t lane) {
+  uint32_t incl = v;
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  return incl - v;
+}
+__device__ __forceinline__ int64_t excl_scan64(int64_t v, uint32_t lane) {
+  int64_t incl = v;
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    uint32_t lo = __shfl_up((uint32_t)(uint64_t)incl, d, 64);
+    uint32_t hi = __shfl_up((uint32_t)((uint64_t)incl >> 32), d, 64);
+    if (lane >= d) incl += (int64_t)(((uint64_t)hi << 32) | lo);
+  }
+  return incl - v;
+}
+
+__device__ __forceinline__ uint32_t ilog2(uint32_t v) { return 31 - __builtin_clz(v); }
+
+// Component c of a slice: its block count and its offset inside the slice's
+// coefficients (Y', then Cb, then Cr; src/encode.rs:557-564).
+__device__ __forceinline__ void component(uint32_t c, uint32_t mbs, uint32_t cpm, uint32_t &nb,
+                                          uint32_t &off) {
+  nb = (c == 0 ? 4u : cpm) * mbs;
+  off = c == 0 ? 0u : (c == 1 ? 256u * mbs : (256u + 64u * cpm) * mbs);
+}
+
+// The bits one lane's coefficient contributes in a 64-coefficient chunk of a
+// component's scanned array, and the chunk-to-chunk state of the run/level
+// contexts (src/encode.rs:366-409).
+struct Ctx {
+  uint32_t idx, run, lvl;  // last nonzero AC index, its run and level symbol
+};
+
+// Length of lane `L`'s codewords in the chunk starting at `base`.
+__device__ __forceinline__ uint32_t chunk_len(int32_t q, uint32_t i, uint32_t base, uint32_t L,
+                                              uint32_t nb, Ctx &cx) {
+  uint32_t len = 0;
+  if (base == 0) {  // the DC coefficients (src/encode.rs:383-395)
+    int32_t d1 = __shfl(q, L ? L - 1 : 0, 64);
+    int32_t d2 = __shfl(q, L > 1 ? L - 2 : 0, 64);
+    if (L < nb) {
+      if (L == 0) {
+        len = book_len(FIRST_DC, signed_to_symbol(q));
+      } else {
+        int32_t diff = (int32_t)((uint32_t)q - (uint32_t)d1);
+        int32_t pd = L == 1 ? 3 : (int32_t)((uint32_t)d1 - (uint32_t)d2);
+        int32_t n = pd < 0 ? (int32_t)(0u - (uint32_t)diff) : diff;
+        uint32_t a = pd < 0 ? 0u - (uint32_t)pd : (uint32_t)pd;
+        len = book_len(dc_book(a < 3 ? a : 3), signed_to_symbol(n));
+      }
+    }
+  }
+  const bool nz = i >= nb && q != 0;
+  const uint64_t mask = __ballot(nz);
+  if (mask) {
+    const uint64_t lower = mask & ((1ull << L) - 1);
+    const int P = lower ? 63 - (int)__builtin_clzll(lower) : -1;
+    const uint32_t prev_idx = P >= 0 ? base + (uint32_t)P : cx.idx;
+    const uint32_t run = i - prev_idx - 1;
+    const uint32_t lvl = (q < 0 ? 0u - (uint32_t)q : (uint32_t)q) - 1;
+    uint32_t prun = __shfl(run, P >= 0 ? P : (int)L, 64);
+    uint32_t plvl = __shfl(lvl, P >= 0 ? P : (int)L, 64);
+    if (P < 0) {
+      prun = cx.run;
+      plvl = cx.lvl;
+    }
+    if (nz)
+      len += book_len(run_book(prun < 15 ? prun : 15), run) + book_len(lvl_book(plvl < 8 ? plvl : 8), lvl) + 1;
+    const int last = 63 - (int)__builtin_clzll(mask);
+    cx.idx = base + (uint32_t)last;
+    cx.run = __shfl(run, last, 64);
+    cx.lvl = __shfl(lvl, last, 64);
+  }
+  return len;
+}
+
+// Sizes of one slice at K quantisers (K <= MAXK), one wavefront: total bytes
+// and whether every size field can hold them (src/encode.rs:671-681).
+template <int MAXK>
+__device__ void slice_sizes(const float *__restrict__ coef, uint32_t mbs, uint32_t cpm,
+                            const float *__restrict__ qt, const uint32_t *qis, uint32_t K,
+                            uint32_t hb, uint32_t alen, uint32_t *size, bool *fits) {
+  const uint32_t L = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < MAXK; k++) {
+    size[k] = hb + alen;
+    fits[k] = true;
+  }
+  for (uint32_t c = 0; c < 3; c++) {
+    uint32_t nb, off;
+    component(c, mbs, cpm, nb, off);
+    const uint32_t lg = ilog2(nb);
+    const float *cf = coef + off;
+    uint32_t lanebits[MAXK];
+    Ctx cx[MAXK];
+    const float *inv[MAXK];
+#pragma unroll
+    for (int k = 0; k < MAXK; k++) {
+      lanebits[k] = 0;
+      cx[k] = Ctx{nb - 1, 4, 1};
+      inv[k] = qt + ((k < (int)K ? qis[k] : qis[0]) * 2 + (c > 0 ? 1 : 0)) * 64;
+    }
+    for (uint32_t base = 0; base < nb * 64; base += 64) {
+      const uint32_t i = base + L;
+      const float f = cf[i];
+      const uint32_t s = i >> lg;
+      const float r = s == 0 ? 0.5f : 0.42f;
+#pragma unroll
+      for (int k = 0; k < MAXK; k++) {
+        if (k < (int)K) {
+          const int32_t q = quant(f, inv[k][s], r);
+          lanebits[k] += chunk_len(q, i, base, L, nb, cx[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < MAXK; k++) {
+      if (k < (int)K) {
+        const uint32_t bytes = (wave_sum(lanebits[k]) + 7) / 8;
+        size[k] += bytes;
+        if (bytes > 65535) fits[k] = false;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < MAXK; k++)
+    if (size[k] > 65535) fits[k] = false;
+}
+
+// One lane's exact size of a slice at `qi`, following coefficient_bits
+// sequentially (src/encode.rs:398-409); for the share-out's rare steps past
+// the window.
+__device__ void slice_size_serial(const float *__restrict__ coef, uint32_t mbs, uint32_t cpm,
+                                  const float *__restrict__ qt, uint32_t qi, uint32_t hb,
+                                  uint32_t alen, uint32_t &size, bool &fits) {
+  size = hb + alen;
+  fits = true;
+  for (uint32_t c = 0; c < 3; c++) {
+    uint32_t nb, off;
+    component(c, mbs, cpm, nb, off);
+    const uint32_t lg = ilog2(nb);
+    const float *cf = coef + off;
+    const float *inv = qt + (qi * 2 + (c > 0 ? 1 : 0)) * 64;
+    uint32_t bits = 0;
+    int32_t prev_dc = quant(cf[0], inv[0], 0.5f), prev_diff = 3;
+    bits += book_len(FIRST_DC, signed_to_symbol(prev_dc));
+    for (uint32_t i = 1; i < nb; i++) {
+      int32_t dc = quant(cf[i], inv[0], 0.5f);
+      int32_t diff = (int32_t)((uint32_t)dc - (uint32_t)prev_dc);
+      int32_t n = prev_diff < 0 ? (int32_t)(0u - (uint32_t)diff) : diff;
+      uint32_t a = prev_diff < 0 ? 0u - (uint32_t)prev_diff : (uint32_t)prev_diff;
+      bits += book_len(dc_book(a < 3 ? a : 3), signed_to_symbol(n));
+      prev_diff = diff;
+      prev_dc = dc;
+    }
+    uint32_t prun = 4, plvl = 1, pos = nb;
+    for (uint32_t i = nb; i < nb * 64; i++) {
+      int32_t q = quant(cf[i], inv[i >> lg], 0.42f);
+      if (!q) continue;
+      uint32_t run = i - pos, lvl = (q < 0 ? 0u - (uint32_t)q : (uint32_t)q) - 1;
+      bits += book_len(run_book(prun < 15 ? prun : 15), run) + book_len(lvl_book(plvl < 8 ? plvl : 8), lvl) + 1;
+      prun = run;
+      plvl = lvl;
+      pos = i + 1;
+    }
+    uint32_t bytes = (bits + 7) / 8;
+    size += bytes;
+    if (bytes > 65535) fits = false;
+  }
+  if (size > 65535) fits = false;
+}
+
+__device__ __forceinline__ uint16_t rescale(uint32_t a, uint32_t identity, uint64_t m) {
+  return identity ? (uint16_t)a : (uint16_t)(((uint64_t)a * m + (1ull << 39)) >> 40);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(256)
+    pe_fdct(const DevTables *__restrict__ T, const EncFrame *__restrict__ frames,
+            const EncPic *__restrict__ pics, const EncSlice *__restrict__ slices,
+            float *__restrict__ coefs) {
+  __shared__ float outc[96 * 64];
+  __shared__ float tmp[32 * 64];
+  const uint32_t g = blockIdx.x, tid = threadIdx.x;
+  const EncSlice S = slices[g];
+  const EncPic &P = pics[S.pic];
+  const EncFrame &F = frames[P.frame];
+  const uint32_t chroma = F.chroma, cpm = chroma == 2 ? 2u : 4u, mbs = S.mbs;
+  const uint32_t nbY = 4 * mbs, nbC = cpm * mbs, TB = nbY + 2 * nbC;
+  const uint8_t *scan = T->scan[P.interlaced ? 1 : 0];
+  const float *basis = T->basis;
+  const float scale = F.in_scale;
+  const uint32_t r = tid & 7, lb = tid >> 3;
+  for (uint32_t t0 = 0; t0 < TB; t0 += 32) {
+    const uint32_t t = t0 + lb;
+    const bool active = t < TB;
+    uint32_t c = 0, b = t, nb = nbY, cbase = 0;
+    if (t >= nbY + nbC) {
+      c = 2; b = t - nbY - nbC; nb = nbC; cbase = 64 * (nbY + nbC);
+    } else if (t >= nbY) {
+      c = 1; b = t - nbY; nb = nbC; cbase = 64 * nbY;
+    }
+    if (active) {
+      // block_position (src/decode.rs:395-413), PictureLayout::block (src/encode.rs:488-505).
+      const uint32_t per_mb = c == 0 ? 4u : cpm;
+      const uint32_t mbx = S.mb_x + b / per_mb, bb = b % per_mb;
+      uint32_t bx, by;
+      if (c == 0) {
+        bx = 16 * mbx + 8 * (bb & 1); by = 16u * S.mb_y + 8 * (bb >> 1);
+      } else if (chroma == 2) {
+        bx = 8 * mbx; by = 16u * S.mb_y + 8 * bb;
+      } else {
+        bx = 16 * mbx + 8 * (bb >> 1); by = 16u * S.mb_y + 8 * (bb & 1);
+      }
+      const uint32_t pw = F.plane_w[c];
+      const uint32_t prow = by + r < P.height - 1 ? by + r : P.height - 1;
+      const uint16_t *src = (const uint16_t *)F.in_plane[c] + (size_t)(P.first_row + P.row_step * prow) * pw;
+      float p[8];
+#pragma unroll
+      for (uint32_t x = 0; x < 8; x++) {
+        const uint32_t col = bx + x < pw - 1 ? bx + x : pw - 1;
+        p[x] = (float)src[col] * scale - 256.0f;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 8; u++) {
+        float acc = 0.0f;
+#pragma unroll
+        for (uint32_t x = 0; x < 8; x++) acc = acc + p[x] * basis[8 * x + u];
+        tmp[lb * 64 + 8 * r + u] = acc;
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (uint32_t u = 0; u < 8; u++) {
+        float acc = 0.0f;
+#pragma unroll
+        for (uint32_t y = 0; y < 8; y++) acc = acc + basis[8 * y + r] * tmp[lb * 64 + 8 * y + u];
+        outc[cbase + nb * scan[8 * r + u] + b] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  float *dst = coefs + S.coef;
+  for (uint32_t i = tid; i < TB * 64; i += 256) dst[i] = outc[i];
+}
+
+// ---------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(256)
+    pe_alpha(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
+             const EncSlice *__restrict__ slices, uint32_t n_slices, uint8_t *__restrict__ scratch,
+             uint32_t *__restrict__ alpha_len, EncState *__restrict__ state) {
+  const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= n_slices) return;
+  const EncSlice S = slices[g];
+  const EncPic &P = pics[S.pic];
+  const EncFrame &F = frames[P.frame];
+  if (F.alpha_bits == 0) {
+    alpha_len[g] = 0;
+    return;
+  }
+  // slice_alpha (src/encode.rs:838-868): whole macroblocks, 16 rows of
+  // 16 * mbs values, the last column and row repeated past the picture.
+  const uint32_t bits = F.alpha_bits, mag = bits == 8 ? 3u : 6u, maxd = bits == 8 ? 8u : 64u;
+  const uint32_t lg = 4 + ilog2(S.mbs), cols = 1u << lg, count = 16 * cols;
+  const uint32_t x0 = 16u * S.mb_x, width = F.width;
+  const uint16_t *alpha = (const uint16_t *)F.in_alpha;
+  const uint32_t ident = F.rescale_identity, max_in = F.max_in;
+  const uint64_t m = F.rescale_m;
+  auto value = [&](uint32_t i) -> uint32_t {
+    const uint32_t rr = i >> lg, n = i & (cols - 1);
+    const uint32_t pr = 16u * S.mb_y + rr < P.height - 1 ? 16u * S.mb_y + rr : P.height - 1;
+    const uint32_t row = P.first_row + P.row_step * pr;
+    const uint32_t col = x0 + n < width - 1 ? x0 + n : width - 1;
+    uint32_t a = alpha[(size_t)row * width + col];
+    a = a < max_in ? a : max_in;
+    return rescale(a, ident, m);
+  };
+  uint32_t *out = (uint32_t *)(scratch + (size_t)g * ALPHA_SLICE_CAP);
+  uint64_t acc = 0;
+  uint32_t nacc = 0, nw = 0, total = 0;
+  auto put = [&](uint32_t v, uint32_t n) {  // n <= 17
+    acc = (acc << n) | (v & ((1u << n) - 1));
+    nacc += n;
+    total += n;
+    if (nacc >= 32) {
+      nacc -= 32;
+      out[nw++] = __builtin_bswap32((uint32_t)(acc >> nacc));
+    }
+  };
+  // encode_alpha (src/encode.rs:457-470) with the codes of src/vlc.rs:254-287.
+  int32_t prev = -1;
+  uint32_t i = 0, cur = value(0);
+  while (i < count) {
+    const uint32_t a = cur;
+    uint32_t run = 1;
+    while (i + run < count) {
+      const uint32_t nv = value(i + run);
+      if (nv != a) {
+        cur = nv;
+        break;
+      }
+      run++;
+    }
+    const int32_t diff = (int32_t)a - prev;
+    const uint32_t ad = diff < 0 ? (uint32_t)(-diff) : (uint32_t)diff;
+    if (ad >= 1 && ad <= maxd)
+      put((((ad - 1) << 1) | (diff < 0 ? 1u : 0u)), 1 + mag + 1);
+    else
+      put((1u << bits) | ((uint32_t)diff & ((1u << bits) - 1)), 1 + bits);
+    if (run == 1)
+      put(1, 1);
+    else if (run <= 16)
+      put(run - 1, 5);
+    else
+      put(run - 1, 16);
+    prev = (int32_t)a;
+    i += run;
+  }
+  if (total & 7) put(0, 8 - (total & 7));
+  if (nacc) out[nw] = __builtin_bswap32((uint32_t)(acc << (32 - nacc)));
+  alpha_len[g] = total / 8;
+  atomicAdd(&state[S.pic].alpha_total, total / 8);
+}
+
+// ---------------------------------------------------------------------------
+// One wavefront per slice, four per workgroup.
+extern "C" __global__ void __launch_bounds__(256)
+    pe_size(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
+            const EncSlice *__restrict__ slices, uint32_t n_slices, const float *__restrict__ coefs,
+            const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
+            EncState *__restrict__ state) {
+  const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= n_slices) return;
+  const EncSlice S = slices[g];
+  const uint32_t qi = state[S.pic].probe;
+  if (qi == 0) return;
+  const EncPic &P = pics[S.pic];
+  const EncFrame &F = frames[P.frame];
+  uint32_t size[1];
+  bool fits[1];
+  slice_sizes<1>(coefs + S.coef, S.mbs, F.chroma == 2 ? 2u : 4u, qtabs + (size_t)P.qtab * 225 * 128,
+                 &qi, 1, P.header_bytes, alpha_len[g], size, fits);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&state[S.pic].sum, (unsigned long long)size[0]);
+    if (!fits[0]) atomicAdd(&state[S.pic].notfit, 1u);
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+    pe_search(const EncPic *__restrict__ pics, EncState *__restrict__ state, uint32_t n_pics,
+              uint32_t window) {
+  const uint32_t p = blockIdx.x * 64 + threadIdx.x;
+  if (p >= n_pics) return;
+  EncState &st = state[p];
+  if (st.phase == 2) return;
+  if (st.phase == 0) {
+    // (budget + alpha) less the picture header and the slice table (src/encode.rs:686-689).
+    const uint64_t b = pics[p].budget + st.alpha_total;
+    const uint64_t over = 8 + 2ull * pics[p].n_slices;
+    st.budget = b > over ? b - over : 0;
+  }
+  const bool ok = st.notfit == 0 && st.sum <= st.budget;
+  if (st.phase == 0) {
+    st.fits_at_all = ok;
+    if (ok) {
+      st.lo = 1;
+      st.hi = 224;
+      st.phase = 1;
+    } else {
+      st.uniform = 224;
+      st.phase = 2;
+    }
+  } else if (ok) {
+    st.hi = st.probe;
+  } else {
+    st.lo = st.probe + 1;
+  }
+  if (st.phase == 1) {
+    if (st.lo < st.hi) {
+      st.probe = st.lo + (st.hi - st.lo) / 2;
+    } else {
+      st.uniform = st.hi;
+      st.phase = 2;
+    }
+  }
+  if (st.phase == 2) {
+    st.probe = 0;
+    st.window_k = (st.fits_at_all && st.uniform > 1) ? (st.uniform < window ? st.uniform : window) : 1;
+  }
+  st.sum = 0;
+  st.notfit = 0;
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    pe_window(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
+              const EncSlice *__restrict__ slices, uint32_t n_slices, const float *__restrict__ coefs,
+              const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
+              const EncState *__restrict__ state, uint32_t *__restrict__ size_tab) {
+  const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= n_slices) return;
+  const EncSlice S = slices[g];
+  const EncState &st = state[S.pic];
+  const EncPic &P = pics[S.pic];
+  const EncFrame &F = frames[P.frame];
+  const uint32_t K = st.window_k;
+  uint32_t qis[ENC_MAXK];
+#pragma unroll
+  for (uint32_t k = 0; k < ENC_MAXK; k++) qis[k] = k < K ? st.uniform - k : st.uniform;
+  uint32_t size[ENC_MAXK];
+  bool fits[ENC_MAXK];
+  slice_sizes<ENC_MAXK>(coefs + S.coef, S.mbs, F.chroma == 2 ? 2u : 4u,
+                        qtabs + (size_t)P.qtab * 225 * 128, qis, K, P.header_bytes, alpha_len[g],
+                        size, fits);
+  const uint32_t L = threadIdx.x & 63;
+  if (L < K) {
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < ENC_MAXK; k++)
+      if (k == L) v = size[k] | (fits[k] ? 0u : 0x80000000u);
+    size_tab[(size_t)g * ENC_MAXK + L] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The share-out (src/encode.rs:769-793). With S the budget left after every
+// slice's size at the uniform qi, slice i sees slack_i = S - sum_{j<i} extra_j
+// (extra_j = used_j - base_j), a triangular recurrence: 64 slices at a time,
+// every lane decides from a guess of its slack, the guesses are replaced by
+// the prefix sums of the decisions, and the loop stops at the fixed point,
+// which is the sequential answer (lane j is exact after j rounds).
+extern "C" __global__ void __launch_bounds__(64)
+    pe_shareout(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
+                const EncSlice *__restrict__ slices, const float *__restrict__ coefs,
+                const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
+                EncState *__restrict__ state, const uint32_t *__restrict__ size_tab,
+                uint32_t *__restrict__ used, uint32_t *__restrict__ qidx) {
+  const uint32_t p = blockIdx.x, L = threadIdx.x;
+  const EncPic &P = pics[p];
+  EncState &st = state[p];
+  const uint32_t n = P.n_slices, g0 = P.slice_base, U = st.uniform, K = st.window_k;
+  if (!(U > 1 && st.fits_at_all)) {
+    for (uint32_t i = L; i < n; i += 64) {
+      used[g0 + i] = size_tab[(size_t)(g0 + i) * ENC_MAXK] & 0x7FFFFFFFu;
+      qidx[g0 + i] = U;
+    }
+    return;
+  }
+  const EncFrame &F = frames[P.frame];
+  const uint32_t cpm = F.chroma == 2 ? 2u : 4u;
+  const float *qt = qtabs + (size_t)P.qtab * 225 * 128;
+  uint64_t sum_base = 0, total_mbs = 0;
+  for (uint32_t i = L; i < n; i += 64) {
+    sum_base += size_tab[(size_t)(g0 + i) * ENC_MAXK] & 0x7FFFFFFFu;
+    total_mbs += slices[g0 + i].mbs;
+  }
+  sum_base = wave_sum64(sum_base);
+  total_mbs = wave_sum64(total_mbs);
+  int64_t slack = (int64_t)(st.budget - sum_base);
+  uint64_t mbs_before = 0;
+  uint32_t fallbacks = 0;
+  for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+    const uint32_t i = c0 + L;
+    const bool valid = i < n;
+    const uint32_t g = g0 + (valid ? i : 0);
+    const uint32_t base = valid ? (size_tab[(size_t)g * ENC_MAXK] & 0x7FFFFFFFu) : 0u;
+    const uint32_t m = valid ? slices[g].mbs : 0u;
+    const uint64_t M = total_mbs - mbs_before - excl_scan(m, L);
+    int64_t s_i = slack, extra = 0;
+    uint32_t q_i = U, u_i = base;
+    for (uint32_t round = 0; round <= 64; round++) {
+      // Slice i's own steps for a given slack (src/encode.rs:773-789).
+      extra = 0;
+      q_i = U;
+      u_i = base;
+      if (valid) {
+        const uint64_t sc = s_i < 0 ? 0ull : (uint64_t)s_i;
+        const uint64_t allowed = base + sc * m / M;
+        uint32_t qi = U;
+        while (qi > 1) {
+          const uint32_t t = U - qi + 1;
+          uint32_t sz;
+          bool fits;
+          if (t < K) {
+            const uint32_t v = size_tab[(size_t)g * ENC_MAXK + t];
+            sz = v & 0x7FFFFFFFu;
+            fits = (v >> 31) == 0;
+          } else {
+            slice_size_serial(coefs + slices[g].coef, m, cpm, qt, qi - 1, P.header_bytes, alpha_len[g], sz, fits);
+            fallbacks++;
+          }
+          if (!(fits && sz <= allowed)) break;
+          qi--;
+          u_i = sz;
+        }
+        q_i = qi;
+        extra = (int64_t)u_i - (int64_t)base;
+      }
+      const int64_t s_new = slack - excl_scan64(extra, L);
+      if (!__ballot(valid && s_new != s_i)) break;
+      s_i = s_new;
+    }
+    if (valid) {
+      used[g] = u_i;
+      qidx[g] = q_i;
+    }
+    int64_t ext = extra;
+    for (int o = 32; o > 0; o >>= 1) {
+      uint32_t lo = __shfl_xor((uint32_t)(uint64_t)ext, o, 64);
+      uint32_t hi = __shfl_xor((uint32_t)((uint64_t)ext >> 32), o, 64);
+      ext += (int64_t)(((uint64_t)hi << 32) | lo);
+    }
+    slack -= ext;
+    mbs_before += wave_sum64(m);
+  }
+  fallbacks = wave_sum(fallbacks);
+  if (L == 0) st.fallback = fallbacks;
+}
+
+// ---------------------------------------------------------------------------
+// One workgroup per frame: the picture header, the slice table and the
+// slices, back to back, after the frame header (src/encode.rs:820-829,
+// 284-286). Offsets are relative to the frame's first byte.
+extern "C" __global__ void __launch_bounds__(256)
+    pe_offsets(const EncFrame *__restrict__ frames, EncPic *__restrict__ pics,
+               const uint32_t *__restrict__ used, uint64_t *__restrict__ slice_out,
+               uint64_t *__restrict__ frame_size) {
+  const uint32_t f = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const EncFrame &F = frames[f];
+  __shared__ uint64_t wsum[4];
+  __shared__ uint64_t carry;
+  uint64_t pos = F.hdr_len;
+  for (uint32_t p = F.first_pic; p < F.first_pic + F.n_pics; p++) {
+    const uint32_t n = pics[p].n_slices, g0 = pics[p].slice_base;
+    const uint64_t data0 = pos + 8 + 2ull * n;
+    if (tid == 0) {
+      pics[p].out_off = pos;
+      carry = 0;
+    }
+    __syncthreads();
+    for (uint32_t b = 0; b < n; b += 256) {
+      const uint32_t k = b + tid;
+      const uint32_t v = k < n ? used[g0 + k] : 0u;
+      uint32_t incl = v;
+      for (uint32_t d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+      }
+      if (lane == 63) wsum[wave] = incl;
+      __syncthreads();
+      uint64_t before = 0;
+      for (uint32_t w = 0; w < wave; w++) before += wsum[w];
+      if (k < n) slice_out[g0 + k] = data0 + carry + before + incl - v;
+      __syncthreads();
+      if (tid == 255) carry += before + incl;
+      __syncthreads();
+    }
+    pos = data0 + carry;
+    __syncthreads();
+  }
+  if (tid == 0) frame_size[f] = pos;
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+constexpr uint32_t RING = 512;  // words of the per-slice bit ring
+
+// ORs the `n` (1..64) low bits of `v` into the ring at bit `o` (MSB first).
+__device__ __forceinline__ void ring_put(uint32_t *ring, uint32_t o, uint64_t v, uint32_t n) {
+  const uint32_t sh = o & 31, w = o >> 5;
+  const uint64_t X = v << (64 - n);
+  atomicOr(&ring[w & (RING - 1)], (uint32_t)(X >> (32 + sh)));
+  if (n + sh > 32) atomicOr(&ring[(w + 1) & (RING - 1)], (uint32_t)(X >> sh));
+  if (n + sh > 64) atomicOr(&ring[(w + 2) & (RING - 1)], (uint32_t)(X << (32 - sh)));
+}
+
+// Stores bytes [lo, hi) of big-endian ring word `w` at `A + 4 w`.
+__device__ __forceinline__ void store_word(uint8_t *A, uint32_t w, uint32_t val, uint32_t lo,
+                                           uint32_t hi) {
+  if (lo == 0 && hi == 4) {
+    *(uint32_t *)(A + 4 * (size_t)w) = __builtin_bswap32(val);
+  } else {
+    for (uint32_t b = lo; b < hi; b++) A[4 * (size_t)w + b] = (uint8_t)(val >> (24 - 8 * b));
+  }
+}
+}  // namespace
+
+// One wavefront per slice: slice header, Y', Cb, Cr codewords (src/encode.rs:
+// 413-437, 796-819) and the alpha bytes; the slice-table entry; the picture
+// header for the first slice of a picture and the frame header for the first
+// slice of a frame.
+extern "C" __global__ void __launch_bounds__(64)
+    pe_pack(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
+            const EncSlice *__restrict__ slices, const float *__restrict__ coefs,
+            const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
+            const uint8_t *__restrict__ alpha_scratch, const uint32_t *__restrict__ used,
+            const uint32_t *__restrict__ qidx, const uint64_t *__restrict__ slice_out,
+            const uint64_t *__restrict__ frame_size, const uint8_t *__restrict__ hdr_table) {
+  __shared__ uint32_t ring[RING];
+  const uint32_t g = blockIdx.x, L = threadIdx.x;
+  const EncSlice S = slices[g];
+  const EncPic &P = pics[S.pic];
+  const EncFrame &F = frames[P.frame];
+  uint8_t *out = (uint8_t *)F.out;
+  const uint32_t k = g - P.slice_base;
+  const uint32_t u = used[g], qi = qidx[g], hb = P.header_bytes;
+  if (L == 0) {
+    uint8_t *te = out + P.out_off + 8 + 2ull * k;
+    te[0] = (uint8_t)(u >> 8);
+    te[1] = (uint8_t)u;
+  }
+  if (k == 0 && L == 0) {
+    // PictureHeader::write (src/header.rs:292-304) with picture_size patched.
+    const uint32_t n = P.n_slices;
+    const uint64_t end = (S.pic + 1 < F.first_pic + F.n_pics) ? pics[S.pic + 1].out_off : frame_size[P.frame];
+    const uint32_t psz = (uint32_t)(end - P.out_off);
+    uint8_t *ph = out + P.out_off;
+    ph[0] = 8 << 3;
+    ph[1] = (uint8_t)(psz >> 24);
+    ph[2] = (uint8_t)(psz >> 16);
+    ph[3] = (uint8_t)(psz >> 8);
+    ph[4] = (uint8_t)psz;
+    const uint32_t dep = n <= 65535 ? n : 0;
+    ph[5] = (uint8_t)(dep >> 8);
+    ph[6] = (uint8_t)dep;
+    ph[7] = (uint8_t)((P.log2_slice_mbs & 3) << 4);
+  }
+  if (g == F.first_slice) {
+    // The frame header (src/header.rs:174-199) with frame_size patched, and
+    // the header of any picture without slices (a one-row frame's empty field).
+    const uint32_t fs = (uint32_t)frame_size[P.frame];
+    for (uint32_t b = L; b < F.hdr_len; b += 64) {
+      uint8_t v = hdr_table[F.hdr_off + b];
+      if (b < 4) v = (uint8_t)(fs >> (24 - 8 * b));
+      out[b] = v;
+    }
+    if (L == 0)
+      for (uint32_t p = F.first_pic; p < F.first_pic + F.n_pics; p++) {
+        if (pics[p].n_slices) continue;
+        uint8_t *ph = out + pics[p].out_off;
+        ph[0] = 8 << 3;
+        ph[1] = 0; ph[2] = 0; ph[3] = 0; ph[4] = 8;
+        ph[5] = 0; ph[6] = 0;
+        ph[7] = (uint8_t)((pics[p].log2_slice_mbs & 3) << 4);
+      }
+  }
+  for (uint32_t i = L; i < RING; i += 64) ring[i] = 0;
+  __syncthreads();
+
+  uint8_t *const sa = out + slice_out[g] + hb;  // first byte of the colour data
+  const uint32_t lead = (uint32_t)((uintptr_t)sa & 3);
+  uint8_t *const A = sa - lead;
+  uint32_t pos = 8 * lead, flushed = 0;
+  const uint32_t cpm = F.chroma == 2 ? 2u : 4u;
+  const float *qt = qtabs + (size_t)P.qtab * 225 * 128;
+  const float *cf0 = coefs + S.coef;
+  uint32_t comp_bytes[3];
+  for (uint32_t c = 0; c < 3; c++) {
+    uint32_t nb, off;
+    component(c, S.mbs, cpm, nb, off);
+    const uint32_t lg = ilog2(nb);
+    const float *cf = cf0 + off;
+    const float *inv = qt + (qi * 2 + (c > 0 ? 1 : 0)) * 64;
+    const uint32_t start = pos;
+    Ctx cx{nb - 1, 4, 1};
+    for (uint32_t base = 0; base < nb * 64; base += 64) {
+      const uint32_t i = base + L;
+      const uint32_t s = i >> lg;
+      const int32_t q = quant(cf[i], inv[s], s == 0 ? 0.5f : 0.42f);
+      uint64_t v1 = 0, v2 = 0;
+      uint32_t n1 = 0, n2 = 0;
+      if (base == 0) {
+        const int32_t d1 = __shfl(q, L ? L - 1 : 0, 64);
+        const int32_t d2 = __shfl(q, L > 1 ? L - 2 : 0, 64);
+        if (L < nb) {
+          if (L == 0) {
+            v1 = book_code(FIRST_DC, signed_to_symbol(q), n1);
+          } else {
+            const int32_t diff = (int32_t)((uint32_t)q - (uint32_t)d1);
+            const int32_t pd = L == 1 ? 3 : (int32_t)((uint32_t)d1 - (uint32_t)d2);
+            const int32_t nn = pd < 0 ? (int32_t)(0u - (uint32_t)diff) : diff;
+            const uint32_t a = pd < 0 ? 0u - (uint32_t)pd : (uint32_t)pd;
+            v1 = book_code(dc_book(a < 3 ? a : 3), signed_to_symbol(nn), n1);
+          }
+        }
+      }
+      const bool nz = i >= nb && q != 0;
+      const uint64_t mask = __ballot(nz);
+      if (mask) {
+        const uint64_t lower = mask & ((1ull << L) - 1);
+        const int Pn = lower ? 63 - (int)__builtin_clzll(lower) : -1;
+        const uint32_t prev_idx = Pn >= 0 ? base + (uint32_t)Pn : cx.idx;
+        const uint32_t run = i - prev_idx - 1;
+        const uint32_t lvl = (q < 0 ? 0u - (uint32_t)q : (uint32_t)q) - 1;
+        uint32_t prun = __shfl(run, Pn >= 0 ? Pn : (int)L, 64);
+        uint32_t plvl = __shfl(lvl, Pn >= 0 ? Pn : (int)L, 64);
+        if (Pn < 0) {
+          prun = cx.run;
+          plvl = cx.lvl;
+        }
+        if (nz) {
+          // Run, then level and sign (src/encode.rs:419-432).
+          v1 = book_code(run_book(prun < 15 ? prun : 15), run, n1);
+          uint32_t ll;
+          const uint64_t lc = book_code(lvl_book(plvl < 8 ? plvl : 8), lvl, ll);
+          v2 = (lc << 1) | (q < 0 ? 1u : 0u);
+          n2 = ll + 1;
+        }
+        const int last = 63 - (int)__builtin_clzll(mask);
+        cx.idx = base + (uint32_t)last;
+        cx.run = __shfl(run, last, 64);
+        cx.lvl = __shfl(lvl, last, 64);
+      }
+      const uint32_t T = n1 + n2;
+      const uint32_t o = pos + excl_scan(T, L);
+      const uint32_t chunk = __shfl(o + T, 63, 64) - pos;
+      if (n1) ring_put(ring, o, v1, n1);
+      if (n2) ring_put(ring, o + n1, v2, n2);
+      __syncthreads();
+      const uint32_t done = (pos + chunk) >> 5;
+      for (uint32_t w = flushed + L; w < done; w += 64) {
+        const uint32_t val = ring[w & (RING - 1)];
+        store_word(A, w, val, w == 0 ? lead : 0u, 4);
+        ring[w & (RING - 1)] = 0;
+      }
+      flushed = done;
+      pos += chunk;
+      __syncthreads();
+    }
+    pos = (pos + 7) & ~7u;  // zero bits to the byte boundary (src/encode.rs:436)
+    comp_bytes[c] = (pos - start) / 8;
+  }
+  // The last, partial word.
+  if ((pos & 31) && L == 0) {
+    const uint32_t w = pos >> 5;
+    store_word(A, w, ring[w & (RING - 1)], w == 0 ? lead : 0u, (pos & 31) / 8);
+  }
+  // Slice header (src/encode.rs:800-817).
+  uint8_t *sh = out + slice_out[g];
+  if (L < hb) {
+    uint8_t v = 0;
+    if (L == 0) v = (uint8_t)(hb << 3);
+    else if (L == 1) v = (uint8_t)qi;
+    else if (L < 8) {
+      const uint32_t cs = comp_bytes[(L - 2) / 2];
+      v = (L & 1) ? (uint8_t)cs : (uint8_t)(cs >> 8);
+    }
+    sh[L] = v;
+  }
+  // Alpha bytes after the colour data.
+  const uint32_t alen = alpha_len[g];
+  uint8_t *ad = sa + comp_bytes[0] + comp_bytes[1] + comp_bytes[2];
+  const uint8_t *as = alpha_scratch + (size_t)g * ALPHA_SLICE_CAP;
+  for (uint32_t b = L; b < alen; b += 64) ad[b] = as[b];
+}

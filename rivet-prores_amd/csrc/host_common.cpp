@@ -1,0 +1,314 @@
+// Host runtime pieces shared by decode and encode: errors, frame and picture
+// header syntax (src/header.rs), the constant tables the kernels read, and
+// the per-device context (code objects loaded with hipModuleLoadData).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "host_internal.h"
+
+extern "C" const unsigned char prores_decode_co[];
+extern "C" const unsigned char prores_encode_co[];
+
+namespace prores_hip {
+
+const char *const TIMES = "\xC3\x97";
+const char *const EN_DASH = "\xE2\x80\x93";
+
+std::string fmtstr(const char *f, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, f);
+  vsnprintf(buf, sizeof buf, f, ap);
+  va_end(ap);
+  return buf;
+}
+
+Err invalid(const std::string &m) { return Err{PRORES_ERR_INVALID, m}; }
+Err unsupported(const std::string &m) { return Err{PRORES_ERR_UNSUPPORTED, m}; }
+Err config_error(const std::string &m) { return Err{PRORES_ERR_CONFIG, m}; }
+Err device_error(const std::string &where, hipError_t e) {
+  return Err{PRORES_ERR_DEVICE, where + ": " + hipGetErrorString(e)};
+}
+Err check(hipError_t e, const char *where) { return e == hipSuccess ? Err{} : device_error(where, e); }
+
+void fill_error(prores_error *out, const Err &e) {
+  if (!out) return;
+  out->kind = e.kind;
+  size_t n = e.msg.size() < sizeof(out->msg) - 1 ? e.msg.size() : sizeof(out->msg) - 1;
+  memcpy(out->msg, e.msg.data(), n);
+  out->msg[n] = 0;
+}
+
+std::vector<uint32_t> slice_sizes(uint32_t width_in_mb, uint32_t log2) {
+  // Slices of 2^log2 macroblocks from the left, then halving (RDD 36 §6.2).
+  std::vector<uint32_t> v;
+  uint32_t size = 1u << log2, left = width_in_mb;
+  while (left) {
+    while (left >= size) {
+      v.push_back(size);
+      left -= size;
+    }
+    size >>= 1;
+  }
+  return v;
+}
+
+void Header::luma_weights(uint8_t *w) const {
+  if (h.has_luma_matrix)
+    memcpy(w, h.luma_matrix, 64);
+  else
+    memset(w, 4, 64);
+}
+void Header::chroma_weights(uint8_t *w) const {
+  if (h.has_chroma_matrix)
+    memcpy(w, h.chroma_matrix, 64);
+  else
+    luma_weights(w);
+}
+
+static uint32_t be32(const uint8_t *p) {
+  return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3];
+}
+static uint16_t be16(const uint8_t *p) { return (uint16_t)((p[0] << 8) | p[1]); }
+
+Err parse_frame_header(const uint8_t *data, size_t len, Header &out) {
+  // The checks and their order follow FrameHeader::parse (src/header.rs:82-170).
+  prores_frame_header &h = out.h;
+  memset(&h, 0, sizeof h);
+  if (len < 28) return invalid(fmtstr("%zu bytes is too short for a ProRes frame header", len));
+  h.frame_size = be32(data);
+  if (memcmp(data + 4, "icpf", 4) != 0) return invalid("no 'icpf' frame identifier");
+  if ((size_t)h.frame_size > len)
+    return invalid(fmtstr("frame_size %u exceeds the %zu bytes given", h.frame_size, len));
+  if (h.frame_size < 28) return invalid("frame_size leaves no room for the frame header");
+  const uint8_t *p = data + 8;
+  const size_t avail = h.frame_size - 8;
+  h.header_size = be16(p);
+  if (h.header_size < 20 || h.header_size > avail)
+    return invalid(fmtstr("frame_header_size %u is out of range", h.header_size));
+  h.bitstream_version = p[3];
+  if (h.bitstream_version > 1) return unsupported(fmtstr("bitstream_version %u", h.bitstream_version));
+  memcpy(h.encoder_identifier, p + 4, 4);
+  h.width = be16(p + 8);
+  h.height = be16(p + 10);
+  if (h.width == 0 || h.height == 0) return invalid(fmtstr("a %u%s%u frame", h.width, TIMES, h.height));
+  const uint32_t chroma = p[12] >> 6;
+  if (chroma != 2 && chroma != 3) return invalid(fmtstr("chroma_format %u is reserved", chroma));
+  h.chroma = chroma;
+  const uint32_t il = (p[12] >> 2) & 3;
+  if (il == 3) return invalid("interlace_mode 3 is reserved");
+  h.interlace = il;
+  h.metadata.aspect_ratio = p[13] >> 4;
+  h.metadata.frame_rate_code = p[13] & 15;
+  h.metadata.color_primaries = p[14];
+  h.metadata.transfer_characteristic = p[15];
+  h.metadata.matrix_coefficients = p[16];
+  const uint32_t alpha = p[17] & 15;
+  if (alpha > 2) return invalid(fmtstr("alpha_channel_type %u is reserved", alpha));
+  h.alpha = alpha;
+  size_t pos = 20;
+  if (p[19] & 2) {
+    if (pos + 64 > h.header_size) return invalid("a quantisation matrix runs past frame_header_size");
+    memcpy(h.luma_matrix, p + pos, 64);
+    h.has_luma_matrix = 1;
+    pos += 64;
+  }
+  if (p[19] & 1) {
+    if (pos + 64 > h.header_size) return invalid("a quantisation matrix runs past frame_header_size");
+    memcpy(h.chroma_matrix, p + pos, 64);
+    h.has_chroma_matrix = 1;
+  }
+  return Err{};
+}
+
+Err check_plausible_size(const Header &hd) {
+  // Every slice needs at least 11 bytes and a picture has at least one slice
+  // per 8 macroblocks (src/decode.rs:107-128).
+  const uint64_t wmb = ((uint64_t)hd.h.width + 15) / 16;
+  const uint64_t per_row = slice_sizes((uint32_t)wmb, 3).size();
+  uint64_t need = 8 + (uint64_t)hd.h.header_size;
+  for (uint32_t i = 0; i < hd.picture_count(); i++)
+    need += 8 + 11 * per_row * (((uint64_t)hd.picture_height(i) + 15) / 16);
+  if ((uint64_t)hd.h.frame_size < need)
+    return invalid(fmtstr("frame_size %u is too small for a %u%s%u frame (at least %llu bytes)",
+                          hd.h.frame_size, hd.h.width, TIMES, hd.h.height,
+                          (unsigned long long)need));
+  return Err{};
+}
+
+Err parse_picture_header(const uint8_t *d, size_t len, PicHeader &out) {
+  // src/header.rs:269-290
+  if (len < 8) return invalid("a picture header runs past the end of the frame");
+  out.header_size = d[0] >> 3;
+  out.picture_size = be32(d + 1);
+  if (out.header_size < 8 || out.header_size > len)
+    return invalid(fmtstr("picture_header_size %u is out of range", out.header_size));
+  if (out.picture_size < out.header_size || out.picture_size > len)
+    return invalid(fmtstr("picture_size %u is out of range", out.picture_size));
+  out.log2_slice_mbs = (d[7] >> 4) & 3;
+  return Err{};
+}
+
+// ---------------------------------------------------------------------------
+// Constant tables. Codebooks as combination codes (lastRiceQ, kRice, kExp);
+// an exp-Golomb code of order k is (0, k, k+1) (RDD 36 §7.1.1.1).
+namespace {
+struct Book {
+  uint32_t lq, kr, ke;
+};
+const Book RUN_BOOKS[16] = {{2, 0, 1}, {2, 0, 1}, {1, 0, 1}, {1, 0, 1}, {0, 0, 1}, {1, 1, 2},
+                            {1, 1, 2}, {1, 1, 2}, {1, 1, 2}, {0, 1, 2}, {0, 1, 2}, {0, 1, 2},
+                            {0, 1, 2}, {0, 1, 2}, {0, 1, 2}, {0, 2, 3}};
+const Book LEVEL_BOOKS[9] = {{2, 0, 2}, {1, 0, 1}, {2, 0, 1}, {0, 0, 1}, {0, 1, 2},
+                             {0, 1, 2}, {0, 1, 2}, {0, 1, 2}, {0, 2, 3}};
+uint32_t bitlen(uint64_t v) { return v ? 64 - __builtin_clzll(v) : 0; }
+// (bits, length) of symbol n.
+void code_of(Book b, uint32_t n, uint64_t &bits, uint32_t &len) {
+  uint64_t th = (uint64_t)(b.lq + 1) << b.kr;
+  if (n < th) {
+    bits = (1ull << b.kr) | (n & ((1ull << b.kr) - 1));
+    len = (uint32_t)(n >> b.kr) + 1 + b.kr;
+  } else {
+    uint64_t v = n - th + (1ull << b.ke);
+    bits = v;
+    len = b.lq + 1 + 2 * bitlen(v) - b.ke - 1;
+  }
+}
+void fill_lut(const Book *books, int nb, uint16_t (*lut)[512]) {
+  for (int c = 0; c < nb; c++) {
+    memset(lut[c], 0, sizeof lut[c]);
+    for (uint32_t n = 0; n < 1024; n++) {
+      uint64_t bits;
+      uint32_t len;
+      code_of(books[c], n, bits, len);
+      if (len > 9) continue;
+      uint32_t first = (uint32_t)bits << (9 - len);
+      for (uint32_t p = 0; p < (1u << (9 - len)); p++) lut[c][first + p] = (uint16_t)((n << 5) | len);
+    }
+  }
+}
+void fill_codes(const Book *books, int nb, uint32_t (*t)[32]) {
+  for (int c = 0; c < nb; c++)
+    for (uint32_t n = 0; n < 32; n++) {
+      uint64_t bits;
+      uint32_t len;
+      code_of(books[c], n, bits, len);
+      t[c][n] = ((uint32_t)bits << 5) | len;
+    }
+}
+const uint8_t SCAN_P[64] = {0,  1,  4,  5,  16, 17, 21, 22, 2,  3,  6,  7,  18, 20, 23, 28,
+                            8,  9,  12, 13, 19, 24, 27, 29, 10, 11, 14, 15, 25, 26, 30, 31,
+                            32, 33, 37, 38, 45, 46, 53, 54, 34, 36, 39, 44, 47, 52, 55, 60,
+                            35, 40, 43, 48, 51, 56, 59, 61, 41, 42, 49, 50, 57, 58, 62, 63};
+}  // namespace
+
+void build_tables(DevTables &t) {
+  // The DCT basis T[x][u] = C(u)/2 cos((2x+1)u pi/16) in double, rounded to
+  // f32 (src/dct.rs:11-25). Every value is far from an f32 rounding tie.
+  const double PI = 3.14159265358979323846;
+  for (int x = 0; x < 8; x++)
+    for (int u = 0; u < 8; u++) {
+      double c = u == 0 ? 0.70710678118654752440 : 1.0;
+      t.basis[8 * x + u] = (float)(c / 2.0 * cos((double)((2 * x + 1) * u) * PI / 16.0));
+    }
+  // Figures 4 and 5 of RDD 36: the interlaced scan is the progressive one
+  // transposed (src/tables.rs:6-26).
+  for (int v = 0; v < 8; v++)
+    for (int u = 0; u < 8; u++) {
+      t.scan[0][8 * v + u] = SCAN_P[8 * v + u];
+      t.scan[1][8 * u + v] = SCAN_P[8 * v + u];
+    }
+  fill_lut(RUN_BOOKS, 16, t.run_lut);
+  fill_lut(LEVEL_BOOKS, 9, t.level_lut);
+  fill_codes(RUN_BOOKS, 16, t.run_codes);
+  fill_codes(LEVEL_BOOKS, 9, t.level_codes);
+}
+
+// ---------------------------------------------------------------------------
+Err DevBuf::reserve(size_t bytes) {
+  if (bytes <= cap) return Err{};
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+  size_t want = bytes + bytes / 4 + 256;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) return device_error("hipMalloc", e);
+  cap = want;
+  return Err{};
+}
+
+Err launch(Device &d, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args) {
+  if (gx == 0 || gy == 0) return Err{};
+  return check(hipModuleLaunchKernel(f, gx, gy, 1, bx, 1, 1, 0, d.stream, args, nullptr),
+               "hipModuleLaunchKernel");
+}
+
+static Err load_functions(hipModule_t m, const char *const *names, hipFunction_t *const *out,
+                          int n) {
+  for (int i = 0; i < n; i++) {
+    Err e = check(hipModuleGetFunction(out[i], m, names[i]), names[i]);
+    if (!e.ok()) return e;
+  }
+  return Err{};
+}
+
+static Err init_device(Device &d, int id) {
+  Err e;
+  d.id = id;
+  if (!(e = check(hipModuleLoadData(&d.dec_mod, prores_decode_co), "hipModuleLoadData(decode)")).ok())
+    return e;
+  const char *const names[] = {"pd_slice_table", "pd_entropy", "pd_idct"};
+  hipFunction_t *const fns[] = {&d.k_slice_table, &d.k_entropy, &d.k_idct};
+  if (!(e = load_functions(d.dec_mod, names, fns, 3)).ok()) return e;
+  DevTables host;
+  build_tables(host);
+  if (!(e = check(hipMalloc(&d.tables, sizeof(DevTables)), "hipMalloc(tables)")).ok()) return e;
+  if (!(e = check(hipMemcpy(d.tables, &host, sizeof host, hipMemcpyHostToDevice), "tables")).ok())
+    return e;
+  if (!(e = check(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking), "hipStreamCreate")).ok())
+    return e;
+  for (auto &ev : d.ev)
+    if (!(e = check(hipEventCreate(&ev), "hipEventCreate")).ok()) return e;
+  return Err{};
+}
+
+Err ensure_encoder(Device &d) {
+  // The encoder's code object is loaded on the first encode (callers hold d.mu).
+  if (d.enc_mod) return Err{};
+  Err e = check(hipModuleLoadData(&d.enc_mod, prores_encode_co), "hipModuleLoadData(encode)");
+  if (!e.ok()) return e;
+  const char *const names[] = {"pe_fdct",     "pe_alpha",   "pe_size", "pe_search",
+                               "pe_window",   "pe_shareout", "pe_offsets", "pe_pack"};
+  hipFunction_t *const fns[] = {&d.k_fdct,   &d.k_alpha_enc, &d.k_size,    &d.k_search,
+                                &d.k_window, &d.k_shareout,  &d.k_offsets, &d.k_pack};
+  e = load_functions(d.enc_mod, names, fns, 8);
+  if (!e.ok()) {
+    (void)hipModuleUnload(d.enc_mod);
+    d.enc_mod = nullptr;
+  }
+  return e;
+}
+
+Err device(Device *&out) {
+  static std::mutex mu;
+  static Device *devices[64] = {nullptr};
+  int id = 0;
+  hipError_t he = hipGetDevice(&id);
+  if (he != hipSuccess) return device_error("hipGetDevice", he);
+  if (id < 0 || id >= 64) return Err{PRORES_ERR_DEVICE, "device index out of range"};
+  std::lock_guard<std::mutex> g(mu);
+  if (!devices[id]) {
+    Device *d = new Device();
+    Err e = init_device(*d, id);
+    if (!e.ok()) {
+      delete d;
+      return e;
+    }
+    devices[id] = d;
+  }
+  out = devices[id];
+  return Err{};
+}
+
+}  // namespace prores_hip

@@ -1,0 +1,103 @@
+// Host-side internals shared by the decode and encode runtimes: the error
+// type, header syntax, slice layout, constant tables and the device context.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/prores_hip.h"
+#include "layout.h"
+
+namespace prores_hip {
+
+// The crate's error (src/error.rs:5-21): a kind and the message text.
+struct Err {
+  int kind = PRORES_OK;
+  std::string msg;
+  bool ok() const { return kind == PRORES_OK; }
+};
+Err invalid(const std::string &m);
+Err unsupported(const std::string &m);
+Err config_error(const std::string &m);
+Err device_error(const std::string &where, hipError_t e);
+std::string fmtstr(const char *f, ...) __attribute__((format(printf, 1, 2)));
+void fill_error(prores_error *out, const Err &e);
+
+extern const char *const TIMES;    // U+00D7 in the reference's size messages
+extern const char *const EN_DASH;  // U+2013 in its ranges
+
+// header::slice_sizes (src/header.rs:308-320).
+std::vector<uint32_t> slice_sizes(uint32_t width_in_mb, uint32_t log2);
+inline uint32_t qscale(uint32_t q) { return q <= 128 ? q : 128 + 4 * (q - 128); }
+
+// FrameHeader (src/header.rs:44-255).
+struct Header {
+  prores_frame_header h;
+  uint32_t alpha_bits() const { return h.alpha == 1 ? 8 : h.alpha == 2 ? 16 : 0; }
+  uint32_t picture_count() const { return h.interlace == 0 ? 1 : 2; }
+  bool top_field(uint32_t i) const { return (h.interlace == 1) == (i == 0); }
+  uint32_t picture_height(uint32_t i) const {
+    if (h.interlace == 0) return h.height;
+    return top_field(i) ? (h.height + 1u) / 2 : h.height / 2u;
+  }
+  void picture_rows(uint32_t i, uint32_t &first, uint32_t &step) const {
+    if (h.interlace == 0) {
+      first = 0;
+      step = 1;
+    } else {
+      first = top_field(i) ? 0 : 1;
+      step = 2;
+    }
+  }
+  void luma_weights(uint8_t *w) const;
+  void chroma_weights(uint8_t *w) const;
+};
+Err parse_frame_header(const uint8_t *data, size_t len, Header &out);
+Err check_plausible_size(const Header &h);
+
+// PictureHeader (src/header.rs:262-290).
+struct PicHeader {
+  uint32_t header_size, picture_size, log2_slice_mbs;
+};
+Err parse_picture_header(const uint8_t *d, size_t len, PicHeader &out);
+
+// The per-device context: modules, kernels, constant tables, one stream.
+struct Device {
+  int id = -1;
+  hipModule_t dec_mod = nullptr, enc_mod = nullptr;
+  hipFunction_t k_slice_table = nullptr, k_entropy = nullptr, k_idct = nullptr;
+  hipFunction_t k_fdct = nullptr, k_alpha_enc = nullptr, k_size = nullptr, k_search = nullptr,
+                k_shareout = nullptr, k_offsets = nullptr, k_pack = nullptr,
+                k_window = nullptr;
+  DevTables *tables = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::mutex mu;  // one batch at a time per device
+};
+// The context of the current device, created on first use.
+Err device(Device *&out);
+// Loads the encoder's kernels into `d` (once).
+Err ensure_encoder(Device &d);
+
+// A grow-only device allocation.
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  Err reserve(size_t bytes);
+  template <class T>
+  T *as() const {
+    return (T *)p;
+  }
+};
+
+Err launch(Device &d, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args);
+Err check(hipError_t e, const char *where);
+
+void build_tables(DevTables &t);
+
+}  // namespace prores_hip

@@ -1,0 +1,86 @@
+// Plain-old-data records shared by the host runtime and the gfx950 kernels.
+// Both sides include this file; nothing here depends on HIP or on the host
+// compiler. Sizes are fixed-width so the layouts agree.
+#pragma once
+#include <stdint.h>
+
+namespace prores_hip {
+
+// Slice-level error codes the kernels record (one 32-bit word per slice
+// component: code | arg << 8). The host turns them into the reference's
+// messages (src/decode.rs:301-338, src/bits.rs:59-94, src/vlc.rs:206-230,
+// src/decode.rs:484-595).
+enum SliceError : uint32_t {
+  E_NONE = 0,
+  E_SLICE_SHORT = 1,     // "a slice is shorter than its header"
+  E_SLICE_HDR_SIZE = 2,  // "slice_header_size {arg} is out of range"
+  E_QINDEX = 3,          // "quantization_index {arg} is reserved"
+  E_COMP_SIZES = 4,      // "slice component sizes exceed the slice"
+  E_PAST_END = 5,        // "a codeword runs past the end of its data"
+  E_PREFIX = 6,          // "a variable-length code's prefix is too long"
+  E_RANGE = 7,           // "a variable-length code's value is out of range"
+  E_DC_OVERFLOW = 8,     // "a DC coefficient overflows"
+  E_RUN_PAST = 9,        // "a run passes the end of the coefficient array"
+  E_ALPHA_RUN = 10,      // "an alpha run passes the end of the slice"
+};
+
+// Process-wide constant tables, uploaded once per device.
+struct DevTables {
+  float basis[64];              // T[x][u] at 8x+u, C(u)/2 cos((2x+1)u pi/16) rounded to f32
+  uint8_t scan[2][64];          // [progressive, interlaced]: scanned index of raster position
+  uint16_t run_lut[16][512];    // (symbol << 5) | length on 9-bit prefixes, 0 = longer
+  uint16_t level_lut[9][512];
+  uint32_t run_codes[16][32];   // (bits << 5) | length, symbols 0..31
+  uint32_t level_codes[9][32];
+};
+
+// A frame to decode into (device pointers and per-frame constants).
+struct DecFrame {
+  uint64_t out_plane[3];  // u16* Y', Cb, Cr (each tightly packed, stride = width)
+  uint64_t out_alpha;     // u16* or 0
+  uint32_t plane_w[3];
+  uint32_t width, height;
+  uint32_t chroma;        // 2 = 4:2:2, 3 = 4:4:4
+  uint32_t alpha_bits;    // 0, 8, 16
+  uint32_t bit_depth;     // output depth
+  uint32_t interlaced;    // selects the scan
+  float out_scale, out_bias, out_max;
+  uint32_t rescale_identity;
+  uint64_t rescale_m;
+  uint8_t luma_w[64], chroma_w[64];
+};
+
+// One picture (a frame, or one field of an interlaced frame).
+struct DecPic {
+  uint64_t pic_off;        // device address of the picture header
+  uint64_t coef_base;      // element offset of the picture's first slice coefficients
+  uint32_t header_size, picture_size;
+  uint32_t n_slices, row_slices;
+  uint32_t slice_base;     // global index of the first slice
+  uint32_t layout_base;    // index of the first RowSlot of this picture
+  uint32_t row_coefs;      // coefficients in one row of slices
+  uint32_t frame;          // DecFrame index
+  uint32_t height, height_in_mb, first_row, row_step;
+};
+
+// Where slice j of a row sits: its first macroblock, its size and the
+// offset of its coefficients inside the row.
+struct RowSlot {
+  uint16_t mb_x, mbs;
+  uint32_t coef_prefix;
+};
+
+// One slice, located by the device slice-table walk.
+struct SliceRec {
+  uint64_t off;    // device address of the slice
+  uint64_t coef;   // element offset of its coefficients
+  uint32_t len;    // bytes
+  uint32_t pic;    // DecPic index
+  uint16_t mb_x, mb_y;
+  uint32_t mbs;
+};
+
+// Picture status words written by the kernels.
+enum PicStatus : uint32_t { PIC_OK = 0, PIC_TABLE_ERR = 1 };
+
+}  // namespace prores_hip

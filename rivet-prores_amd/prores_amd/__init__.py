@@ -1,0 +1,533 @@
+"""prores_amd — the MI355X ProRes slice engine, Python face.
+
+Mirrors the rivet-prores crate's API (src/lib.rs:52-56): ``Decoder``,
+``Encoder``, ``Config``, ``Profile``, ``Frame``, ``FrameHeader``, ``Error``
+(``Invalid`` / ``Unsupported`` / ``Config``), over the C ABI of
+``include/prores_hip.h`` (``libprores_hip.so``). Every decode and encode runs
+the gfx950 kernels; there is no CPU fallback: without the built library or a
+GPU the calls raise.
+"""
+
+import ctypes as C
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libprores_hip.so")
+
+# ---------------------------------------------------------------------------
+# Errors (src/error.rs:5-21). Equal when the kind and the message are equal,
+# as the reference's `Error: PartialEq` is.
+
+
+class Error(Exception):
+    kind = "Error"
+    prefix = ""
+
+    def __init__(self, msg):
+        super().__init__(msg)
+        self.msg = msg
+
+    def __eq__(self, other):
+        return type(self) is type(other) and self.msg == other.msg
+
+    def __hash__(self):
+        return hash((self.kind, self.msg))
+
+    def __str__(self):
+        return self.prefix + self.msg
+
+
+class Invalid(Error):
+    kind = "Invalid"
+    prefix = "invalid ProRes data: "
+
+
+class Unsupported(Error):
+    kind = "Unsupported"
+    prefix = "unsupported ProRes feature: "
+
+
+class ConfigError(Error):
+    kind = "Config"
+    prefix = "invalid ProRes configuration: "
+
+
+class DeviceError(Error):
+    """A HIP runtime failure (no device, out of memory): not a codec error."""
+    kind = "Device"
+    prefix = "device: "
+
+
+_KINDS = {1: Invalid, 2: Unsupported, 3: ConfigError, 4: DeviceError}
+
+# ---------------------------------------------------------------------------
+# C structures (include/prores_hip.h).
+
+
+class _Err(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("msg", C.c_char * 256)]
+
+
+class _Meta(C.Structure):
+    _fields_ = [("aspect_ratio", C.c_uint8), ("frame_rate_code", C.c_uint8),
+                ("color_primaries", C.c_uint8), ("transfer_characteristic", C.c_uint8),
+                ("matrix_coefficients", C.c_uint8)]
+
+
+class _Plane(C.Structure):
+    _fields_ = [("offset", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class _Frame(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("bit_depth", C.c_uint32),
+                ("chroma", C.c_uint32), ("interlace", C.c_uint32), ("metadata", _Meta),
+                ("data", C.c_void_p), ("data_len", C.c_size_t), ("planes", _Plane * 3),
+                ("alpha", C.c_void_p), ("alpha_len", C.c_size_t)]
+
+
+class _Header(C.Structure):
+    _fields_ = [("frame_size", C.c_uint32), ("header_size", C.c_uint16),
+                ("bitstream_version", C.c_uint8), ("encoder_identifier", C.c_uint8 * 4),
+                ("width", C.c_uint16), ("height", C.c_uint16), ("chroma", C.c_uint32),
+                ("interlace", C.c_uint32), ("metadata", _Meta), ("alpha", C.c_uint32),
+                ("has_luma_matrix", C.c_uint8), ("has_chroma_matrix", C.c_uint8),
+                ("luma_matrix", C.c_uint8 * 64), ("chroma_matrix", C.c_uint8 * 64)]
+
+
+class _Config(C.Structure):
+    _fields_ = [("profile", C.c_uint32), ("log2_slice_mbs", C.c_uint32), ("alpha", C.c_uint32),
+                ("has_luma_matrix", C.c_uint8), ("has_chroma_matrix", C.c_uint8),
+                ("luma_matrix", C.c_uint8 * 64), ("chroma_matrix", C.c_uint8 * 64),
+                ("target_frame_bytes", C.c_uint64), ("encoder_identifier", C.c_uint8 * 4),
+                ("threads", C.c_uint32)]
+
+
+class _DevDecodeJob(C.Structure):
+    _fields_ = [("host_packet", C.c_void_p), ("len", C.c_size_t), ("dev_packet", C.c_void_p),
+                ("dev_data", C.c_void_p), ("dev_alpha", C.c_void_p)]
+
+
+class _DevEncodeJob(C.Structure):
+    _fields_ = [("cfg", C.POINTER(_Config)), ("frame", _Frame), ("dev_out", C.c_void_p),
+                ("out_len", C.c_size_t)]
+
+
+# The exported symbols of include/prores_hip.h.
+EXPORTS = [
+    "prores_decoder_create", "prores_decoder_destroy", "prores_decode", "prores_decode_into",
+    "prores_frame_free", "prores_frame_header_parse", "prores_frame_layout",
+    "prores_config_default", "prores_encode", "prores_buffer_free",
+    "prores_profile_target_frame_bytes", "prores_profile_fourcc", "prores_profile_from_fourcc",
+    "prores_decode_device_batch", "prores_encode_device_batch", "prores_last_batch_timing",
+    "prores_last_encode_timing", "prores_version",
+]
+
+_lib = None
+
+
+def lib():
+    """The loaded C library; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run rivet-prores_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.prores_decoder_create.argtypes = [C.c_uint32, C.c_uint32, P(C.c_void_p), P(_Err)]
+        L.prores_decoder_destroy.argtypes = [C.c_void_p]
+        L.prores_decode.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, P(_Frame), P(_Err)]
+        L.prores_decode_into.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, P(_Frame), P(_Err)]
+        L.prores_frame_free.argtypes = [P(_Frame)]
+        L.prores_frame_header_parse.argtypes = [C.c_char_p, C.c_size_t, P(_Header), P(_Err)]
+        L.prores_frame_layout.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, P(_Plane)]
+        L.prores_frame_layout.restype = C.c_size_t
+        L.prores_config_default.argtypes = [C.c_uint32, P(_Config)]
+        L.prores_encode.argtypes = [P(_Config), P(_Frame), P(C.c_void_p), P(C.c_size_t), P(_Err)]
+        L.prores_buffer_free.argtypes = [C.c_void_p]
+        L.prores_profile_target_frame_bytes.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.prores_profile_target_frame_bytes.restype = C.c_uint64
+        L.prores_profile_fourcc.argtypes = [C.c_uint32, C.c_char_p]
+        L.prores_profile_from_fourcc.argtypes = [C.c_char_p]
+        L.prores_profile_from_fourcc.restype = C.c_int32
+        L.prores_decode_device_batch.argtypes = [C.c_void_p, C.c_size_t, P(_DevDecodeJob), P(_Err)]
+        L.prores_encode_device_batch.argtypes = [C.c_size_t, P(_DevEncodeJob), P(_Err)]
+        L.prores_last_batch_timing.argtypes = [P(C.c_double)]
+        L.prores_last_encode_timing.argtypes = [P(C.c_double)]
+        L.prores_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _raise(err):
+    if err.kind:
+        raise _KINDS.get(err.kind, Error)(err.msg.decode("utf-8", "replace"))
+
+
+# ---------------------------------------------------------------------------
+# Frame types (src/frame.rs).
+
+
+class ChromaFormat(enum.IntEnum):
+    Yuv422 = 2
+    Yuv444 = 3
+
+    def subsampling(self):
+        return (2, 1) if self is ChromaFormat.Yuv422 else (1, 1)
+
+
+class Interlace(enum.IntEnum):
+    Progressive = 0
+    TopFieldFirst = 1
+    BottomFieldFirst = 2
+
+
+class AlphaType(enum.IntEnum):
+    None_ = 0
+    Bits8 = 1
+    Bits16 = 2
+
+    def bits(self):
+        return (0, 8, 16)[self]
+
+
+class Metadata:
+    """src/frame.rs:83-110; the default is every field unspecified."""
+
+    def __init__(self, aspect_ratio=0, frame_rate_code=0, color_primaries=2,
+                 transfer_characteristic=2, matrix_coefficients=2):
+        self.aspect_ratio = aspect_ratio
+        self.frame_rate_code = frame_rate_code
+        self.color_primaries = color_primaries
+        self.transfer_characteristic = transfer_characteristic
+        self.matrix_coefficients = matrix_coefficients
+
+    def astuple(self):
+        return (self.aspect_ratio, self.frame_rate_code, self.color_primaries,
+                self.transfer_characteristic, self.matrix_coefficients)
+
+    def __eq__(self, o):
+        return isinstance(o, Metadata) and self.astuple() == o.astuple()
+
+    _RATES = {1: (24000, 1001), 2: (24, 1), 3: (25, 1), 4: (30000, 1001), 5: (30, 1), 6: (50, 1),
+              7: (60000, 1001), 8: (60, 1), 9: (100, 1), 10: (120000, 1001), 11: (120, 1)}
+
+    def frame_rate(self):
+        return self._RATES.get(self.frame_rate_code)
+
+    @staticmethod
+    def frame_rate_code_for(num, den):
+        for c, (n, d) in Metadata._RATES.items():
+            if n * den == num * d:
+                return c
+        return 0
+
+
+def layout(width, height, chroma):
+    """Frame::layout (src/frame.rs:247-266): [(offset, width, height)] x 3."""
+    cw = (width + 1) // 2 if int(chroma) == 2 else width
+    y = (0, width, height)
+    cb = (width * height, cw, height)
+    cr = (cb[0] + cw * height, cw, height)
+    return [y, cb, cr]
+
+
+class Frame:
+    """A picture (src/frame.rs:180-203): ``data`` holds Y' then Cb then Cr as
+    u16, each tightly packed; ``alpha`` is a separate plane or None."""
+
+    def __init__(self, width, height, chroma, bit_depth, data=None, alpha=None,
+                 interlace=Interlace.Progressive, metadata=None):
+        self.width, self.height = int(width), int(height)
+        self.chroma = ChromaFormat(int(chroma))
+        self.bit_depth = int(bit_depth)
+        self.interlace = Interlace(int(interlace))
+        self.metadata = metadata if metadata is not None else Metadata()
+        self.planes = layout(self.width, self.height, self.chroma)
+        total = self.planes[2][0] + self.planes[2][1] * self.planes[2][2]
+        if data is None:
+            data = np.full(total, 1 << (self.bit_depth - 1), dtype=np.uint16)
+        self.data = np.ascontiguousarray(data, dtype=np.uint16)
+        self.alpha = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.uint16)
+
+    @staticmethod
+    def new(width, height, chroma, bit_depth):
+        """Frame::new (src/frame.rs:209-213): mid-grey, checked."""
+        if width == 0 or height == 0 or width > 65535 or height > 65535:
+            raise ConfigError(f"a frame of {width}×{height} cannot be coded (1–65535 each way)")
+        if not 8 <= bit_depth <= 16:
+            raise ConfigError(f"bit depth {bit_depth} is outside 8–16")
+        return Frame(width, height, chroma, bit_depth)
+
+    def plane(self, i):
+        off, w, h = self.planes[i]
+        return self.data[off:off + w * h].reshape(h, w)
+
+    def packed(self):
+        return self.data
+
+    def to_le_bytes(self):
+        return self.data.astype("<u2").tobytes()
+
+    def _c(self):
+        f = _Frame()
+        f.width, f.height, f.bit_depth = self.width, self.height, self.bit_depth
+        f.chroma, f.interlace = int(self.chroma), int(self.interlace)
+        (f.metadata.aspect_ratio, f.metadata.frame_rate_code, f.metadata.color_primaries,
+         f.metadata.transfer_characteristic, f.metadata.matrix_coefficients) = self.metadata.astuple()
+        f.data = self.data.ctypes.data
+        f.data_len = self.data.size
+        for i, (o, w, h) in enumerate(self.planes):
+            f.planes[i].offset, f.planes[i].width, f.planes[i].height = o, w, h
+        if self.alpha is not None:
+            f.alpha = self.alpha.ctypes.data
+            f.alpha_len = self.alpha.size
+        return f
+
+    def __eq__(self, o):
+        return (isinstance(o, Frame) and (self.width, self.height, self.bit_depth, self.chroma, self.interlace)
+                == (o.width, o.height, o.bit_depth, o.chroma, o.interlace) and self.metadata == o.metadata
+                and np.array_equal(self.data, o.data)
+                and ((self.alpha is None and o.alpha is None)
+                     or (self.alpha is not None and o.alpha is not None and np.array_equal(self.alpha, o.alpha))))
+
+
+class FrameHeader:
+    """FrameHeader::parse (src/header.rs:82-170)."""
+
+    @staticmethod
+    def parse(data):
+        h = _Header()
+        err = _Err()
+        data = bytes(data)
+        lib().prores_frame_header_parse(data, len(data), C.byref(h), C.byref(err))
+        _raise(err)
+        out = FrameHeader()
+        out.frame_size, out.header_size, out.bitstream_version = h.frame_size, h.header_size, h.bitstream_version
+        out.encoder_identifier = bytes(h.encoder_identifier)
+        out.width, out.height = h.width, h.height
+        out.chroma, out.interlace = ChromaFormat(h.chroma), Interlace(h.interlace)
+        out.metadata = Metadata(h.metadata.aspect_ratio, h.metadata.frame_rate_code, h.metadata.color_primaries,
+                                h.metadata.transfer_characteristic, h.metadata.matrix_coefficients)
+        out.alpha = AlphaType(h.alpha)
+        out.luma_matrix = bytes(h.luma_matrix) if h.has_luma_matrix else None
+        out.chroma_matrix = bytes(h.chroma_matrix) if h.has_chroma_matrix else None
+        return out
+
+    def picture_count(self):
+        return 1 if self.interlace == Interlace.Progressive else 2
+
+    def picture_height(self, index):
+        if self.interlace == Interlace.Progressive:
+            return self.height
+        top = (self.interlace == Interlace.TopFieldFirst) == (index == 0)
+        return (self.height + 1) // 2 if top else self.height // 2
+
+
+# ---------------------------------------------------------------------------
+# Decoder (src/decode.rs:19-104).
+
+
+class Decoder:
+    def __init__(self, bit_depth=0, threads=0):
+        err = _Err()
+        h = C.c_void_p()
+        lib().prores_decoder_create(bit_depth, threads, C.byref(h), C.byref(err))
+        _raise(err)
+        self._h = h
+        self.bit_depth = bit_depth
+        self.threads = threads
+
+    @staticmethod
+    def new():
+        return Decoder()
+
+    @staticmethod
+    def with_bit_depth(bit_depth):
+        return Decoder(bit_depth)
+
+    def with_threads(self, threads):
+        return Decoder(self.bit_depth, threads)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.prores_decoder_destroy(self._h)
+            self._h = None
+
+    def decode(self, data):
+        """Decodes one frame on the GPU; raises Invalid / Unsupported."""
+        data = bytes(data)
+        f = _Frame()
+        err = _Err()
+        lib().prores_decode(self._h, data, len(data), C.byref(f), C.byref(err))
+        _raise(err)
+        try:
+            arr = np.ctypeslib.as_array(C.cast(f.data, C.POINTER(C.c_uint16)), shape=(f.data_len,)).copy()
+            alpha = None
+            if f.alpha:
+                alpha = np.ctypeslib.as_array(C.cast(f.alpha, C.POINTER(C.c_uint16)), shape=(f.alpha_len,)).copy()
+        finally:
+            lib().prores_frame_free(C.byref(f))
+        m = f.metadata
+        return Frame(f.width, f.height, f.chroma, f.bit_depth, arr, alpha, f.interlace,
+                     Metadata(m.aspect_ratio, m.frame_rate_code, m.color_primaries, m.transfer_characteristic,
+                              m.matrix_coefficients))
+
+    def decode_device_batch(self, jobs):
+        """Decodes device-resident packets: ``jobs`` is a list of
+        (host_packet bytes, dev_packet ptr, dev_data ptr, dev_alpha ptr or 0).
+        Returns one Error or None per job."""
+        n = len(jobs)
+        arr = (_DevDecodeJob * n)()
+        keep = []
+        for i, (hp, dp, dd, da) in enumerate(jobs):
+            hb = C.create_string_buffer(bytes(hp), len(hp))
+            keep.append(hb)
+            arr[i].host_packet = C.addressof(hb)
+            arr[i].len = len(hp)
+            arr[i].dev_packet, arr[i].dev_data, arr[i].dev_alpha = dp, dd, da or None
+        errs = (_Err * n)()
+        rc = lib().prores_decode_device_batch(self._h, n, arr, errs)
+        if rc == 4:
+            _raise(errs[0])
+        return [None if e.kind == 0 else _KINDS.get(e.kind, Error)(e.msg.decode()) for e in errs]
+
+
+def last_batch_timing():
+    us = (C.c_double * 4)()
+    lib().prores_last_batch_timing(us)
+    return list(us)
+
+
+def last_encode_timing():
+    us = (C.c_double * 4)()
+    lib().prores_last_encode_timing(us)
+    return list(us)
+
+
+# ---------------------------------------------------------------------------
+# Encoder (src/encode.rs:26-287).
+
+
+class Profile(enum.IntEnum):
+    Proxy = 0
+    Lt = 1
+    Standard = 2
+    Hq = 3
+    P4444 = 4
+    P4444Xq = 5
+
+    def fourcc(self):
+        buf = C.create_string_buffer(5)
+        lib().prores_profile_fourcc(int(self), buf)
+        return buf.raw[:4]
+
+    @staticmethod
+    def from_fourcc(code):
+        p = lib().prores_profile_from_fourcc(bytes(code))
+        return None if p < 0 else Profile(p)
+
+    def chroma(self):
+        return ChromaFormat.Yuv444 if self >= Profile.P4444 else ChromaFormat.Yuv422
+
+    def reference_bitrate(self):
+        return (45_000_000, 102_000_000, 147_000_000, 220_000_000, 330_000_000, 500_000_000)[self]
+
+    def target_frame_bytes(self, width, height):
+        return int(lib().prores_profile_target_frame_bytes(int(self), width, height))
+
+
+Profile.ALL = list(Profile)
+
+
+class Config:
+    """src/encode.rs:135-193; ``Config(profile)`` is ``Config::new``."""
+
+    def __init__(self, profile, log2_slice_mbs=3, alpha=AlphaType.Bits16, luma_matrix=None,
+                 chroma_matrix=None, target_frame_bytes=None, encoder_identifier=b"rivt", threads=0):
+        self.profile = Profile(int(profile))
+        self.log2_slice_mbs = log2_slice_mbs
+        self.alpha = AlphaType(int(alpha))
+        self.luma_matrix = luma_matrix
+        self.chroma_matrix = chroma_matrix
+        self.target_frame_bytes = target_frame_bytes
+        self.encoder_identifier = bytes(encoder_identifier)
+        self.threads = threads
+
+    def _c(self):
+        c = _Config()
+        lib().prores_config_default(int(self.profile), C.byref(c))
+        c.log2_slice_mbs = self.log2_slice_mbs
+        c.alpha = int(self.alpha)
+        for name, m in (("luma", self.luma_matrix), ("chroma", self.chroma_matrix)):
+            if m is not None:
+                setattr(c, f"has_{name}_matrix", 1)
+                arr = getattr(c, f"{name}_matrix")
+                for k in range(64):
+                    arr[k] = int(m[k])
+        c.target_frame_bytes = self.target_frame_bytes or 0
+        for i in range(4):
+            c.encoder_identifier[i] = self.encoder_identifier[i]
+        c.threads = self.threads
+        return c
+
+
+class Encoder:
+    def __init__(self, config):
+        self._config = config
+
+    def config(self):
+        return self._config
+
+    def encode(self, frame):
+        """Encodes one frame on the GPU -> bytes; raises ConfigError."""
+        c = self._config._c()
+        f = frame._c()
+        out = C.c_void_p()
+        n = C.c_size_t()
+        err = _Err()
+        lib().prores_encode(C.byref(c), C.byref(f), C.byref(out), C.byref(n), C.byref(err))
+        _raise(err)
+        try:
+            return C.string_at(out, n.value)
+        finally:
+            lib().prores_buffer_free(out)
+
+
+def encode_device_batch(items):
+    """Encodes device-resident frames. ``items``: list of (Config, Frame
+    metadata, dev_data ptr, dev_alpha ptr or 0). Returns a list of
+    (dev_out ptr, length) or an Error per item."""
+    n = len(items)
+    jobs = (_DevEncodeJob * n)()
+    cfgs = []
+    for i, (cfg, frame, dd, da) in enumerate(items):
+        c = cfg._c()
+        cfgs.append(c)
+        jobs[i].cfg = C.pointer(c)
+        f = frame._c()
+        f.data = dd
+        if frame.alpha is not None or da:
+            f.alpha = da
+            f.alpha_len = frame.width * frame.height
+        else:
+            f.alpha = None
+            f.alpha_len = 0
+        jobs[i].frame = f
+    errs = (_Err * n)()
+    rc = lib().prores_encode_device_batch(n, jobs, errs)
+    if rc == 4:
+        _raise(errs[0])
+    out = []
+    for i in range(n):
+        if errs[i].kind:
+            out.append(_KINDS.get(errs[i].kind, Error)(errs[i].msg.decode()))
+        else:
+            out.append((jobs[i].dev_out, jobs[i].out_len))
+    return out
+
+
+def version():
+    return lib().prores_version().decode()
