@@ -1708,7 +1708,7 @@ static Frame frame_from_c(const oracle_frame *f) {
   fr.data.assign(f->data, f->data + f->data_len);
   for (int i = 0; i < 3; i++) fr.planes[i] = {(size_t)f->plane_offset[i], f->plane_width[i], f->plane_height[i]};
   fr.has_alpha = f->has_alpha != 0;
-  if (fr.has_alpha) fr.alpha.assign(f->alpha, f->alpha + (size_t)f->width * f->height);
+  if (fr.has_alpha) fr.alpha.assign(f->alpha, f->alpha + f->alpha_len);
   return fr;
 }
 
@@ -1743,6 +1743,7 @@ int oracle_decode(const uint8_t *data, size_t len, uint32_t bit_depth, uint32_t 
     if (f.has_alpha) {
       out->alpha = (uint16_t *)malloc(f.alpha.size() * 2 + 2);
       memcpy(out->alpha, f.alpha.data(), f.alpha.size() * 2);
+      out->alpha_len = f.alpha.size();
     }
     return ORACLE_OK;
   } catch (const Error &e) {
@@ -1868,6 +1869,10 @@ uint32_t oracle_alpha_diff_bits(int32_t d, uint32_t nbits, char *bits, size_t ca
   return to_string(v, nb, bits, cap);
 }
 uint16_t oracle_rescale(uint32_t in, uint32_t out, uint16_t a) { return Rescale(in, out).apply(a); }
+void oracle_rescale_many(uint32_t in, uint32_t out, const uint16_t *a, size_t n, uint16_t *o) {
+  Rescale r(in, out);
+  for (size_t i = 0; i < n; i++) o[i] = r.apply(a[i]);
+}
 
 size_t oracle_encode_coefficients(const int32_t *scanned, size_t nb, uint8_t *out, size_t cap) {
   size_t n = nb * 64;

@@ -33,6 +33,7 @@ typedef struct {
   uint16_t *data;
   size_t data_len;
   uint16_t *alpha; /* width*height samples, or NULL */
+  size_t alpha_len;
   uint64_t plane_offset[3];
   uint32_t plane_width[3], plane_height[3];
 } oracle_frame;
@@ -103,6 +104,7 @@ uint32_t oracle_alpha_run_bits(uint32_t run, char *bits, size_t cap);
 uint32_t oracle_alpha_diff_bits(int32_t diff, uint32_t nbits, char *bits, size_t cap);
 /* Rescale, src/decode.rs:446-469 */
 uint16_t oracle_rescale(uint32_t bits_in, uint32_t bits_out, uint16_t a);
+void oracle_rescale_many(uint32_t bits_in, uint32_t bits_out, const uint16_t *a, size_t n, uint16_t *out);
 /* encode_coefficients over a scanned array (src/encode.rs:441-454); returns bytes */
 size_t oracle_encode_coefficients(const int32_t *scanned, size_t n_blocks, uint8_t *out,
                                   size_t cap);

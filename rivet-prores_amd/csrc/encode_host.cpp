@@ -381,9 +381,15 @@ prores_status prores_encode(const prores_config *cfg, const prores_frame *frame,
                             prores_error *err) {
   *out = nullptr;
   *out_len = 0;
-  Device *dev = nullptr;
-  Err e = device(dev);
+  // Checks first: a refused frame or configuration needs no device.
+  EncBatch probe;
+  Err e = plan_encode(*cfg, *frame, 0, 0, probe);
   if (!e.ok()) {
+    fill_error(err, e);
+    return (prores_status)e.kind;
+  }
+  Device *dev = nullptr;
+  if (!(e = device(dev)).ok()) {
     fill_error(err, e);
     return (prores_status)e.kind;
   }
@@ -393,13 +399,6 @@ prores_status prores_encode(const prores_config *cfg, const prores_frame *frame,
     return (prores_status)e.kind;
   }
   EncWorkspace &ws = enc_workspace(*dev);
-  // Checks first, so that a refused frame costs no copies.
-  EncBatch probe;
-  e = plan_encode(*cfg, *frame, 0, 0, probe);
-  if (!e.ok()) {
-    fill_error(err, e);
-    return (prores_status)e.kind;
-  }
   const size_t nsamp = frame->data_len, nalpha = frame->alpha ? frame->alpha_len : 0;
   if ((e = ws.planes.reserve(nsamp * 2 + 64)).ok() && (e = ws.alpha_in.reserve(nalpha * 2 + 64)).ok() &&
       (e = check(hipMemcpyAsync(ws.planes.p, frame->data, nsamp * 2, hipMemcpyHostToDevice, dev->stream), "H2D frame")).ok() &&

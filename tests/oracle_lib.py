@@ -24,7 +24,7 @@ class OFrame(C.Structure):
         ("color_primaries", C.c_uint8), ("transfer_characteristic", C.c_uint8),
         ("matrix_coefficients", C.c_uint8), ("has_alpha", C.c_uint8),
         ("data", C.POINTER(C.c_uint16)), ("data_len", C.c_size_t),
-        ("alpha", C.POINTER(C.c_uint16)),
+        ("alpha", C.POINTER(C.c_uint16)), ("alpha_len", C.c_size_t),
         ("plane_offset", C.c_uint64 * 3), ("plane_width", C.c_uint32 * 3),
         ("plane_height", C.c_uint32 * 3),
     ]
@@ -83,6 +83,7 @@ def lib():
         L.oracle_alpha_diff_bits.restype = C.c_uint32
         L.oracle_rescale.argtypes = [C.c_uint32, C.c_uint32, C.c_uint16]
         L.oracle_rescale.restype = C.c_uint16
+        L.oracle_rescale_many.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p]
         L.oracle_encode_coefficients.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t]
         L.oracle_encode_coefficients.restype = C.c_size_t
         L.oracle_coefficient_bits.argtypes = [C.c_void_p, C.c_size_t]
@@ -147,6 +148,7 @@ class Frame:
         f.has_alpha = 1 if self.alpha is not None else 0
         if self.alpha is not None:
             f.alpha = self.alpha.ctypes.data_as(C.POINTER(C.c_uint16))
+            f.alpha_len = self.alpha.size
         for i, (off, w, h) in enumerate(self.planes):
             f.plane_offset[i], f.plane_width[i], f.plane_height[i] = off, w, h
         return f
@@ -203,7 +205,7 @@ def decode(data, bit_depth=0, threads=0):
     data_arr = np.ctypeslib.as_array(f.data, shape=(f.data_len,)).copy()
     alpha = None
     if f.has_alpha:
-        alpha = np.ctypeslib.as_array(f.alpha, shape=(f.width * f.height,)).copy()
+        alpha = np.ctypeslib.as_array(f.alpha, shape=(f.alpha_len,)).copy()
     out = Frame(f.width, f.height, f.chroma, f.bit_depth, data_arr, alpha, f.interlace,
                 (f.aspect_ratio, f.frame_rate_code, f.color_primaries, f.transfer_characteristic,
                  f.matrix_coefficients))
