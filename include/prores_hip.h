@@ -201,6 +201,13 @@ void prores_last_batch_timing(double *us4);
  * (host sync for the output sizes included). */
 void prores_last_encode_timing(double *us4);
 
+/* Device memory on the current device, for callers without an allocator of
+ * their own (the benchmark, the serving loop, the tests). No counterpart in
+ * the reference. kind: 1 host->device, 2 device->host, 3 device->device. */
+void *prores_device_alloc(size_t bytes);
+void prores_device_free(void *p);
+prores_status prores_memcpy(void *dst, const void *src, size_t bytes, int32_t kind);
+
 /* Library build string. */
 const char *prores_version(void);
 

@@ -836,7 +836,9 @@ extern "C" __global__ void __launch_bounds__(64)
     pos = (pos + 7) & ~7u;  // zero bits to the byte boundary (src/encode.rs:436)
     comp_bytes[c] = (pos - start) / 8;
   }
-  // The last, partial word.
+  // Words completed by the last byte alignment, then the last, partial word.
+  for (uint32_t w = flushed + L; w < (pos >> 5); w += 64)
+    store_word(A, w, ring[w & (RING - 1)], w == 0 ? lead : 0u, 4);
   if ((pos & 31) && L == 0) {
     const uint32_t w = pos >> 5;
     store_word(A, w, ring[w & (RING - 1)], w == 0 ? lead : 0u, (pos & 31) / 8);

@@ -312,3 +312,25 @@ Err device(Device *&out) {
 }
 
 }  // namespace prores_hip
+
+using namespace prores_hip;
+
+extern "C" {
+
+void *prores_device_alloc(size_t bytes) {
+  void *p = nullptr;
+  return hipMalloc(&p, bytes ? bytes : 1) == hipSuccess ? p : nullptr;
+}
+
+void prores_device_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+prores_status prores_memcpy(void *dst, const void *src, size_t bytes, int32_t kind) {
+  const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                          : kind == 2 ? hipMemcpyDeviceToHost
+                                      : hipMemcpyDeviceToDevice;
+  return hipMemcpy(dst, src, bytes, k) == hipSuccess ? PRORES_OK : PRORES_ERR_DEVICE;
+}
+
+}  // extern "C"

@@ -122,7 +122,8 @@ EXPORTS = [
     "prores_config_default", "prores_encode", "prores_buffer_free",
     "prores_profile_target_frame_bytes", "prores_profile_fourcc", "prores_profile_from_fourcc",
     "prores_decode_device_batch", "prores_encode_device_batch", "prores_last_batch_timing",
-    "prores_last_encode_timing", "prores_version",
+    "prores_last_encode_timing", "prores_version", "prores_device_alloc", "prores_device_free",
+    "prores_memcpy",
 ]
 
 _lib = None
@@ -157,6 +158,10 @@ def lib():
         L.prores_last_batch_timing.argtypes = [P(C.c_double)]
         L.prores_last_encode_timing.argtypes = [P(C.c_double)]
         L.prores_version.restype = C.c_char_p
+        L.prores_device_alloc.argtypes = [C.c_size_t]
+        L.prores_device_alloc.restype = C.c_void_p
+        L.prores_device_free.argtypes = [C.c_void_p]
+        L.prores_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int32]
         _lib = L
     return _lib
 
@@ -531,3 +536,50 @@ def encode_device_batch(items):
 
 def version():
     return lib().prores_version().decode()
+
+
+# ---------------------------------------------------------------------------
+# Device buffers (plumbing for device-resident batches).
+
+
+class DeviceBuffer:
+    """A device allocation on the current device."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().prores_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise DeviceError(f"could not allocate {nbytes} bytes of device memory")
+
+    def upload(self, arr, offset=0):
+        a = np.ascontiguousarray(arr)
+        assert offset + a.nbytes <= self.nbytes
+        if lib().prores_memcpy(self.ptr + offset, a.ctypes.data, a.nbytes, 1):
+            raise DeviceError("host-to-device copy failed")
+
+    def download(self, nbytes=None, offset=0, dtype=np.uint8):
+        n = self.nbytes - offset if nbytes is None else nbytes
+        out = np.empty(n // np.dtype(dtype).itemsize, dtype=dtype)
+        if n and lib().prores_memcpy(out.ctypes.data, self.ptr + offset, n, 2):
+            raise DeviceError("device-to-host copy failed")
+        return out
+
+    def free(self):
+        if self.ptr and _lib is not None:
+            _lib.prores_device_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def copy_device(dst_ptr, src_ptr, nbytes):
+    if nbytes and lib().prores_memcpy(dst_ptr, src_ptr, nbytes, 3):
+        raise DeviceError("device-to-device copy failed")
+
+
+def download_ptr(src_ptr, nbytes):
+    out = np.empty(nbytes, dtype=np.uint8)
+    if nbytes and lib().prores_memcpy(out.ctypes.data, src_ptr, nbytes, 2):
+        raise DeviceError("device-to-host copy failed")
+    return out
