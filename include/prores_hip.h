@@ -192,9 +192,10 @@ typedef struct {
 prores_status prores_encode_device_batch(size_t n, prores_dev_encode_job *jobs, prores_error *errs);
 
 /* Microseconds of the last decode batch's kernels by stage, measured with
- * HIP events on the library's stream: [0] slice table, [1] entropy, [2]
- * idct, [3] whole batch. No counterpart in the reference. */
-void prores_last_batch_timing(double *us4);
+ * HIP events on the library's stream: [0] slice table, [1] colour entropy
+ * decode, [2] alpha decode, [3] idct, [4] all kernels. No counterpart in the
+ * reference. */
+void prores_last_batch_timing(double *us5);
 
 /* Microseconds of the last encode batch by stage: [0] fdct + alpha, [1] rate
  * control (probes, window, share-out, offsets), [2] pack, [3] whole batch

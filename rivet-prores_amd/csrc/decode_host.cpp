@@ -53,6 +53,7 @@ struct Batch {
   std::vector<RowSlot> rows;
   std::vector<FramePlan> plans;
   uint64_t n_slices = 0, n_coefs = 0;
+  bool any_alpha = false;
 };
 
 struct Workspace {
@@ -63,7 +64,7 @@ Workspace &workspace(Device &d) {
   return ws[d.id];
 }
 
-double g_last_us[4] = {0, 0, 0, 0};
+double g_last_us[5] = {0, 0, 0, 0, 0};
 
 // Lays out one frame's pictures. `dev_packet` is the device address of the
 // packet; `dev_data` / `dev_alpha` the device output planes.
@@ -112,6 +113,7 @@ void plan_frame(const uint8_t *host, size_t len, uint64_t dev_packet, uint32_t w
   hd.chroma_weights(F.chroma_w);
   const uint32_t frame_index = (uint32_t)b.frames.size();
   b.frames.push_back(F);
+  if (F.alpha_bits) b.any_alpha = true;
 
   fp.first_pic = (uint32_t)b.pics.size();
   const uint32_t wmb = (h.width + 15u) / 16u, cpm = h.chroma == 2 ? 2u : 4u;
@@ -206,14 +208,19 @@ Err run_batch(Device &d, Batch &b, std::vector<Err> &results) {
   (void)hipEventRecord(d.ev[1], d.stream);
   {
     void *args[] = {&tables, &frames, &pics, &slices, &status, &g32, &coefs, &wide, &serr, &ferr};
-    if (!(e = launch(d, d.k_entropy, (unsigned)((G + 255) / 256), 4, 256, args)).ok()) return e;
+    if (!(e = launch(d, d.k_entropy, (unsigned)((G + 255) / 256), 3, 256, args)).ok()) return e;
   }
   (void)hipEventRecord(d.ev[2], d.stream);
+  if (b.any_alpha) {
+    void *args[] = {&frames, &pics, &slices, &status, &serr, &ferr};
+    if (!(e = launch(d, d.k_alpha, (unsigned)G, 1, 64, args)).ok()) return e;
+  }
+  (void)hipEventRecord(d.ev[3], d.stream);
   {
     void *args[] = {&tables, &frames, &pics, &slices, &status, &serr, &coefs, &wide};
     if (!(e = launch(d, d.k_idct, (unsigned)G, 1, 256, args)).ok()) return e;
   }
-  (void)hipEventRecord(d.ev[3], d.stream);
+  (void)hipEventRecord(d.ev[4], d.stream);
   std::vector<uint32_t> st(np), fe(np);
   if (np) {
     if (!(e = check(hipMemcpyAsync(st.data(), ws.status.p, np * 4, hipMemcpyDeviceToHost, d.stream),
@@ -224,13 +231,11 @@ Err run_batch(Device &d, Batch &b, std::vector<Err> &results) {
       return e;
   }
   if (!(e = check(hipStreamSynchronize(d.stream), "hipStreamSynchronize")).ok()) return e;
-  float ms[3] = {0, 0, 0}, all = 0;
-  (void)hipEventElapsedTime(&ms[0], d.ev[0], d.ev[1]);
-  (void)hipEventElapsedTime(&ms[1], d.ev[1], d.ev[2]);
-  (void)hipEventElapsedTime(&ms[2], d.ev[2], d.ev[3]);
-  (void)hipEventElapsedTime(&all, d.ev[0], d.ev[3]);
-  for (int i = 0; i < 3; i++) g_last_us[i] = ms[i] * 1000.0;
-  g_last_us[3] = all * 1000.0;
+  float ms[4] = {0, 0, 0, 0}, all = 0;
+  for (int i = 0; i < 4; i++) (void)hipEventElapsedTime(&ms[i], d.ev[i], d.ev[i + 1]);
+  (void)hipEventElapsedTime(&all, d.ev[0], d.ev[4]);
+  for (int i = 0; i < 4; i++) g_last_us[i] = ms[i] * 1000.0;
+  g_last_us[4] = all * 1000.0;
 
   results.assign(b.plans.size(), Err{});
   for (size_t f = 0; f < b.plans.size(); f++) {
@@ -408,8 +413,8 @@ prores_status prores_decode_device_batch(const prores_decoder *d, size_t n, cons
   return PRORES_OK;
 }
 
-void prores_last_batch_timing(double *us4) {
-  for (int i = 0; i < 4; i++) us4[i] = g_last_us[i];
+void prores_last_batch_timing(double *us5) {
+  for (int i = 0; i < 5; i++) us5[i] = g_last_us[i];
 }
 
 const char *prores_version(void) { return "prores-hip 0.1 gfx950"; }

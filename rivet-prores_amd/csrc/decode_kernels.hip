@@ -1,14 +1,16 @@
 // gfx950 decode kernels for the ProRes slice engine.
 //
-// Three launches per batch of pictures:
+// Four launches per batch of pictures:
 // Packet and slice offsets are absolute device addresses.
 //   pd_slice_table  one workgroup per picture: the u16 big-endian slice-size
 //                   table becomes byte offsets (an exclusive prefix), plus the
 //                   bounds check of the table walk (src/decode.rs:202-221).
-//   pd_entropy      one lane per coded stream (a colour component or the alpha
-//                   of one slice): the RDD 36 adaptive Rice/exp-Golomb DC and
-//                   run/level decode (src/decode.rs:484-540, src/vlc.rs) into a
-//                   coefficient arena in scanned order, alpha straight to the
+//   pd_entropy      one lane per colour component of a slice: the RDD 36
+//                   adaptive Rice/exp-Golomb DC and run/level decode
+//                   (src/decode.rs:484-540, src/vlc.rs) into a coefficient
+//                   arena in scanned order.
+//   pd_alpha        one wavefront per slice with alpha: the run/difference
+//                   decode from an LDS copy of the stream, straight to the
 //                   output plane (src/decode.rs:570-595, 365-389).
 //   pd_idct         one workgroup per slice: dequantisation, the separable f32
 //                   IDCT in the reference's operation order (src/dct.rs:36-66,
@@ -43,91 +45,131 @@ __device__ __forceinline__ int32_t symbol_to_signed(uint32_t s) {
   return (s & 1) == 0 ? (int32_t)(s >> 1) : -(int32_t)((s >> 1) + 1);
 }
 
-// MSB-first reader over one stream of `len` bytes at `base`. The window
-// holds `nbits` loaded bits (at least 33 between operations); bits past the
-// end of the stream read as zero, as the reference's 64-bit peek does
-// (src/bits.rs:30-40). Words are loaded aligned, one ahead of use.
-struct Reader {
-  const uint32_t *src;
-  uint64_t win;
-  uint32_t nbits, wj, maxw, pos, lenbits, nextw;
-  int32_t end_byte;
+// MSB-first reader over one stream of `len` bytes at `base`. The window is
+// two big-endian 64-bit words A|B and a bit offset into A (128 - off >= 65
+// bits held, so a 64-bit window is always whole). The next two words wait in
+// q0/q1, and the two after them are in flight as one raw 16-byte load that is
+// byte-swapped only when q0/q1 are refilled, a 16-byte stretch of the stream
+// after it was issued: the load's latency hides behind that decode, and no
+// register holding a pending load is ever copied. Bits past the end of the
+// stream read as zero, as the reference's 64-bit peek does
+// (src/bits.rs:30-40).
+typedef const __attribute__((address_space(1))) uint64_t gword_t;
 
-  __device__ __forceinline__ uint32_t load_word(uint32_t j) const {
-    return __builtin_bswap32(src[j < maxw ? j : maxw]);
+struct Reader {
+  gword_t *src;
+  uint64_t A, B, q0, q1, r0, r1;  // r0/r1: raw words rj, rj + 1
+  uint32_t off, half, rj, maxw, pos, lenbits;
+  int32_t end_bit;
+
+  __device__ __forceinline__ uint64_t fix(uint64_t raw, uint32_t j) const {
+    uint64_t w = __builtin_bswap64(raw);
+    const int32_t valid = end_bit - 64 * (int32_t)j;
+    if (valid < 64) w = valid <= 0 ? 0ull : (w & (~0ull << (64 - valid)));
+    return w;
   }
-  __device__ __forceinline__ void refill() {
-    uint32_t w = nextw;
-    int32_t valid = end_byte - 4 * (int32_t)wj;
-    if (valid < 4) w = valid <= 0 ? 0u : (w & (0xFFFFFFFFu << (8 * (4 - valid))));
-    win |= (uint64_t)w << (32 - nbits);
-    nbits += 32;
-    wj++;
-    nextw = load_word(wj);
+  // Words j and j + 1 (j clamped to the last word holding stream bytes; the
+  // caller's buffer has 64 readable bytes past every packet).
+  __device__ __forceinline__ void issue(uint32_t j) {
+    const uint32_t c = j < maxw ? j : maxw;
+    r0 = src[c];
+    r1 = src[c + 1];
   }
   __device__ __forceinline__ void init(const uint8_t *base, uint32_t len) {
-    uintptr_t a = (uintptr_t)base;
-    uint32_t m = (uint32_t)(a & 3);
-    src = (const uint32_t *)(a - m);
-    end_byte = (int32_t)(m + len);
-    maxw = (m + len + 3) / 4;
+    const uintptr_t a = (uintptr_t)base;
+    const uint32_t m = (uint32_t)(a & 7);
+    src = (gword_t *)(a - m);
+    end_bit = (int32_t)(8 * (m + len));
+    maxw = (m + len + 7) / 8;
     maxw = maxw ? maxw - 1 : 0;
-    win = 0;
-    nbits = 0;
-    wj = 0;
-    nextw = load_word(0);
-    refill();
-    refill();
-    win <<= 8 * m;
-    nbits -= 8 * m;
-    if (nbits <= 32) refill();
+    issue(0);
+    A = fix(r0, 0);
+    B = fix(r1, 1);
+    issue(2);
+    q0 = fix(r0, 2);
+    q1 = fix(r1, 3);
+    rj = 4;
+    issue(4);
+    half = 0;
+    off = 8 * m;
     pos = 0;
     lenbits = 8 * len;
   }
+  __device__ __forceinline__ uint64_t window() const {
+    return off ? (A << off) | (B >> (64 - off)) : A;
+  }
   __device__ __forceinline__ uint32_t remaining() const { return lenbits - pos; }
-  __device__ __forceinline__ void consume(uint32_t n) {
-    win <<= n;
-    nbits -= n;
+  __device__ __forceinline__ void consume(uint32_t n) {  // n <= 64
     pos += n;
-    if (nbits <= 32) refill();
+    off += n;
+    if (off >= 64) {
+      off -= 64;
+      A = B;
+      if (!half) {
+        B = q0;
+        half = 1;
+      } else {
+        B = q1;
+        q0 = fix(r0, rj);
+        q1 = fix(r1, rj + 1);
+        rj += 2;
+        issue(rj);
+        half = 0;
+      }
+    }
   }
   __device__ __forceinline__ uint32_t take(uint32_t n) {  // n <= 32
-    uint32_t v = n ? (uint32_t)(win >> (64 - n)) : 0u;
+    const uint32_t v = n ? (uint32_t)(window() >> (64 - n)) : 0u;
     consume(n);
     return v;
   }
   // RDD 36 endOfData(): 31 or fewer bits remain and all are zero.
   __device__ __forceinline__ bool end_of_data() const {
-    uint32_t r = remaining();
-    return r <= 31 && (r == 0 || (win >> (64 - r)) == 0);
+    const uint32_t r = remaining();
+    return r <= 31 && (r == 0 || (window() >> (64 - r)) == 0);
   }
 };
+
+// A codeword decoded from the window `w` without branches: the combination
+// code (lastRiceQ, kRice, kExp) of RDD 36 §7.1.1.1. False when the prefix is
+// over 32 zeros, the codeword is over 57 bits or the value over 2^30 — the
+// general reader then says which (src/vlc.rs:179-204).
+__device__ __forceinline__ bool peek(uint64_t w, uint32_t book, uint32_t &sym, uint32_t &len) {
+  const uint32_t lq = book & 15, kr = (book >> 4) & 15, ke = book >> 8;
+  const uint32_t q = clz64(w);
+  const uint32_t qs = q < 63 ? q : 63;
+  const uint32_t lr = qs + 1 + kr;
+  const uint32_t sym_r = (qs << kr) | (uint32_t)((w >> (64 - (lr < 63 ? lr : 63))) & ((1u << kr) - 1));
+  uint32_t tail = qs + ke - lq;  // q - (lq + 1) + ke + 1
+  tail = tail < 1 ? 1 : (tail > 63 ? 63 : tail);
+  const uint64_t v = (w << qs) >> (64 - tail);
+  const uint64_t sym_e = v - (1ull << ke) + ((uint64_t)(lq + 1) << kr);
+  const bool rice = q <= lq;
+  len = rice ? lr : qs + tail;
+  sym = rice ? sym_r : (uint32_t)sym_e;
+  return rice || (q <= MAX_PREFIX && qs + tail <= 57 && sym_e <= MAX_SYMBOL);
+}
 
 // Codebook::read with the reference's checks, in its order
 // (src/vlc.rs:206-230 over src/bits.rs:59-94).
 __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t &sym) {
-  uint32_t last_q = book & 15, k_rice = (book >> 4) & 15, k_exp = book >> 8;
-  uint32_t lz = clz64(r.win);
+  const uint32_t last_q = book & 15, k_rice = (book >> 4) & 15, k_exp = book >> 8;
+  const uint32_t lz = clz64(r.window());
   if (lz > MAX_PREFIX) return E_PREFIX;
   if (lz >= r.remaining()) return E_PAST_END;
   r.consume(lz);
   if (lz <= last_q) {
-    uint32_t n = 1 + k_rice;
+    const uint32_t n = 1 + k_rice;
     if (n > r.remaining()) return E_PAST_END;
-    uint32_t v = r.take(n) & ((1u << k_rice) - 1);
+    const uint32_t v = r.take(n) & ((1u << k_rice) - 1);
     sym = (lz << k_rice) | v;
     return E_NONE;
   }
-  uint32_t n = lz - (last_q + 1) + k_exp + 1;  // <= 39
+  const uint32_t n = lz - (last_q + 1) + k_exp + 1;  // <= 39
   if (n > r.remaining()) return E_PAST_END;
-  uint64_t v;
-  if (n > 32) {
-    uint64_t hi = r.take(n - 32);
-    v = (hi << 32) | r.take(32);
-  } else {
-    v = r.take(n);
-  }
-  uint64_t val = v - (1ull << k_exp) + ((uint64_t)(last_q + 1) << k_rice);
+  const uint64_t v = r.window() >> (64 - n);
+  r.consume(n);
+  const uint64_t val = v - (1ull << k_exp) + ((uint64_t)(last_q + 1) << k_rice);
   if (val > MAX_SYMBOL) return E_RANGE;
   sym = (uint32_t)val;
   return E_NONE;
@@ -146,7 +188,7 @@ struct CoefOut {
     *(uint4 *)(base + 8 * grp) = make_uint4(a, b, c, d);
   }
   __device__ __forceinline__ void put(uint32_t n, int32_t v) {
-    uint32_t grp = n >> 3;
+    const uint32_t grp = n >> 3;
     if (grp != cur) {
       store(cur, w0, w1, w2, w3);
       for (uint32_t g = cur + 1; g < grp; g++) store(g, 0, 0, 0, 0);
@@ -158,8 +200,8 @@ struct CoefOut {
       h = -32768;
       wide[n] = v;
     }
-    uint32_t word = (uint32_t)(uint16_t)(int16_t)h << (16 * (n & 1));
-    uint32_t wi = (n >> 1) & 3;
+    const uint32_t word = (uint32_t)(uint16_t)(int16_t)h << (16 * (n & 1));
+    const uint32_t wi = (n >> 1) & 3;
     w0 |= wi == 0 ? word : 0u;
     w1 |= wi == 1 ? word : 0u;
     w2 |= wi == 2 ? word : 0u;
@@ -171,10 +213,26 @@ struct CoefOut {
   }
 };
 
+// Codebook parameters of the run and level contexts, 12 bits each as in
+// RUN_BOOK / LEVEL_BOOK, packed for a select-free lookup.
+__device__ __forceinline__ uint32_t run_book_of(uint32_t prev) {
+  const uint32_t c = prev < 15 ? prev : 15;
+  // contexts 0-1: (2,0,1); 2-3: (1,0,1); 4: (0,0,1); 5-8: (1,1,2); 9-14: (0,1,2); 15: (0,2,3)
+  return c < 2 ? 0x102u : c < 4 ? 0x101u : c == 4 ? 0x100u : c < 9 ? 0x211u : c < 15 ? 0x210u : 0x320u;
+}
+__device__ __forceinline__ uint32_t level_book_of(uint32_t prev) {
+  const uint32_t c = prev < 8 ? prev : 8;
+  // 0: (2,0,2); 1: (1,0,1); 2: (2,0,1); 3: (0,0,1); 4-7: (0,1,2); 8: (0,2,3)
+  return c == 0 ? 0x202u : c == 1 ? 0x101u : c == 2 ? 0x102u : c == 3 ? 0x100u : c < 8 ? 0x210u : 0x320u;
+}
+
 // scanned_coefficients() of one colour component (src/decode.rs:484-540).
-__device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out,
-                                     const uint16_t *__restrict__ lut_run,
-                                     const uint16_t *__restrict__ lut_lvl) {
+// The common case decodes run, level and sign from one window without
+// branches; it is taken when both codewords are valid, lie in the window and
+// 32 or more bits remain (so endOfData cannot hold and every bit read is
+// real), which gives the reference's slow path's result. Otherwise the lane
+// follows the slow path check for check.
+__device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out) {
   uint32_t sym;
   uint32_t e = read_code(r, FIRST_DC_BOOK, sym);
   if (e) return e;
@@ -182,59 +240,87 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out,
   out.put(0, prev_dc);
   int32_t prev_diff = 3;
   for (uint32_t i = 1; i < nb; i++) {
-    uint32_t a = prev_diff < 0 ? (uint32_t)(-prev_diff) : (uint32_t)prev_diff;
-    e = read_code(r, DC_BOOK[a < 3 ? a : 3], sym);
-    if (e) return e;
-    int32_t n = symbol_to_signed(sym);
-    int32_t diff = prev_diff < 0 ? -n : n;
-    int64_t s = (int64_t)prev_dc + diff;
+    const uint32_t a = prev_diff < 0 ? (uint32_t)(-prev_diff) : (uint32_t)prev_diff;
+    const uint32_t book = DC_BOOK[a < 3 ? a : 3];
+    const uint64_t w = r.window();
+    uint32_t len;
+    if (r.remaining() >= 32 && peek(w, book, sym, len) && len <= r.remaining()) {
+      r.consume(len);
+    } else {
+      e = read_code(r, book, sym);
+      if (e) return e;
+    }
+    const int32_t n = symbol_to_signed(sym);
+    const int32_t diff = prev_diff < 0 ? -n : n;
+    const int64_t s = (int64_t)prev_dc + diff;
     if (s > INT32_MAX || s < INT32_MIN) return E_DC_OVERFLOW;
     prev_dc = (int32_t)s;
     out.put(i, prev_dc);
     prev_diff = diff;
   }
-  uint32_t total = nb * 64, n = nb, prev_run = 4, prev_lvl = 1;
+  const uint32_t total = nb * 64;
+  uint32_t n = nb, prev_run = 4, prev_lvl = 1;
   for (;;) {
-    if (r.remaining() >= 32) {
-      // Both codewords from the 9-bit tables: at most 19 bits, all real.
-      uint32_t e1 = lut_run[(prev_run < 15 ? prev_run : 15) * 512 + (uint32_t)(r.win >> 55)];
-      if (e1) {
-        uint32_t l1 = e1 & 31;
-        uint64_t w2 = r.win << l1;
-        uint32_t e2 = lut_lvl[(prev_lvl < 8 ? prev_lvl : 8) * 512 + (uint32_t)(w2 >> 55)];
-        if (e2) {
-          uint32_t l2 = e2 & 31, run = e1 >> 5, lsym = e2 >> 5;
-          bool neg = ((w2 << l2) >> 63) != 0;
-          n += run;
-          if (n >= total) return E_RUN_PAST;
-          r.consume(l1 + l2 + 1);
-          int32_t level = (int32_t)lsym + 1;
-          out.put(n, neg ? -level : level);
-          n++;
-          prev_run = run;
-          prev_lvl = lsym;
-          continue;
-        }
-      }
+    const uint64_t w = r.window();
+    uint32_t run, l1, lsym, l2;
+    const bool okr = peek(w, run_book_of(prev_run), run, l1);
+    const uint64_t w2 = w << (l1 < 63 ? l1 : 63);
+    const bool okl = peek(w2, level_book_of(prev_lvl), lsym, l2);
+    const uint32_t T = l1 + l2 + 1;
+    const uint32_t rem = r.remaining();
+    if (okr && okl && T <= 64 && rem >= 32 && T <= rem) {
+      n += run;
+      if (n >= total) return E_RUN_PAST;
+      const bool neg = ((w2 << l2) >> 63) != 0;
+      r.consume(T);
+      const int32_t level = (int32_t)lsym + 1;
+      out.put(n, neg ? -level : level);
+      n++;
+      prev_run = run;
+      prev_lvl = lsym;
+      continue;
     }
     if (r.end_of_data()) break;
-    uint32_t run;
-    e = read_code(r, RUN_BOOK[prev_run < 15 ? prev_run : 15], run);
+    e = read_code(r, run_book_of(prev_run), run);
     if (e) return e;
     prev_run = run;
     n += run;
     if (n >= total) return E_RUN_PAST;
-    uint32_t lsym;
-    e = read_code(r, LEVEL_BOOK[prev_lvl < 8 ? prev_lvl : 8], lsym);
+    e = read_code(r, level_book_of(prev_lvl), lsym);
     if (e) return e;
     prev_lvl = lsym;
     if (r.remaining() < 1) return E_PAST_END;
-    uint32_t sgn = r.take(1);
-    int32_t level = (int32_t)lsym + 1;
+    const uint32_t sgn = r.take(1);
+    const int32_t level = (int32_t)lsym + 1;
     out.put(n, sgn ? -level : level);
     n++;
   }
   out.finish(total);
+  return E_NONE;
+}
+
+// The slice header (src/decode.rs:301-338): header size, component sizes
+// and their checks. Returns the error word, E_NONE when the header is sound.
+__device__ __forceinline__ uint32_t slice_header(const uint8_t *s, uint32_t slen, uint32_t alpha_bits,
+                                                 uint32_t &hs, uint32_t &ysz, uint32_t &cbsz,
+                                                 uint32_t &crsz) {
+  const uint32_t min_header = alpha_bits ? 8u : 6u;
+  hs = ysz = cbsz = crsz = 0;
+  if (slen < min_header) return E_SLICE_SHORT;
+  hs = s[0] >> 3;
+  const uint32_t q = s[1];
+  if (hs < min_header || hs > slen) return E_SLICE_HDR_SIZE | (hs << 8);
+  if (q < 1 || q > 224) return E_QINDEX | (q << 8);
+  ysz = ((uint32_t)s[2] << 8) | s[3];
+  cbsz = ((uint32_t)s[4] << 8) | s[5];
+  const uint32_t body = slen - hs;
+  if (alpha_bits) {
+    crsz = ((uint32_t)s[6] << 8) | s[7];
+  } else {
+    if (ysz + cbsz > body) return E_COMP_SIZES;
+    crsz = body - ysz - cbsz;
+  }
+  if (ysz + cbsz + crsz > body) return E_COMP_SIZES;
   return E_NONE;
 }
 
@@ -292,24 +378,15 @@ extern "C" __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// grid.x covers slices 256 at a time, grid.y is the stream kind: 0 Y', 1 Cb,
-// 2 Cr, 3 alpha.
+// grid.x covers slices 256 at a time, grid.y is the component: 0 Y', 1 Cb,
+// 2 Cr.
 extern "C" __global__ void __launch_bounds__(256)
     pd_entropy(const DevTables *__restrict__ T,
                const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
                const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
                uint32_t n_slices, int16_t *__restrict__ coefs, int32_t *__restrict__ coef_wide,
                uint32_t *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err) {
-  __shared__ uint16_t lut_run[16 * 512];
-  __shared__ uint16_t lut_lvl[9 * 512];
   const uint32_t comp = blockIdx.y, tid = threadIdx.x;
-  if (comp < 3) {
-    const uint4 *sr = (const uint4 *)T->run_lut;
-    const uint4 *sl = (const uint4 *)T->level_lut;
-    for (uint32_t i = tid; i < 16 * 512 / 8; i += 256) ((uint4 *)lut_run)[i] = sr[i];
-    for (uint32_t i = tid; i < 9 * 512 / 8; i += 256) ((uint4 *)lut_lvl)[i] = sl[i];
-    __syncthreads();
-  }
   const uint32_t g = blockIdx.x * 256 + tid;
   if (g >= n_slices) return;
   const SliceRec S = slices[g];
@@ -317,116 +394,206 @@ extern "C" __global__ void __launch_bounds__(256)
   const DecPic &P = pics[S.pic];
   const DecFrame &F = frames[P.frame];
   const uint32_t alpha_bits = F.alpha_bits, chroma = F.chroma;
-  if (comp == 3 && alpha_bits == 0) return;
 
-  // Slice header (src/decode.rs:301-338).
   const uint8_t *s = (const uint8_t *)S.off;
   const uint32_t slen = S.len;
-  const uint32_t min_header = alpha_bits ? 8u : 6u;
-  uint32_t err = E_NONE, hs = 0, ysz = 0, cbsz = 0, crsz = 0;
-  if (slen < min_header) {
-    err = E_SLICE_SHORT;
-  } else {
-    hs = s[0] >> 3;
-    uint32_t q = s[1];
-    if (hs < min_header || hs > slen) {
-      err = E_SLICE_HDR_SIZE | (hs << 8);
-    } else if (q < 1 || q > 224) {
-      err = E_QINDEX | (q << 8);
-    } else {
-      ysz = ((uint32_t)s[2] << 8) | s[3];
-      cbsz = ((uint32_t)s[4] << 8) | s[5];
-      uint32_t body = slen - hs;
-      if (alpha_bits) {
-        crsz = ((uint32_t)s[6] << 8) | s[7];
-      } else if (ysz + cbsz > body) {
-        err = E_COMP_SIZES;
-      } else {
-        crsz = body - ysz - cbsz;
-      }
-      if (!err && ysz + cbsz + crsz > body) err = E_COMP_SIZES;
-    }
-  }
+  uint32_t hs, ysz, cbsz, crsz;
+  uint32_t err = slice_header(s, slen, alpha_bits, hs, ysz, cbsz, crsz);
   if (!err) {
     Reader r;
-    if (comp < 3) {
-      const uint32_t cpm = chroma == 2 ? 2u : 4u;
-      const uint32_t nb = (comp == 0 ? 4u : cpm) * S.mbs;
-      const uint32_t off = comp == 0 ? hs : (comp == 1 ? hs + ysz : hs + ysz + cbsz);
-      const uint32_t len = comp == 0 ? ysz : (comp == 1 ? cbsz : crsz);
-      const uint64_t cofs = S.coef + (comp == 0 ? 0ull : (comp == 1 ? 256ull * S.mbs : (256ull + 64ull * cpm) * S.mbs));
-      r.init(s + off, len);
-      CoefOut out{coefs + cofs, coef_wide + cofs, 0, 0, 0, 0, 0};
-      err = decode_component(r, nb, out, lut_run, lut_lvl);
-    } else {
-      // scanned_alpha() (src/decode.rs:570-595) and its placement (365-389).
-      const uint32_t used = hs + ysz + cbsz + crsz;
-      r.init(s + used, slen - used);
-      const uint32_t bits = alpha_bits, mag = bits == 8 ? 3u : 6u, mask = (1u << bits) - 1;
-      const uint32_t log2c = 4 + (31 - __builtin_clz(S.mbs));
-      const uint32_t cols = 1u << log2c, count = cols * 16;
-      const uint32_t rows =
-          (uint32_t)S.mb_y + 1 < P.height_in_mb ? 16u : P.height - 16u * (P.height_in_mb - 1);
-      const uint32_t needed = cols * rows;
-      const uint32_t x0 = 16u * S.mb_x, width = F.width;
-      const uint32_t nvis = cols < width - x0 ? cols : width - x0;
-      uint16_t *aout = (uint16_t *)F.out_alpha;
-      const uint32_t ident = F.rescale_identity;
-      const uint64_t m = F.rescale_m;
-      uint32_t len = 0;
-      int32_t prev = -1;
-      while (len < count) {
-        if (len >= needed && r.end_of_data()) break;
-        int32_t d;
-        if (r.remaining() < 1) { err = E_PAST_END; break; }
-        if (r.take(1)) {
-          if (r.remaining() < bits) { err = E_PAST_END; break; }
-          d = (int32_t)r.take(bits);
-        } else {
-          if (r.remaining() < mag) { err = E_PAST_END; break; }
-          int32_t a = (int32_t)r.take(mag) + 1;
-          if (r.remaining() < 1) { err = E_PAST_END; break; }
-          d = r.take(1) ? -a : a;
-        }
-        uint32_t alpha = (uint32_t)(prev + d) & mask;
-        prev = (int32_t)alpha;
-        uint32_t run;
-        if (r.remaining() < 1) { err = E_PAST_END; break; }
-        if (r.take(1)) {
-          run = 1;
-        } else {
-          if (r.remaining() < 4) { err = E_PAST_END; break; }
-          uint32_t v = r.take(4);
-          if (v) {
-            run = v + 1;
-          } else {
-            if (r.remaining() < 11) { err = E_PAST_END; break; }
-            run = r.take(11) + 1;
-          }
-        }
-        if (len + run > count) { err = E_ALPHA_RUN; break; }
-        const uint16_t val = rescale(alpha, ident, m);
-        uint32_t end = len + run;
-        if (len < needed) {
-          uint32_t stop = end < needed ? end : needed;
-          for (uint32_t i = len; i < stop;) {
-            uint32_t row = i >> log2c, col = i & (cols - 1);
-            uint32_t seg_end = (row + 1) << log2c;
-            uint32_t e2 = seg_end < stop ? seg_end : stop;
-            uint32_t c_end = col + (e2 - i);
-            if (c_end > nvis) c_end = nvis;
-            uint16_t *o = aout + (size_t)(P.first_row + P.row_step * (16u * S.mb_y + row)) * width + x0;
-            for (uint32_t c = col; c < c_end; c++) o[c] = val;
-            i = e2;
-          }
-        }
-        len = end;
-      }
-    }
+    const uint32_t cpm = chroma == 2 ? 2u : 4u;
+    const uint32_t nb = (comp == 0 ? 4u : cpm) * S.mbs;
+    const uint32_t off = comp == 0 ? hs : (comp == 1 ? hs + ysz : hs + ysz + cbsz);
+    const uint32_t len = comp == 0 ? ysz : (comp == 1 ? cbsz : crsz);
+    const uint64_t cofs = S.coef + (comp == 0 ? 0ull : (comp == 1 ? 256ull * S.mbs : (256ull + 64ull * cpm) * S.mbs));
+    r.init(s + off, len);
+    CoefOut out{coefs + cofs, coef_wide + cofs, 0, 0, 0, 0, 0};
+    err = decode_component(r, nb, out);
   }
   if (err) {
     slice_err[4 * (size_t)g + comp] = err;
     atomicMin(&pic_first_err[S.pic], g - P.slice_base);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One wavefront per slice with alpha: scanned_alpha() (src/decode.rs:570-595)
+// and its placement (src/decode.rs:365-389). The wave stages the slice's
+// alpha stream in LDS (big-endian 64-bit words, zero past the end), every
+// lane parses the same codes from it (a broadcast read; the parse is
+// wave-uniform), runs are filled into an LDS copy of the slice's values by
+// the whole wave, and the visible rows are stored coalesced at the end. The
+// parse never waits on a global store.
+// A code is at most 33 bits (a raw difference of 1 + 16, a run of 1 + 4 +
+// 11), so 2048 values read at most 8448 bytes of the stream.
+constexpr uint32_t ALPHA_WORDS = (2048 * 33 / 8) / 8 + 6;
+
+extern "C" __global__ void __launch_bounds__(64)
+    pd_alpha(const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
+             const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
+             uint32_t *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err) {
+  __shared__ uint64_t words[ALPHA_WORDS];
+  __shared__ __attribute__((aligned(16))) uint16_t vals[2048];
+  const uint32_t g = blockIdx.x, L = threadIdx.x;
+  const SliceRec S = slices[g];
+  if (pic_status[S.pic] != PIC_OK) return;
+  const DecPic &P = pics[S.pic];
+  const DecFrame &F = frames[P.frame];
+  const uint32_t bits = F.alpha_bits;
+  if (bits == 0) return;
+  const uint8_t *s = (const uint8_t *)S.off;
+  const uint32_t slen = S.len;
+  uint32_t hs, ysz, cbsz, crsz;
+  uint32_t err = slice_header(s, slen, bits, hs, ysz, cbsz, crsz);
+  if (err) {
+    if (L == 0) {
+      slice_err[4 * (size_t)g + 3] = err;
+      atomicMin(&pic_first_err[S.pic], g - P.slice_base);
+    }
+    return;
+  }
+  // Stage the stream: aligned words from base - m, bytes outside the stream
+  // zeroed.
+  const uint32_t used = hs + ysz + cbsz + crsz, len = slen - used;
+  const uintptr_t a = (uintptr_t)(s + used);
+  const uint32_t m = (uint32_t)(a & 7);
+  const uint64_t *src = (const uint64_t *)(a - m);
+  const uint32_t end_byte = m + len;  // bytes of stream data end here
+  const uint32_t have = (end_byte + 7) / 8;
+  for (uint32_t j = L; j < ALPHA_WORDS; j += 64) {
+    uint64_t w = 0;
+    if (j < have) {
+      w = __builtin_bswap64(src[j]);
+      const uint32_t lo = 8 * j, hi = lo + 8;
+      if (lo < m) w &= ~0ull >> (8 * (m - lo));
+      if (hi > end_byte) w &= ~0ull << (8 * (hi - end_byte));
+    }
+    words[j] = w;
+  }
+  __syncthreads();
+
+  const uint32_t mag = bits == 8 ? 3u : 6u, mask = (1u << bits) - 1;
+  const uint32_t log2c = 4 + (31 - __builtin_clz(S.mbs));
+  const uint32_t cols = 1u << log2c, count = cols * 16;
+  const uint32_t rows =
+      (uint32_t)S.mb_y + 1 < P.height_in_mb ? 16u : P.height - 16u * (P.height_in_mb - 1);
+  const uint32_t needed = cols * rows;
+  const uint32_t ident = F.rescale_identity;
+  const uint64_t rm = F.rescale_m;
+  // The reader: bit `pos` of the stream is bit `off` of word `wj`.
+  uint32_t wj = 0, off = 8 * m, pos = 0;
+  const uint32_t lenbits = 8 * len;
+  uint64_t A = words[0], B = words[1];
+  auto window = [&]() -> uint64_t { return off ? (A << off) | (B >> (64 - off)) : A; };
+  auto consume = [&](uint32_t n) {  // n <= 64
+    pos += n;
+    off += n;
+    if (off >= 64) {
+      off -= 64;
+      wj++;
+      A = B;
+      B = words[wj + 1 < ALPHA_WORDS ? wj + 1 : ALPHA_WORDS - 1];
+    }
+  };
+  auto take = [&](uint32_t n) -> uint32_t {  // n <= 32
+    const uint32_t v = n ? (uint32_t)(window() >> (64 - n)) : 0u;
+    consume(n);
+    return v;
+  };
+  uint32_t n = 0;
+  int32_t prev = -1;
+  while (n < count) {
+    const uint32_t rem = lenbits - pos;
+    int32_t d;
+    uint32_t run;
+    if (rem >= 33) {
+      // Both codes lie in the window and every bit is real.
+      const uint64_t w = window();
+      uint32_t used_bits;
+      if (w >> 63) {
+        d = (int32_t)((w >> (63 - bits)) & mask);
+        used_bits = 1 + bits;
+      } else {
+        const int32_t av = (int32_t)((w >> (63 - mag)) & ((1u << mag) - 1)) + 1;
+        d = ((w >> (62 - mag)) & 1) ? -av : av;
+        used_bits = 2 + mag;
+      }
+      const uint64_t w2 = w << used_bits;
+      if (w2 >> 63) {
+        run = 1;
+        used_bits += 1;
+      } else {
+        const uint32_t v = (uint32_t)(w2 >> 59) & 15;
+        if (v) {
+          run = v + 1;
+          used_bits += 5;
+        } else {
+          run = (uint32_t)((w2 >> 48) & 0x7FF) + 1;
+          used_bits += 16;
+        }
+      }
+      consume(used_bits);
+    } else {
+      // Fewer than 33 bits left: the reference's reads, check for check.
+      if (n >= needed && (rem == 0 || (window() >> (64 - rem)) == 0)) break;  // endOfData
+      if (rem < 1) { err = E_PAST_END; break; }
+      if (take(1)) {
+        if (lenbits - pos < bits) { err = E_PAST_END; break; }
+        d = (int32_t)take(bits);
+      } else {
+        if (lenbits - pos < mag) { err = E_PAST_END; break; }
+        const int32_t av = (int32_t)take(mag) + 1;
+        if (lenbits - pos < 1) { err = E_PAST_END; break; }
+        d = take(1) ? -av : av;
+      }
+      if (lenbits - pos < 1) { err = E_PAST_END; break; }
+      if (take(1)) {
+        run = 1;
+      } else {
+        if (lenbits - pos < 4) { err = E_PAST_END; break; }
+        const uint32_t v = take(4);
+        if (v) {
+          run = v + 1;
+        } else {
+          if (lenbits - pos < 11) { err = E_PAST_END; break; }
+          run = take(11) + 1;
+        }
+      }
+    }
+    const uint32_t value = (uint32_t)(prev + d) & mask;
+    prev = (int32_t)value;
+    if (n + run > count) { err = E_ALPHA_RUN; break; }
+    const uint16_t out = rescale(value, ident, rm);
+    for (uint32_t i = n + L; i < n + run; i += 64) vals[i] = out;
+    n += run;
+  }
+  if (err) {
+    if (L == 0) {
+      slice_err[4 * (size_t)g + 3] = err;
+      atomicMin(&pic_first_err[S.pic], g - P.slice_base);
+    }
+    return;
+  }
+  __syncthreads();
+  // Visible rows, clipped to the plane width.
+  const uint32_t x0 = 16u * S.mb_x, width = F.width;
+  const uint32_t nvis = cols < width - x0 ? cols : width - x0;
+  uint16_t *aout = (uint16_t *)F.out_alpha;
+  const size_t row0 = P.first_row + (size_t)P.row_step * (16u * S.mb_y);
+  const size_t rstride = (size_t)P.row_step * width;
+  if ((width & 7) == 0 && (((uintptr_t)aout) & 15) == 0) {
+    // x0 and nvis are multiples of 8: 16-byte stores.
+    const uint32_t per_row = nvis / 8;
+    for (uint32_t c = L; c < rows * per_row; c += 64) {
+      const uint32_t r = c / per_row, k = c - r * per_row;
+      *(uint4 *)(aout + (row0 * width + r * rstride) + x0 + 8 * k) =
+          *(const uint4 *)(vals + (r << log2c) + 8 * k);
+    }
+  } else {
+    for (uint32_t i = L; i < rows * nvis; i += 64) {
+      const uint32_t r = i / nvis, k = i - r * nvis;
+      aout[(row0 * width + r * rstride) + x0 + k] = vals[(r << log2c) + k];
+    }
   }
 }
 
@@ -440,7 +607,7 @@ extern "C" __global__ void __launch_bounds__(256)
             const uint32_t *__restrict__ slice_err, const int16_t *__restrict__ coefs,
             const int32_t *__restrict__ coef_wide) {
   __shared__ __attribute__((aligned(16))) int16_t sc[96 * 64];
-  __shared__ float tmp[32 * 64];
+  __shared__ float tmp[32 * 65];
   const uint32_t g = blockIdx.x, tid = threadIdx.x;
   const SliceRec S = slices[g];
   if (pic_status[S.pic] != PIC_OK) return;
@@ -486,7 +653,7 @@ extern "C" __global__ void __launch_bounds__(256)
         float acc = 0.0f;
 #pragma unroll
         for (uint32_t u = 0; u < 8; u++) acc = acc + fu[u] * basis[8 * x + u];
-        tmp[lb * 64 + 8 * r + x] = acc;
+        tmp[lb * 65 + 8 * r + x] = acc;
       }
     }
     __syncthreads();
@@ -496,7 +663,7 @@ extern "C" __global__ void __launch_bounds__(256)
       for (uint32_t x = 0; x < 8; x++) {
         float acc = 0.0f;
 #pragma unroll
-        for (uint32_t v = 0; v < 8; v++) acc = acc + basis[8 * r + v] * tmp[lb * 64 + 8 * v + x];
+        for (uint32_t v = 0; v < 8; v++) acc = acc + basis[8 * r + v] * tmp[lb * 65 + 8 * v + x];
         float sv = floorf(acc * osc + obias);
         sv = sv < 0.0f ? 0.0f : (sv > omax ? omax : sv);
         outv[x] = (uint16_t)(uint32_t)sv;

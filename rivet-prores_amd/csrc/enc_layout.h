@@ -5,6 +5,7 @@
 namespace prores_hip {
 
 constexpr uint32_t ENC_MAXK = 8;         // window columns of the size table
+constexpr uint32_t ENC_DEEP = 24;        // cached columns past the window, per slice
 constexpr uint32_t ALPHA_SLICE_CAP = 4608;  // bytes: 2048 values x 18 bits, the worst case
 
 struct EncFrame {

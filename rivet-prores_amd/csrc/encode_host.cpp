@@ -235,7 +235,7 @@ Err plan_encode(const prores_config &cfg, const prores_frame &f, uint64_t dev_da
 }
 
 struct EncWorkspace {
-  DevBuf frames, pics, slices, states, qtabs, hdrs, coefs, alpha_scratch, alpha_len, size_tab, used, qidx,
+  DevBuf frames, pics, slices, states, qtabs, hdrs, coefs, alpha_scratch, alpha_len, size_tab, deep, used, qidx,
       slice_out, frame_size, out, planes, alpha_in;
 };
 EncWorkspace &enc_workspace(Device &d) {
@@ -270,6 +270,7 @@ Err run_encode(Device &d, EncBatch &b, std::vector<uint64_t> &offsets, std::vect
   if (!(e = ws.alpha_scratch.reserve(G * ALPHA_SLICE_CAP + 64)).ok()) return e;
   if (!(e = ws.alpha_len.reserve(G * 4 + 64)).ok()) return e;
   if (!(e = ws.size_tab.reserve(G * ENC_MAXK * 4 + 64)).ok()) return e;
+  if (!(e = ws.deep.reserve(G * ENC_DEEP * 4 + 64)).ok()) return e;
   if (!(e = ws.used.reserve(G * 4 + 64)).ok()) return e;
   if (!(e = ws.qidx.reserve(G * 4 + 64)).ok()) return e;
   if (!(e = ws.slice_out.reserve(G * 8 + 64)).ok()) return e;
@@ -277,7 +278,7 @@ Err run_encode(Device &d, EncBatch &b, std::vector<uint64_t> &offsets, std::vect
   void *frames = ws.frames.p, *pics = ws.pics.p, *slices = ws.slices.p, *states = ws.states.p;
   void *qtabs = ws.qtabs.p, *hdrs = ws.hdrs.p, *coefs = ws.coefs.p, *ascr = ws.alpha_scratch.p;
   void *alen = ws.alpha_len.p, *stab = ws.size_tab.p, *used = ws.used.p, *qidx = ws.qidx.p;
-  void *sout = ws.slice_out.p, *fsize = ws.frame_size.p, *tables = d.tables;
+  void *sout = ws.slice_out.p, *fsize = ws.frame_size.p, *tables = d.tables, *deep = ws.deep.p;
   uint32_t g32 = (uint32_t)G, np32 = NP, window = WINDOW;
   (void)hipEventRecord(d.ev[0], d.stream);
   {
@@ -286,7 +287,7 @@ Err run_encode(Device &d, EncBatch &b, std::vector<uint64_t> &offsets, std::vect
   }
   {
     void *args[] = {&frames, &pics, &slices, &g32, &ascr, &alen, &states};
-    if (!(e = launch(d, d.k_alpha_enc, (g32 + 255) / 256, 1, 256, args)).ok()) return e;
+    if (!(e = launch(d, d.k_alpha_enc, g32, 1, 64, args)).ok()) return e;
   }
   (void)hipEventRecord(d.ev[1], d.stream);
   // qi = 224, then at most 8 bisection steps (224 < 2^8).
@@ -300,8 +301,9 @@ Err run_encode(Device &d, EncBatch &b, std::vector<uint64_t> &offsets, std::vect
     void *args[] = {&frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &stab};
     if (!(e = launch(d, d.k_window, (g32 + 3) / 4, 1, 256, args)).ok()) return e;
   }
+  if (G && !(e = check(hipMemsetAsync(ws.deep.p, 0, G * ENC_DEEP * 4, d.stream), "memset")).ok()) return e;
   {
-    void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &states, &stab, &used, &qidx};
+    void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &states, &stab, &deep, &used, &qidx};
     if (!(e = launch(d, d.k_shareout, np32, 1, 64, args)).ok()) return e;
   }
   {
@@ -329,8 +331,7 @@ Err run_encode(Device &d, EncBatch &b, std::vector<uint64_t> &offsets, std::vect
     void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &ascr, &used, &qidx, &sout, &fsize, &hdrs};
     if (!(e = launch(d, d.k_pack, g32, 1, 64, args)).ok()) return e;
   }
-  hipEvent_t done;
-  (void)hipEventCreate(&done);
+  hipEvent_t done = d.ev[4];
   (void)hipEventRecord(done, d.stream);
   if (!(e = check(hipStreamSynchronize(d.stream), "hipStreamSynchronize")).ok()) return e;
   float a = 0, r = 0, p = 0, all = 0;
@@ -338,7 +339,6 @@ Err run_encode(Device &d, EncBatch &b, std::vector<uint64_t> &offsets, std::vect
   (void)hipEventElapsedTime(&r, d.ev[1], d.ev[2]);
   (void)hipEventElapsedTime(&p, d.ev[3], done);
   (void)hipEventElapsedTime(&all, d.ev[0], done);
-  (void)hipEventDestroy(done);
   g_enc_us[0] = a * 1000.0;
   g_enc_us[1] = r * 1000.0;
   g_enc_us[2] = p * 1000.0;

@@ -4,8 +4,8 @@
 //                f32 forward DCT in the reference's order (src/dct.rs:70-93,
 //                src/dsp/scalar.rs:14-17), scattered to scanned-interleaved
 //                order per component (src/encode.rs:518-555).
-//   pe_alpha     one lane per slice: alpha rescaled and run/difference coded
-//                (src/encode.rs:838-868, 457-470).
+//   pe_alpha     one wavefront per slice: alpha rescaled and run/difference
+//                coded in parallel (src/encode.rs:838-868, 457-470).
 //   pe_size      one wavefront per slice: bytes at the picture's pending probe
 //                qi (quantise + exact bit count, src/encode.rs:671-681,
 //                398-409), summed per picture.
@@ -15,7 +15,8 @@
 //                one pass over the coefficients.
 //   pe_shareout  one wavefront per picture: the sequential slice-by-slice
 //                share-out (src/encode.rs:769-793) solved 64 slices at a time
-//                by fixed-point iteration of its triangular recurrence.
+//                by fixed-point iteration of its triangular recurrence; sizes
+//                past the table are coded by the whole wave and cached.
 //   pe_offsets   one workgroup per frame: slice and picture offsets.
 //   pe_pack      one wavefront per slice: quantise at the chosen qi and place
 //                every codeword with a wave prefix sum over their lengths.
@@ -231,49 +232,6 @@ __device__ void slice_sizes(const float *__restrict__ coef, uint32_t mbs, uint32
     if (size[k] > 65535) fits[k] = false;
 }
 
-// One lane's exact size of a slice at `qi`, following coefficient_bits
-// sequentially (src/encode.rs:398-409); for the share-out's rare steps past
-// the window.
-__device__ void slice_size_serial(const float *__restrict__ coef, uint32_t mbs, uint32_t cpm,
-                                  const float *__restrict__ qt, uint32_t qi, uint32_t hb,
-                                  uint32_t alen, uint32_t &size, bool &fits) {
-  size = hb + alen;
-  fits = true;
-  for (uint32_t c = 0; c < 3; c++) {
-    uint32_t nb, off;
-    component(c, mbs, cpm, nb, off);
-    const uint32_t lg = ilog2(nb);
-    const float *cf = coef + off;
-    const float *inv = qt + (qi * 2 + (c > 0 ? 1 : 0)) * 64;
-    uint32_t bits = 0;
-    int32_t prev_dc = quant(cf[0], inv[0], 0.5f), prev_diff = 3;
-    bits += book_len(FIRST_DC, signed_to_symbol(prev_dc));
-    for (uint32_t i = 1; i < nb; i++) {
-      int32_t dc = quant(cf[i], inv[0], 0.5f);
-      int32_t diff = (int32_t)((uint32_t)dc - (uint32_t)prev_dc);
-      int32_t n = prev_diff < 0 ? (int32_t)(0u - (uint32_t)diff) : diff;
-      uint32_t a = prev_diff < 0 ? 0u - (uint32_t)prev_diff : (uint32_t)prev_diff;
-      bits += book_len(dc_book(a < 3 ? a : 3), signed_to_symbol(n));
-      prev_diff = diff;
-      prev_dc = dc;
-    }
-    uint32_t prun = 4, plvl = 1, pos = nb;
-    for (uint32_t i = nb; i < nb * 64; i++) {
-      int32_t q = quant(cf[i], inv[i >> lg], 0.42f);
-      if (!q) continue;
-      uint32_t run = i - pos, lvl = (q < 0 ? 0u - (uint32_t)q : (uint32_t)q) - 1;
-      bits += book_len(run_book(prun < 15 ? prun : 15), run) + book_len(lvl_book(plvl < 8 ? plvl : 8), lvl) + 1;
-      prun = run;
-      plvl = lvl;
-      pos = i + 1;
-    }
-    uint32_t bytes = (bits + 7) / 8;
-    size += bytes;
-    if (bytes > 65535) fits = false;
-  }
-  if (size > 65535) fits = false;
-}
-
 __device__ __forceinline__ uint16_t rescale(uint32_t a, uint32_t identity, uint64_t m) {
   return identity ? (uint16_t)a : (uint16_t)(((uint64_t)a * m + (1ull << 39)) >> 40);
 }
@@ -352,81 +310,116 @@ extern "C" __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(256)
+// One wavefront per slice. slice_alpha (src/encode.rs:838-868): the slice's
+// whole macroblocks, 16 rows of 16 * mbs values, the last column and row
+// repeated past the picture, rescaled to the coded depth; then encode_alpha
+// (src/encode.rs:457-470) in parallel: every lane codes the runs that start
+// in its 1/64 of the values (a run ends at the next start, found across lanes
+// with a suffix minimum), a prefix sum places each lane's bits, and the bits
+// are OR-ed into an LDS buffer that is copied out word by word.
+extern "C" __global__ void __launch_bounds__(64)
     pe_alpha(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
              const EncSlice *__restrict__ slices, uint32_t n_slices, uint8_t *__restrict__ scratch,
              uint32_t *__restrict__ alpha_len, EncState *__restrict__ state) {
-  const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+  __shared__ uint16_t vals[2048];
+  __shared__ uint32_t buf[ALPHA_SLICE_CAP / 4 + 4];
+  const uint32_t g = blockIdx.x, L = threadIdx.x;
   if (g >= n_slices) return;
   const EncSlice S = slices[g];
   const EncPic &P = pics[S.pic];
   const EncFrame &F = frames[P.frame];
   if (F.alpha_bits == 0) {
-    alpha_len[g] = 0;
+    if (L == 0) alpha_len[g] = 0;
     return;
   }
-  // slice_alpha (src/encode.rs:838-868): whole macroblocks, 16 rows of
-  // 16 * mbs values, the last column and row repeated past the picture.
   const uint32_t bits = F.alpha_bits, mag = bits == 8 ? 3u : 6u, maxd = bits == 8 ? 8u : 64u;
   const uint32_t lg = 4 + ilog2(S.mbs), cols = 1u << lg, count = 16 * cols;
   const uint32_t x0 = 16u * S.mb_x, width = F.width;
   const uint16_t *alpha = (const uint16_t *)F.in_alpha;
   const uint32_t ident = F.rescale_identity, max_in = F.max_in;
   const uint64_t m = F.rescale_m;
-  auto value = [&](uint32_t i) -> uint32_t {
+  for (uint32_t i = L; i < count; i += 64) {
     const uint32_t rr = i >> lg, n = i & (cols - 1);
     const uint32_t pr = 16u * S.mb_y + rr < P.height - 1 ? 16u * S.mb_y + rr : P.height - 1;
     const uint32_t row = P.first_row + P.row_step * pr;
     const uint32_t col = x0 + n < width - 1 ? x0 + n : width - 1;
     uint32_t a = alpha[(size_t)row * width + col];
     a = a < max_in ? a : max_in;
-    return rescale(a, ident, m);
+    vals[i] = rescale(a, ident, m);
+  }
+  for (uint32_t i = L; i < ALPHA_SLICE_CAP / 4 + 4; i += 64) buf[i] = 0;
+  __syncthreads();
+  // count <= 2048, so a run never reaches the 2048 cap of src/encode.rs:462.
+  const uint32_t seg = count / 64, lo = L * seg, hi = lo + seg;
+  uint32_t first = count;
+  for (uint32_t i = lo; i < hi; i++)
+    if (i == 0 || vals[i] != vals[i - 1]) {
+      first = i;
+      break;
+    }
+  // The first run start in the lanes after this one: an inclusive suffix
+  // minimum over the lanes, shifted down by one lane.
+  uint32_t incl = first;
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_down(incl, d, 64);
+    if (L + d < 64) incl = o < incl ? o : incl;
+  }
+  uint32_t next = __shfl_down(incl, 1, 64);
+  if (L == 63) next = count;
+  // Bits of this lane's runs: a difference code and a run code each
+  // (src/vlc.rs:254-287).
+  auto diff_len = [&](uint32_t i) -> uint32_t {
+    const int32_t prev = i ? (int32_t)vals[i - 1] : -1;
+    const int32_t d = (int32_t)vals[i] - prev;
+    const uint32_t ad = d < 0 ? (uint32_t)(-d) : (uint32_t)d;
+    return (ad >= 1 && ad <= maxd) ? 2 + mag : 1 + bits;
   };
-  uint32_t *out = (uint32_t *)(scratch + (size_t)g * ALPHA_SLICE_CAP);
-  uint64_t acc = 0;
-  uint32_t nacc = 0, nw = 0, total = 0;
+  auto run_len = [](uint32_t run) -> uint32_t { return run == 1 ? 1u : (run <= 16 ? 5u : 16u); };
+  uint32_t mybits = 0;
+  for (uint32_t i = first; i < hi;) {
+    uint32_t j = i + 1;
+    while (j < hi && vals[j] == vals[i]) j++;
+    const uint32_t end = j < hi ? j : next;
+    mybits += diff_len(i) + run_len(end - i);
+    i = j;
+  }
+  const uint32_t off = excl_scan(mybits, L);
+  const uint32_t total = __shfl(off + mybits, 63, 64);
+  uint32_t o = off;
   auto put = [&](uint32_t v, uint32_t n) {  // n <= 17
-    acc = (acc << n) | (v & ((1u << n) - 1));
-    nacc += n;
-    total += n;
-    if (nacc >= 32) {
-      nacc -= 32;
-      out[nw++] = __builtin_bswap32((uint32_t)(acc >> nacc));
-    }
+    const uint32_t sh = o & 31, w = o >> 5;
+    const uint64_t X = (uint64_t)v << (64 - n);
+    atomicOr(&buf[w], (uint32_t)(X >> (32 + sh)));
+    if (n + sh > 32) atomicOr(&buf[w + 1], (uint32_t)(X >> sh));
+    o += n;
   };
-  // encode_alpha (src/encode.rs:457-470) with the codes of src/vlc.rs:254-287.
-  int32_t prev = -1;
-  uint32_t i = 0, cur = value(0);
-  while (i < count) {
-    const uint32_t a = cur;
-    uint32_t run = 1;
-    while (i + run < count) {
-      const uint32_t nv = value(i + run);
-      if (nv != a) {
-        cur = nv;
-        break;
-      }
-      run++;
-    }
-    const int32_t diff = (int32_t)a - prev;
-    const uint32_t ad = diff < 0 ? (uint32_t)(-diff) : (uint32_t)diff;
+  for (uint32_t i = first; i < hi;) {
+    uint32_t j = i + 1;
+    while (j < hi && vals[j] == vals[i]) j++;
+    const uint32_t end = j < hi ? j : next, run = end - i;
+    const int32_t prev = i ? (int32_t)vals[i - 1] : -1;
+    const int32_t d = (int32_t)vals[i] - prev;
+    const uint32_t ad = d < 0 ? (uint32_t)(-d) : (uint32_t)d;
     if (ad >= 1 && ad <= maxd)
-      put((((ad - 1) << 1) | (diff < 0 ? 1u : 0u)), 1 + mag + 1);
+      put(((ad - 1) << 1) | (d < 0 ? 1u : 0u), 2 + mag);
     else
-      put((1u << bits) | ((uint32_t)diff & ((1u << bits) - 1)), 1 + bits);
+      put((1u << bits) | ((uint32_t)d & ((1u << bits) - 1)), 1 + bits);
     if (run == 1)
       put(1, 1);
     else if (run <= 16)
       put(run - 1, 5);
     else
       put(run - 1, 16);
-    prev = (int32_t)a;
-    i += run;
+    i = j;
   }
-  if (total & 7) put(0, 8 - (total & 7));
-  if (nacc) out[nw] = __builtin_bswap32((uint32_t)(acc << (32 - nacc)));
-  alpha_len[g] = total / 8;
-  atomicAdd(&state[S.pic].alpha_total, total / 8);
+  __syncthreads();
+  const uint32_t bytes = (total + 7) / 8;  // zero bits to the byte boundary
+  uint32_t *out = (uint32_t *)(scratch + (size_t)g * ALPHA_SLICE_CAP);
+  for (uint32_t w = L; w < (bytes + 3) / 4; w += 64) out[w] = __builtin_bswap32(buf[w]);
+  if (L == 0) {
+    alpha_len[g] = bytes;
+    atomicAdd(&state[S.pic].alpha_total, bytes);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -540,7 +533,7 @@ extern "C" __global__ void __launch_bounds__(64)
                 const EncSlice *__restrict__ slices, const float *__restrict__ coefs,
                 const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
                 EncState *__restrict__ state, const uint32_t *__restrict__ size_tab,
-                uint32_t *__restrict__ used, uint32_t *__restrict__ qidx) {
+                uint32_t *__restrict__ deep, uint32_t *__restrict__ used, uint32_t *__restrict__ qidx) {
   const uint32_t p = blockIdx.x, L = threadIdx.x;
   const EncPic &P = pics[p];
   EncState &st = state[p];
@@ -564,7 +557,7 @@ extern "C" __global__ void __launch_bounds__(64)
   total_mbs = wave_sum64(total_mbs);
   int64_t slack = (int64_t)(st.budget - sum_base);
   uint64_t mbs_before = 0;
-  uint32_t fallbacks = 0;
+  uint32_t computed = 0;
   for (uint32_t c0 = 0; c0 < n; c0 += 64) {
     const uint32_t i = c0 + L;
     const bool valid = i < n;
@@ -575,33 +568,56 @@ extern "C" __global__ void __launch_bounds__(64)
     int64_t s_i = slack, extra = 0;
     uint32_t q_i = U, u_i = base;
     for (uint32_t round = 0; round <= 64; round++) {
-      // Slice i's own steps for a given slack (src/encode.rs:773-789).
-      extra = 0;
-      q_i = U;
-      u_i = base;
-      if (valid) {
-        const uint64_t sc = s_i < 0 ? 0ull : (uint64_t)s_i;
-        const uint64_t allowed = base + sc * m / M;
-        uint32_t qi = U;
-        while (qi > 1) {
-          const uint32_t t = U - qi + 1;
-          uint32_t sz;
-          bool fits;
-          if (t < K) {
-            const uint32_t v = size_tab[(size_t)g * ENC_MAXK + t];
-            sz = v & 0x7FFFFFFFu;
-            fits = (v >> 31) == 0;
+      // Slice i's own steps for a given slack (src/encode.rs:773-789): it
+      // steps to finer quantisers while the next size fits its share.
+      const uint64_t sc = s_i < 0 ? 0ull : (uint64_t)s_i;
+      const uint64_t allowed = valid ? base + sc * m / M : 0ull;
+      uint32_t qi = U, u = base, pv = 0;
+      bool walking = valid, have_pv = false;
+      while (__ballot(walking)) {
+        bool need = false;
+        if (walking) {
+          const uint32_t t = U - qi + 1;  // the column of qi - 1
+          uint32_t v = 0;
+          if (t < K)
+            v = size_tab[(size_t)g * ENC_MAXK + t];
+          else if (have_pv)
+            v = pv;
+          else if (t - K < ENC_DEEP)
+            v = deep[(size_t)g * ENC_DEEP + (t - K)];
+          have_pv = false;
+          if (v == 0) {
+            need = true;
+          } else if ((v >> 31) || (v & 0x7FFFFFFFu) > allowed) {
+            walking = false;
           } else {
-            slice_size_serial(coefs + slices[g].coef, m, cpm, qt, qi - 1, P.header_bytes, alpha_len[g], sz, fits);
-            fallbacks++;
+            qi--;
+            u = v & 0x7FFFFFFFu;
+            if (qi <= 1) walking = false;
           }
-          if (!(fits && sz <= allowed)) break;
-          qi--;
-          u_i = sz;
         }
-        q_i = qi;
-        extra = (int64_t)u_i - (int64_t)base;
+        // Sizes past the table: the whole wave codes them, one slice at a time.
+        uint64_t mk = __ballot(need);
+        while (mk) {
+          const int pl = __builtin_ctzll(mk);
+          mk &= mk - 1;
+          const uint32_t gp = __shfl(g, pl, 64), qp = __shfl(qi, pl, 64) - 1;
+          uint32_t sz[1];
+          bool ft[1];
+          slice_sizes<1>(coefs + slices[gp].coef, slices[gp].mbs, cpm, qt, &qp, 1, P.header_bytes,
+                         alpha_len[gp], sz, ft);
+          computed++;
+          if ((int)L == pl) {
+            pv = sz[0] | (ft[0] ? 0u : 0x80000000u);
+            have_pv = true;
+            const uint32_t t = U - qp;
+            if (t - K < ENC_DEEP) deep[(size_t)gp * ENC_DEEP + (t - K)] = pv;
+          }
+        }
       }
+      q_i = qi;
+      u_i = u;
+      extra = valid ? (int64_t)u - (int64_t)base : 0;
       const int64_t s_new = slack - excl_scan64(extra, L);
       if (!__ballot(valid && s_new != s_i)) break;
       s_i = s_new;
@@ -619,8 +635,7 @@ extern "C" __global__ void __launch_bounds__(64)
     slack -= ext;
     mbs_before += wave_sum64(m);
   }
-  fallbacks = wave_sum(fallbacks);
-  if (L == 0) st.fallback = fallbacks;
+  if (L == 0) st.fallback = computed;
 }
 
 // ---------------------------------------------------------------------------

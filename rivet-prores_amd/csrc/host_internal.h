@@ -70,13 +70,13 @@ Err parse_picture_header(const uint8_t *d, size_t len, PicHeader &out);
 struct Device {
   int id = -1;
   hipModule_t dec_mod = nullptr, enc_mod = nullptr;
-  hipFunction_t k_slice_table = nullptr, k_entropy = nullptr, k_idct = nullptr;
+  hipFunction_t k_slice_table = nullptr, k_entropy = nullptr, k_alpha = nullptr, k_idct = nullptr;
   hipFunction_t k_fdct = nullptr, k_alpha_enc = nullptr, k_size = nullptr, k_search = nullptr,
                 k_shareout = nullptr, k_offsets = nullptr, k_pack = nullptr,
                 k_window = nullptr;
   DevTables *tables = nullptr;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   std::mutex mu;  // one batch at a time per device
 };
 // The context of the current device, created on first use.

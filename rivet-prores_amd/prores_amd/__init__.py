@@ -381,27 +381,37 @@ class Decoder:
                               m.matrix_coefficients))
 
     def decode_device_batch(self, jobs):
-        """Decodes device-resident packets: ``jobs`` is a list of
-        (host_packet bytes, dev_packet ptr, dev_data ptr, dev_alpha ptr or 0).
-        Returns one Error or None per job."""
-        n = len(jobs)
-        arr = (_DevDecodeJob * n)()
-        keep = []
-        for i, (hp, dp, dd, da) in enumerate(jobs):
-            hb = C.create_string_buffer(bytes(hp), len(hp))
-            keep.append(hb)
-            arr[i].host_packet = C.addressof(hb)
-            arr[i].len = len(hp)
-            arr[i].dev_packet, arr[i].dev_data, arr[i].dev_alpha = dp, dd, da or None
-        errs = (_Err * n)()
-        rc = lib().prores_decode_device_batch(self._h, n, arr, errs)
+        """Decodes device-resident packets: ``jobs`` is a DecodeJobs or a list
+        of (host_packet bytes, dev_packet ptr, dev_data ptr, dev_alpha ptr or
+        0). Returns one Error or None per job."""
+        if not isinstance(jobs, DecodeJobs):
+            jobs = DecodeJobs(jobs)
+        rc = lib().prores_decode_device_batch(self._h, jobs.n, jobs.arr, jobs.errs)
         if rc == 4:
-            _raise(errs[0])
-        return [None if e.kind == 0 else _KINDS.get(e.kind, Error)(e.msg.decode()) for e in errs]
+            _raise(jobs.errs[0])
+        return [None if e.kind == 0 else _KINDS.get(e.kind, Error)(e.msg.decode()) for e in jobs.errs]
+
+
+class DecodeJobs:
+    """A prepared list of device decode jobs (the host packets are referenced,
+    not copied), for repeated batches."""
+
+    def __init__(self, jobs):
+        self.n = len(jobs)
+        self.arr = (_DevDecodeJob * self.n)()
+        self.errs = (_Err * self.n)()
+        self._keep = []
+        for i, (hp, dp, dd, da) in enumerate(jobs):
+            hp = bytes(hp)
+            self._keep.append(hp)
+            self.arr[i].host_packet = C.cast(C.c_char_p(hp), C.c_void_p)
+            self.arr[i].len = len(hp)
+            self.arr[i].dev_packet, self.arr[i].dev_data, self.arr[i].dev_alpha = dp, dd, da or None
 
 
 def last_batch_timing():
-    us = (C.c_double * 4)()
+    """[slice table, colour entropy, alpha, idct, all kernels] in microseconds."""
+    us = (C.c_double * 5)()
     lib().prores_last_batch_timing(us)
     return list(us)
 
