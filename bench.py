@@ -123,6 +123,9 @@ def main():
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibration-copy-mb", type=int, default=0,
+                    help="after the timed steps, one device-to-device copy of this many MiB (a kernel of "
+                         "known traffic, for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -171,6 +174,14 @@ def main():
         plain()
     plain_s = (time.perf_counter() - t0) / args.steps
     log(f"bench: plain {plain_s * 1e3:.2f} ms/step")
+
+    if args.calibration_copy_mb:
+        nbytes = args.calibration_copy_mb << 20
+        ca, cb = P.DeviceBuffer(nbytes), P.DeviceBuffer(nbytes)
+        P.copy_device(cb.ptr, ca.ptr, nbytes)
+        log(f"bench: calibration copy of {nbytes} bytes")
+        ca.free()
+        cb.free()
 
     # Algorithmic bytes (the packet read, the planes written).
     packet_bytes = sum(len(p) for p in packets)

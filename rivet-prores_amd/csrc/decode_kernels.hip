@@ -130,26 +130,6 @@ struct Reader {
   }
 };
 
-// A codeword decoded from the window `w` without branches: the combination
-// code (lastRiceQ, kRice, kExp) of RDD 36 §7.1.1.1. False when the prefix is
-// over 32 zeros, the codeword is over 57 bits or the value over 2^30 — the
-// general reader then says which (src/vlc.rs:179-204).
-__device__ __forceinline__ bool peek(uint64_t w, uint32_t book, uint32_t &sym, uint32_t &len) {
-  const uint32_t lq = book & 15, kr = (book >> 4) & 15, ke = book >> 8;
-  const uint32_t q = clz64(w);
-  const uint32_t qs = q < 63 ? q : 63;
-  const uint32_t lr = qs + 1 + kr;
-  const uint32_t sym_r = (qs << kr) | (uint32_t)((w >> (64 - (lr < 63 ? lr : 63))) & ((1u << kr) - 1));
-  uint32_t tail = qs + ke - lq;  // q - (lq + 1) + ke + 1
-  tail = tail < 1 ? 1 : (tail > 63 ? 63 : tail);
-  const uint64_t v = (w << qs) >> (64 - tail);
-  const uint64_t sym_e = v - (1ull << ke) + ((uint64_t)(lq + 1) << kr);
-  const bool rice = q <= lq;
-  len = rice ? lr : qs + tail;
-  sym = rice ? sym_r : (uint32_t)sym_e;
-  return rice || (q <= MAX_PREFIX && qs + tail <= 57 && sym_e <= MAX_SYMBOL);
-}
-
 // Codebook::read with the reference's checks, in its order
 // (src/vlc.rs:206-230 over src/bits.rs:59-94).
 __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t &sym) {
@@ -214,7 +194,7 @@ struct CoefOut {
 };
 
 // Codebook parameters of the run and level contexts, 12 bits each as in
-// RUN_BOOK / LEVEL_BOOK, packed for a select-free lookup.
+// RUN_BOOK / LEVEL_BOOK, for the general reader.
 __device__ __forceinline__ uint32_t run_book_of(uint32_t prev) {
   const uint32_t c = prev < 15 ? prev : 15;
   // contexts 0-1: (2,0,1); 2-3: (1,0,1); 4: (0,0,1); 5-8: (1,1,2); 9-14: (0,1,2); 15: (0,2,3)
@@ -226,12 +206,40 @@ __device__ __forceinline__ uint32_t level_book_of(uint32_t prev) {
   return c == 0 ? 0x202u : c == 1 ? 0x101u : c == 2 ? 0x102u : c == 3 ? 0x100u : c < 8 ? 0x210u : 0x320u;
 }
 
+// The same codebooks for the fast decode, 4 bits per context: lastRiceQ in
+// bits 0-1, kRice in bits 2-3; kExp = kRice + 1, except level context 0 whose
+// kExp is 2.
+constexpr uint64_t RUN_NIB = 0x8444444555501122ull;  // contexts 0..15
+constexpr uint64_t LVL_NIB = 0x844440212ull;         // contexts 0..8
+constexpr uint32_t DC_NIB = 0xC940u;                 // contexts 0..3
+
+// A combination code (lq, kr, ke) from the top of a 32-bit window `top`
+// (RDD 36 §7.1.1.1): q leading zeros; if q <= lq, a Rice code of q + 1 + kr
+// bits with value (q << kr) + r; else an exp-Golomb code of 2q - lq + ke bits
+// whose trailing 1 + (q - lq - 1) + ke bits v give v - 2^ke + ((lq + 1) << kr).
+// Both read as the code's leading bits p minus a base: p - 2^k + (q' << kr).
+// True when the code lies in the 32 bits and its value is at most 2^30; the
+// general reader takes every other case and reports its error.
+__device__ __forceinline__ bool fast_code(uint32_t top, uint32_t lq, uint32_t kr, uint32_t ke,
+                                          uint32_t &sym, uint32_t &len) {
+  const uint32_t q = top ? (uint32_t)__builtin_clz(top) : 32u;
+  const bool rice = q <= lq;
+  const uint32_t L = rice ? q + 1 + kr : 2 * q - lq + ke;
+  const uint32_t p = top >> ((32 - L) & 31);
+  const uint32_t k = rice ? kr : ke;
+  const uint32_t qq = rice ? q : lq + 1;
+  sym = p - (1u << k) + (qq << kr);
+  len = L;
+  return L <= 32 && sym <= MAX_SYMBOL;
+}
+
 // scanned_coefficients() of one colour component (src/decode.rs:484-540).
-// The common case decodes run, level and sign from one window without
-// branches; it is taken when both codewords are valid, lie in the window and
-// 32 or more bits remain (so endOfData cannot hold and every bit read is
-// real), which gives the reference's slow path's result. Otherwise the lane
-// follows the slow path check for check.
+// The loop takes the fast decode whenever the codes lie in the window and
+// inside the stream, which gives the reference's result: every check it
+// makes passes. The window reads zero past the end of the stream, so
+// endOfData (at most 31 bits left, all zero) is "top word zero and at most
+// 31 bits left". Every other case goes through the general reader, check for
+// check.
 __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out) {
   uint32_t sym;
   uint32_t e = read_code(r, FIRST_DC_BOOK, sym);
@@ -241,13 +249,14 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out) {
   int32_t prev_diff = 3;
   for (uint32_t i = 1; i < nb; i++) {
     const uint32_t a = prev_diff < 0 ? (uint32_t)(-prev_diff) : (uint32_t)prev_diff;
-    const uint32_t book = DC_BOOK[a < 3 ? a : 3];
-    const uint64_t w = r.window();
+    const uint32_t c = a < 3 ? a : 3;
+    const uint32_t nib = (DC_NIB >> (4 * c)) & 15;
+    const uint32_t top = (uint32_t)(r.window() >> 32);
     uint32_t len;
-    if (r.remaining() >= 32 && peek(w, book, sym, len) && len <= r.remaining()) {
+    if (fast_code(top, nib & 3, nib >> 2, (nib >> 2) + 1, sym, len) && len <= r.remaining()) {
       r.consume(len);
     } else {
-      e = read_code(r, book, sym);
+      e = read_code(r, DC_BOOK[c], sym);
       if (e) return e;
     }
     const int32_t n = symbol_to_signed(sym);
@@ -262,16 +271,21 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out) {
   uint32_t n = nb, prev_run = 4, prev_lvl = 1;
   for (;;) {
     const uint64_t w = r.window();
-    uint32_t run, l1, lsym, l2;
-    const bool okr = peek(w, run_book_of(prev_run), run, l1);
-    const uint64_t w2 = w << (l1 < 63 ? l1 : 63);
-    const bool okl = peek(w2, level_book_of(prev_lvl), lsym, l2);
-    const uint32_t T = l1 + l2 + 1;
+    const uint32_t top = (uint32_t)(w >> 32);
     const uint32_t rem = r.remaining();
-    if (okr && okl && T <= 64 && rem >= 32 && T <= rem) {
+    if (top == 0 && rem <= 31) break;  // endOfData
+    const uint32_t rc = prev_run < 15 ? prev_run : 15, lc = prev_lvl < 8 ? prev_lvl : 8;
+    const uint32_t rn = (uint32_t)(RUN_NIB >> (4 * rc)) & 15;
+    const uint32_t ln = (uint32_t)(LVL_NIB >> (4 * lc)) & 15;
+    uint32_t run, l1, lsym, l2;
+    const bool okr = fast_code(top, rn & 3, rn >> 2, (rn >> 2) + 1, run, l1);
+    const uint64_t w2 = w << (l1 & 63);
+    const bool okl = fast_code((uint32_t)(w2 >> 32), ln & 3, ln >> 2, (ln >> 2) + (lc == 0 ? 2u : 1u), lsym, l2);
+    const uint32_t T = l1 + l2 + 1;
+    if (okr && okl && T <= 64 && T <= rem) {
       n += run;
       if (n >= total) return E_RUN_PAST;
-      const bool neg = ((w2 << l2) >> 63) != 0;
+      const bool neg = ((w2 >> (63 - l2)) & 1) != 0;
       r.consume(T);
       const int32_t level = (int32_t)lsym + 1;
       out.put(n, neg ? -level : level);
