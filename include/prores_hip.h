@@ -172,9 +172,44 @@ typedef struct {
 
 /* Decodes n frames with one launch sequence on the library's stream; waits
  * for completion. errs[i] receives frame i's result (kind 0 = Ok). Returns
- * PRORES_ERR_DEVICE only for runtime failures, else PRORES_OK. */
+ * PRORES_ERR_DEVICE only for runtime failures, else PRORES_OK. Uses the
+ * device's batch tuning (the defaults until prores_set_decode_tuning or
+ * prores_autotune_decode sets it). No counterpart in the reference: the
+ * batched form of Decoder::decode (src/decode.rs:61-104). */
 prores_status prores_decode_device_batch(const prores_decoder *d, size_t n,
                                          const prores_dev_decode_job *jobs, prores_error *errs);
+
+/* How a decode batch is launched. Every setting gives the same samples and
+ * the same errors; only the speed differs. No counterpart in the reference.
+ *   persistent   0: one workgroup (entropy: one wave) per work item;
+ *                1: a grid the size of the device takes items from a device
+ *                   work queue until it is empty.
+ *   chunk        items taken per queue pop (persistent form), 1..64.
+ *   idct_blocks  8x8 blocks per IDCT workgroup item: consecutive slices of
+ *                one picture are packed up to this many blocks, 8..96.
+ *   entropy_wg   threads per entropy workgroup: 64, 128 or 256.
+ *   graph        1: the launch sequence is captured once as a hipGraph and
+ *                replayed while the workspace and launch shapes stay the
+ *                same (with persistent = 1 the shapes never change). */
+typedef struct {
+  uint32_t persistent, chunk, idct_blocks, entropy_wg, graph;
+} prores_decode_tuning;
+
+void prores_decode_tuning_default(prores_decode_tuning *t);
+prores_status prores_decode_device_batch_tuned(const prores_decoder *d, size_t n,
+                                               const prores_dev_decode_job *jobs,
+                                               const prores_decode_tuning *t, prores_error *errs);
+prores_status prores_set_decode_tuning(const prores_decode_tuning *t, prores_error *err);
+void prores_get_decode_tuning(prores_decode_tuning *t);
+
+/* Times every candidate tuning on this batch (`reps` timed runs each after
+ * one untimed run) and makes the fastest the device's batch tuning. best /
+ * best_us (median microseconds of the whole launch sequence) may be NULL.
+ * The candidates: both forms, chunk 1 and 4, idct_blocks 32, 64 and 96,
+ * entropy_wg 64 and 256, with and without graph replay. */
+prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,
+                                     uint32_t reps, prores_decode_tuning *best, double *best_us,
+                                     prores_error *err);
 
 /* One frame to encode from device memory: frame.data / frame.alpha are
  * DEVICE pointers laid out as frame.planes say; the rest of `frame` is host

@@ -56,8 +56,23 @@ struct Batch {
   bool any_alpha = false;
 };
 
+// A captured launch sequence and what it was captured against: the kernel
+// arguments are workspace addresses and launch shapes, so it replays while
+// those stay the same.
+struct GraphKey {
+  void *ptrs[13];
+  unsigned grids[4];
+  prores_decode_tuning t;
+  bool alpha;
+  bool operator==(const GraphKey &o) const { return memcmp(this, &o, sizeof *this) == 0; }
+};
+
 struct Workspace {
-  DevBuf frames, pics, rows, slices, status, first_err, slice_err, coefs, wide, packet, planes;
+  DevBuf frames, pics, rows, slices, status, first_err, slice_err, coefs, wide, packet, planes, hdr, items, segs;
+  hipGraphExec_t graph = nullptr;
+  GraphKey key;
+  bool tuned = false;
+  prores_decode_tuning tuning;  // the batch path's settings (the autotuner sets them)
 };
 Workspace &workspace(Device &d) {
   static Workspace ws[64];
@@ -174,53 +189,162 @@ Err upload(Device &d, DevBuf &buf, const std::vector<T> &v) {
                "hipMemcpyAsync(plan)");
 }
 
-// Runs the three kernels over a planned batch and turns the device's error
-// records into one Result per frame (`results`, in frame order).
-Err run_batch(Device &d, Batch &b, std::vector<Err> &results) {
+// Workgroups of kernel `f` that fit on the device at once (persistent grids).
+unsigned resident_grid(Device &d, hipFunction_t f, unsigned block) {
+  int per_cu = 0, cus = 0;
+  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, (int)block, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d.id) != hipSuccess || cus < 1) cus = 1;
+  return (unsigned)(per_cu * cus);
+}
+
+// IDCT row segments: runs of consecutive slices of a row holding at most
+// `cap` blocks (a larger slice is a segment of its own) and 16 slices. Every
+// row of a picture is cut the same way; returns the batch's item count.
+uint64_t plan_segments(Batch &b, uint32_t cap, std::vector<IdctItem> &segs) {
+  segs.clear();
+  uint64_t items = 0;
+  for (DecPic &P : b.pics) {
+    const uint32_t cpm = b.frames[P.frame].chroma == 2 ? 2u : 4u;
+    P.seg_base = (uint32_t)segs.size();
+    P.item_base = (uint32_t)items;
+    IdctItem cur{0, 0, 0, 0};
+    for (uint32_t j = 0; j < P.row_slices; j++) {
+      const uint32_t tb = b.rows[P.layout_base + j].mbs * (4 + 2 * cpm);
+      if (cur.count && (cur.blocks + tb > cap || cur.count == 16)) {
+        segs.push_back(cur);
+        cur = IdctItem{j, 0, 0, 0};
+      }
+      cur.count++;
+      cur.blocks += tb;
+    }
+    segs.push_back(cur);
+    P.n_segs = (uint32_t)segs.size() - P.seg_base;
+    items += (uint64_t)P.height_in_mb * P.n_segs;
+  }
+  return items;
+}
+
+Err validate_tuning(const prores_decode_tuning &t) {
+  if (t.persistent > 1 || t.graph > 1 || t.chunk < 1 || t.chunk > 64 || t.idct_blocks < 8 ||
+      t.idct_blocks > IDCT_MAX_BLOCKS || (t.entropy_wg != 64 && t.entropy_wg != 128 && t.entropy_wg != 256))
+    return config_error("decode tuning out of range");
+  return Err{};
+}
+
+// Runs the kernels over a planned batch and turns the device's error records
+// into one Result per frame (`results`, in frame order).
+Err run_batch(Device &d, Batch &b, const prores_decode_tuning &t, std::vector<Err> &results) {
   Workspace &ws = workspace(d);
   Err e;
+  if (!(e = validate_tuning(t)).ok()) return e;
   const uint64_t G = b.n_slices;
+  const size_t np = b.pics.size();
+  if (G > 0xFFFFFFF0ull) return invalid("a batch of more than 2^32 slices");
+  std::vector<IdctItem> segs;
+  const uint64_t n_items = plan_segments(b, t.idct_blocks, segs);
+  DecBatch hdr;
+  memset(&hdr, 0, sizeof hdr);
+  hdr.n_pics = (uint32_t)np;
+  hdr.n_slices = (uint32_t)G;
+  hdr.n_entropy_items = (uint32_t)(3 * ((G + 63) / 64));
+  hdr.n_idct_items = (uint32_t)n_items;
+  hdr.chunk = t.persistent ? t.chunk : 1;
+  std::vector<DecBatch> hv{hdr};
   if (!(e = upload(d, ws.frames, b.frames)).ok()) return e;
   if (!(e = upload(d, ws.pics, b.pics)).ok()) return e;
   if (!(e = upload(d, ws.rows, b.rows)).ok()) return e;
-  const size_t np = b.pics.size();
+  if (!(e = upload(d, ws.segs, segs)).ok()) return e;
+  if (!(e = upload(d, ws.hdr, hv)).ok()) return e;
+  if (!(e = ws.items.reserve(n_items * sizeof(IdctItem) + 64)).ok()) return e;
   if (!(e = ws.slices.reserve(G * sizeof(SliceRec) + 64)).ok()) return e;
   if (!(e = ws.slice_err.reserve(G * 16 + 64)).ok()) return e;
   if (!(e = ws.status.reserve(np * 4 + 64)).ok()) return e;
   if (!(e = ws.first_err.reserve(np * 4 + 64)).ok()) return e;
   if (!(e = ws.coefs.reserve(b.n_coefs * 2 + 64)).ok()) return e;
   if (!(e = ws.wide.reserve(b.n_coefs * 4 + 64)).ok()) return e;
-  if (G) {
-    if (!(e = check(hipMemsetAsync(ws.slice_err.p, 0, G * 16, d.stream), "memset")).ok()) return e;
-  }
-  if (np) {
-    if (!(e = check(hipMemsetAsync(ws.first_err.p, 0xFF, np * 4, d.stream), "memset")).ok()) return e;
-  }
-  (void)hipEventRecord(d.ev[0], d.stream);
-  void *pics = ws.pics.p, *rows = ws.rows.p, *slices = ws.slices.p, *status = ws.status.p;
+
+  void *B = ws.hdr.p, *pics = ws.pics.p, *rows = ws.rows.p, *slices = ws.slices.p, *status = ws.status.p;
   void *frames = ws.frames.p, *tables = d.tables, *coefs = ws.coefs.p, *wide = ws.wide.p;
-  void *serr = ws.slice_err.p, *ferr = ws.first_err.p;
-  uint32_t g32 = (uint32_t)G;
-  {
-    void *args[] = {&pics, &rows, &slices, &status};
-    if (!(e = launch(d, d.k_slice_table, (unsigned)np, 1, 256, args)).ok()) return e;
+  void *serr = ws.slice_err.p, *ferr = ws.first_err.p, *itemp = ws.items.p, *segp = ws.segs.p;
+  uint32_t pers = t.persistent;
+  const unsigned ewaves = t.entropy_wg / 64;
+  unsigned grids[4];
+  if (t.persistent) {
+    grids[0] = resident_grid(d, d.k_slice_table, 256);
+    grids[1] = resident_grid(d, d.k_entropy, t.entropy_wg);
+    grids[2] = resident_grid(d, d.k_alpha, 64);
+    grids[3] = resident_grid(d, d.k_idct, 256);
+  } else {
+    grids[0] = (unsigned)np;
+    grids[1] = (unsigned)((hdr.n_entropy_items + ewaves - 1) / ewaves);
+    grids[2] = (unsigned)G;
+    grids[3] = (unsigned)n_items;
   }
-  (void)hipEventRecord(d.ev[1], d.stream);
-  {
-    void *args[] = {&tables, &frames, &pics, &slices, &status, &g32, &coefs, &wide, &serr, &ferr};
-    if (!(e = launch(d, d.k_entropy, (unsigned)((G + 255) / 256), 3, 256, args)).ok()) return e;
+  auto enqueue = [&](bool timed) -> Err {
+    Err x;
+    if (timed) (void)hipEventRecord(d.ev[0], d.stream);
+    {
+      void *args[] = {&B, &pics, &rows, &slices, &status, &serr, &ferr, &segp, &itemp};
+      if (!(x = launch(d, d.k_slice_table, grids[0], 1, 256, args)).ok()) return x;
+    }
+    if (timed) (void)hipEventRecord(d.ev[1], d.stream);
+    {
+      void *args[] = {&B, &frames, &pics, &slices, &status, &coefs, &wide, &serr, &ferr, &pers};
+      if (!(x = launch(d, d.k_entropy, grids[1], 1, t.entropy_wg, args)).ok()) return x;
+    }
+    if (timed) (void)hipEventRecord(d.ev[2], d.stream);
+    if (b.any_alpha) {
+      void *args[] = {&B, &frames, &pics, &slices, &status, &serr, &ferr, &pers};
+      if (!(x = launch(d, d.k_alpha, grids[2], 1, 64, args)).ok()) return x;
+    }
+    if (timed) (void)hipEventRecord(d.ev[3], d.stream);
+    {
+      void *args[] = {&tables, &B, &frames, &pics, &slices, &status, &serr, &coefs, &wide, &itemp, &pers};
+      if (!(x = launch(d, d.k_idct, grids[3], 1, 256, args)).ok()) return x;
+    }
+    return Err{};
+  };
+  if (np == 0) {
+    g_last_us[0] = g_last_us[1] = g_last_us[2] = g_last_us[3] = g_last_us[4] = 0;
+  } else if (t.graph) {
+    GraphKey key;
+    memset(&key, 0, sizeof key);
+    void *ptrs[13] = {B, pics, rows, slices, status, frames, tables, coefs, wide, serr, ferr, itemp, segp};
+    memcpy(key.ptrs, ptrs, sizeof ptrs);
+    memcpy(key.grids, grids, sizeof grids);
+    key.t = t;
+    key.alpha = b.any_alpha;
+    if (!ws.graph || !(key == ws.key)) {
+      if (ws.graph) {
+        (void)hipGraphExecDestroy(ws.graph);
+        ws.graph = nullptr;
+      }
+      hipGraph_t graph = nullptr;
+      if (!(e = check(hipStreamBeginCapture(d.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture")).ok())
+        return e;
+      Err ce = enqueue(false);
+      Err ee = check(hipStreamEndCapture(d.stream, &graph), "hipStreamEndCapture");
+      if (!ce.ok()) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return ce;
+      }
+      if (!ee.ok()) return ee;
+      e = check(hipGraphInstantiate(&ws.graph, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      (void)hipGraphDestroy(graph);
+      if (!e.ok()) {
+        ws.graph = nullptr;
+        return e;
+      }
+      ws.key = key;
+    }
+    (void)hipEventRecord(d.ev[0], d.stream);
+    if (!(e = check(hipGraphLaunch(ws.graph, d.stream), "hipGraphLaunch")).ok()) return e;
+    (void)hipEventRecord(d.ev[4], d.stream);
+  } else {
+    if (!(e = enqueue(true)).ok()) return e;
+    (void)hipEventRecord(d.ev[4], d.stream);
   }
-  (void)hipEventRecord(d.ev[2], d.stream);
-  if (b.any_alpha) {
-    void *args[] = {&frames, &pics, &slices, &status, &serr, &ferr};
-    if (!(e = launch(d, d.k_alpha, (unsigned)G, 1, 64, args)).ok()) return e;
-  }
-  (void)hipEventRecord(d.ev[3], d.stream);
-  {
-    void *args[] = {&tables, &frames, &pics, &slices, &status, &serr, &coefs, &wide};
-    if (!(e = launch(d, d.k_idct, (unsigned)G, 1, 256, args)).ok()) return e;
-  }
-  (void)hipEventRecord(d.ev[4], d.stream);
   std::vector<uint32_t> st(np), fe(np);
   if (np) {
     if (!(e = check(hipMemcpyAsync(st.data(), ws.status.p, np * 4, hipMemcpyDeviceToHost, d.stream),
@@ -231,11 +355,14 @@ Err run_batch(Device &d, Batch &b, std::vector<Err> &results) {
       return e;
   }
   if (!(e = check(hipStreamSynchronize(d.stream), "hipStreamSynchronize")).ok()) return e;
-  float ms[4] = {0, 0, 0, 0}, all = 0;
-  for (int i = 0; i < 4; i++) (void)hipEventElapsedTime(&ms[i], d.ev[i], d.ev[i + 1]);
-  (void)hipEventElapsedTime(&all, d.ev[0], d.ev[4]);
-  for (int i = 0; i < 4; i++) g_last_us[i] = ms[i] * 1000.0;
-  g_last_us[4] = all * 1000.0;
+  if (np) {
+    float ms[4] = {0, 0, 0, 0}, all = 0;
+    if (!t.graph)
+      for (int i = 0; i < 4; i++) (void)hipEventElapsedTime(&ms[i], d.ev[i], d.ev[i + 1]);
+    (void)hipEventElapsedTime(&all, d.ev[0], d.ev[4]);
+    for (int i = 0; i < 4; i++) g_last_us[i] = ms[i] * 1000.0;
+    g_last_us[4] = all * 1000.0;
+  }
 
   results.assign(b.plans.size(), Err{});
   for (size_t f = 0; f < b.plans.size(); f++) {
@@ -368,7 +495,9 @@ static prores_status decode_common(const prores_decoder *d, const uint8_t *data,
     uint16_t *da = alpha_n ? (uint16_t *)((uint8_t *)ws.planes.p + plane_bytes) : nullptr;
     plan_frame(data, len, (uint64_t)ws.packet.p, d->bit_depth, dd, da, b, &hd);
     std::vector<Err> results;
-    e = run_batch(*dev, b, results);
+    prores_decode_tuning t;
+    prores_decode_tuning_default(&t);
+    e = run_batch(*dev, b, t, results);
     if (e.ok()) e = results[0];
     if (e.ok()) {
       e = check(hipMemcpy(out->data, dd, total * 2, hipMemcpyDeviceToHost), "D2H planes");
@@ -390,8 +519,8 @@ prores_status prores_decode_into(const prores_decoder *d, const uint8_t *data, s
   return decode_common(d, data, len, out, err, false);
 }
 
-prores_status prores_decode_device_batch(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,
-                                         prores_error *errs) {
+static prores_status batch_common(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,
+                                  const prores_decode_tuning *tuning, prores_error *errs) {
   Device *dev = nullptr;
   Err e = device(dev);
   if (!e.ok()) {
@@ -399,17 +528,118 @@ prores_status prores_decode_device_batch(const prores_decoder *d, size_t n, cons
     return (prores_status)e.kind;
   }
   std::lock_guard<std::mutex> g(dev->mu);
+  Workspace &ws = workspace(*dev);
+  prores_decode_tuning t;
+  if (tuning)
+    t = *tuning;
+  else if (ws.tuned)
+    t = ws.tuning;
+  else
+    prores_decode_tuning_default(&t);
   Batch b;
   for (size_t i = 0; i < n; i++)
     plan_frame(jobs[i].host_packet, jobs[i].len, (uint64_t)jobs[i].dev_packet, d->bit_depth, jobs[i].dev_data,
                jobs[i].dev_alpha, b, nullptr);
   std::vector<Err> results;
-  e = run_batch(*dev, b, results);
+  e = run_batch(*dev, b, t, results);
   if (!e.ok()) {
     for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
     return (prores_status)e.kind;
   }
   for (size_t i = 0; i < n; i++) fill_error(&errs[i], results[i]);
+  return PRORES_OK;
+}
+
+prores_status prores_decode_device_batch(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,
+                                         prores_error *errs) {
+  return batch_common(d, n, jobs, nullptr, errs);
+}
+
+prores_status prores_decode_device_batch_tuned(const prores_decoder *d, size_t n,
+                                               const prores_dev_decode_job *jobs,
+                                               const prores_decode_tuning *t, prores_error *errs) {
+  return batch_common(d, n, jobs, t, errs);
+}
+
+void prores_decode_tuning_default(prores_decode_tuning *t) {
+  t->persistent = 0;
+  t->chunk = 1;
+  t->idct_blocks = IDCT_MAX_BLOCKS;
+  t->entropy_wg = 256;
+  t->graph = 0;
+}
+
+prores_status prores_set_decode_tuning(const prores_decode_tuning *t, prores_error *err) {
+  Err e = validate_tuning(*t);
+  Device *dev = nullptr;
+  if (e.ok()) e = device(dev);
+  if (e.ok()) {
+    std::lock_guard<std::mutex> g(dev->mu);
+    workspace(*dev).tuning = *t;
+    workspace(*dev).tuned = true;
+  }
+  fill_error(err, e);
+  return (prores_status)e.kind;
+}
+
+void prores_get_decode_tuning(prores_decode_tuning *t) {
+  Device *dev = nullptr;
+  prores_decode_tuning_default(t);
+  if (!device(dev).ok()) return;
+  std::lock_guard<std::mutex> g(dev->mu);
+  if (workspace(*dev).tuned) *t = workspace(*dev).tuning;
+}
+
+prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,
+                                     uint32_t reps, prores_decode_tuning *best, double *best_us,
+                                     prores_error *err) {
+  Device *dev = nullptr;
+  Err e = device(dev);
+  if (!e.ok()) {
+    fill_error(err, e);
+    return (prores_status)e.kind;
+  }
+  std::lock_guard<std::mutex> g(dev->mu);
+  Batch b;
+  for (size_t i = 0; i < n; i++)
+    plan_frame(jobs[i].host_packet, jobs[i].len, (uint64_t)jobs[i].dev_packet, d->bit_depth, jobs[i].dev_data,
+               jobs[i].dev_alpha, b, nullptr);
+  if (reps < 1) reps = 1;
+  std::vector<prores_decode_tuning> cands;
+  for (uint32_t pers = 0; pers < 2; pers++)
+    for (uint32_t chunk : {1u, 4u})
+      for (uint32_t blocks : {32u, 64u, 96u})
+        for (uint32_t wg : {64u, 256u})
+          for (uint32_t graph = 0; graph < 2; graph++) {
+            if (!pers && chunk != 1) continue;
+            cands.push_back(prores_decode_tuning{pers, chunk, blocks, wg, graph});
+          }
+  double best_t = 1e300;
+  prores_decode_tuning best_c = cands[0];
+  std::vector<Err> results;
+  std::vector<double> times;
+  for (const prores_decode_tuning &c : cands) {
+    times.clear();
+    for (uint32_t r = 0; r <= reps; r++) {
+      if (!(e = run_batch(*dev, b, c, results)).ok()) {
+        fill_error(err, e);
+        return (prores_status)e.kind;
+      }
+      if (r) times.push_back(g_last_us[4]);
+    }
+    std::sort(times.begin(), times.end());
+    const double med = times[times.size() / 2];
+    if (med < best_t) {
+      best_t = med;
+      best_c = c;
+    }
+  }
+  Workspace &ws = workspace(*dev);
+  ws.tuning = best_c;
+  ws.tuned = true;
+  if (best) *best = best_c;
+  if (best_us) *best_us = best_t;
+  fill_error(err, Err{});
   return PRORES_OK;
 }
 

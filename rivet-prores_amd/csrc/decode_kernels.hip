@@ -20,6 +20,7 @@
 // Built with -ffp-contract=off: every product and sum is its own IEEE op.
 #include <hip/hip_runtime.h>
 
+#include "dct_basis.h"
 #include "layout.h"
 
 using namespace prores_hip;
@@ -342,67 +343,21 @@ __device__ __forceinline__ uint16_t rescale(uint32_t a, uint32_t identity, uint6
   return identity ? (uint16_t)a : (uint16_t)(((uint64_t)a * m + (1ull << 39)) >> 40);
 }
 
-}  // namespace
-
-// ---------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(256)
-    pd_slice_table(const DecPic *__restrict__ pics,
-                   const RowSlot *__restrict__ rows, SliceRec *__restrict__ slices,
-                   uint32_t *__restrict__ pic_status) {
-  const uint32_t p = blockIdx.x, tid = threadIdx.x;
-  const DecPic P = pics[p];
-  const uint8_t *tab = (const uint8_t *)(P.pic_off + P.header_size);
-  const uint64_t data0 = P.header_size + 2ull * P.n_slices;  // relative to the picture
-  __shared__ uint32_t wsum[4];
-  __shared__ uint64_t carry;
-  if (tid == 0) carry = 0;
-  __syncthreads();
-  const uint32_t lane = tid & 63, wave = tid >> 6;
-  for (uint32_t base = 0; base < P.n_slices; base += 256) {
-    uint32_t k = base + tid;
-    uint32_t size = k < P.n_slices ? ((uint32_t)tab[2 * k] << 8) | tab[2 * k + 1] : 0u;
-    uint32_t incl = size;
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-      uint32_t o = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += o;
-    }
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    uint32_t before = 0;
-    for (uint32_t w = 0; w < wave; w++) before += wsum[w];
-    uint64_t excl = carry + before + incl - size;
-    if (k < P.n_slices) {
-      uint32_t row = k / P.row_slices, j = k - row * P.row_slices;
-      RowSlot rs = rows[P.layout_base + j];
-      SliceRec s;
-      s.off = P.pic_off + data0 + excl;
-      s.coef = P.coef_base + (uint64_t)row * P.row_coefs + rs.coef_prefix;
-      s.len = size;
-      s.pic = p;
-      s.mb_x = rs.mb_x;
-      s.mb_y = (uint16_t)row;
-      s.mbs = rs.mbs;
-      slices[P.slice_base + k] = s;
-    }
-    __syncthreads();
-    if (tid == 255) carry += before + incl;
-    __syncthreads();
-  }
-  if (tid == 0) pic_status[p] = (data0 + carry > P.picture_size) ? PIC_TABLE_ERR : PIC_OK;
+// Work distribution. In grid form each workgroup (or wave) takes the item
+// its index names; in persistent form a grid sized to the device takes items
+// from the batch's queue, `chunk` at a time, until the queue is empty.
+__device__ __forceinline__ uint32_t wave_pop(uint32_t *head, uint32_t chunk) {
+  uint32_t v = 0;
+  if ((threadIdx.x & 63) == 0) v = atomicAdd(head, chunk);
+  return __shfl(v, 0, 64);
 }
 
-// ---------------------------------------------------------------------------
-// grid.x covers slices 256 at a time, grid.y is the component: 0 Y', 1 Cb,
-// 2 Cr.
-extern "C" __global__ void __launch_bounds__(256)
-    pd_entropy(const DevTables *__restrict__ T,
-               const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
-               const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
-               uint32_t n_slices, int16_t *__restrict__ coefs, int32_t *__restrict__ coef_wide,
-               uint32_t *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err) {
-  const uint32_t comp = blockIdx.y, tid = threadIdx.x;
-  const uint32_t g = blockIdx.x * 256 + tid;
-  if (g >= n_slices) return;
+// One colour stream: component `comp` of slice `g`.
+__device__ __forceinline__ void entropy_stream(
+    uint32_t g, uint32_t comp, const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
+    const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
+    int16_t *__restrict__ coefs, int32_t *__restrict__ coef_wide, uint32_t *__restrict__ slice_err,
+    uint32_t *__restrict__ pic_first_err) {
   const SliceRec S = slices[g];
   if (pic_status[S.pic] != PIC_OK) return;
   const DecPic &P = pics[S.pic];
@@ -430,25 +385,15 @@ extern "C" __global__ void __launch_bounds__(256)
   }
 }
 
-// ---------------------------------------------------------------------------
-// One wavefront per slice with alpha: scanned_alpha() (src/decode.rs:570-595)
-// and its placement (src/decode.rs:365-389). The wave stages the slice's
-// alpha stream in LDS (big-endian 64-bit words, zero past the end), every
-// lane parses the same codes from it (a broadcast read; the parse is
-// wave-uniform), runs are filled into an LDS copy of the slice's values by
-// the whole wave, and the visible rows are stored coalesced at the end. The
-// parse never waits on a global store.
-// A code is at most 33 bits (a raw difference of 1 + 16, a run of 1 + 4 +
-// 11), so 2048 values read at most 8448 bytes of the stream.
+// One slice's alpha (see pd_alpha).
 constexpr uint32_t ALPHA_WORDS = (2048 * 33 / 8) / 8 + 6;
 
-extern "C" __global__ void __launch_bounds__(64)
-    pd_alpha(const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
-             const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
-             uint32_t *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err) {
-  __shared__ uint64_t words[ALPHA_WORDS];
-  __shared__ __attribute__((aligned(16))) uint16_t vals[2048];
-  const uint32_t g = blockIdx.x, L = threadIdx.x;
+__device__ __forceinline__ void alpha_slice(
+    uint32_t g, uint64_t *__restrict__ words, uint16_t *__restrict__ vals,
+    const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
+    const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
+    uint32_t *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err) {
+  const uint32_t L = threadIdx.x;
   const SliceRec S = slices[g];
   if (pic_status[S.pic] != PIC_OK) return;
   const DecPic &P = pics[S.pic];
@@ -611,105 +556,309 @@ extern "C" __global__ void __launch_bounds__(64)
   }
 }
 
+}  // namespace
+
 // ---------------------------------------------------------------------------
-// One workgroup per slice; 8 lanes per block (a row each in the row pass, an
-// output row each in the column pass), 32 blocks per pass.
+// One workgroup per picture (a grid-stride loop in persistent form): the
+// u16 big-endian slice-size table becomes byte offsets (an exclusive
+// prefix), with the bounds check of the table walk (src/decode.rs:202-221).
+// It also clears the batch's queue heads, its slice error words and its
+// first-error slots, so the launch sequence needs no memset, and writes the
+// picture's IDCT items.
 extern "C" __global__ void __launch_bounds__(256)
-    pd_idct(const DevTables *__restrict__ T,
+    pd_slice_table(DecBatch *__restrict__ B, const DecPic *__restrict__ pics,
+                   const RowSlot *__restrict__ rows, SliceRec *__restrict__ slices,
+                   uint32_t *__restrict__ pic_status, uint4 *__restrict__ slice_err,
+                   uint32_t *__restrict__ pic_first_err, const IdctItem *__restrict__ segs,
+                   IdctItem *__restrict__ items) {
+  const uint32_t tid = threadIdx.x;
+  if (blockIdx.x == 0 && tid < 4) B->q[tid] = 0;
+  const uint32_t np = B->n_pics;
+  __shared__ uint32_t wsum[4];
+  __shared__ uint64_t carry;
+  const uint32_t lane = tid & 63, wave = tid >> 6;
+  for (uint32_t p = blockIdx.x; p < np; p += gridDim.x) {
+    const DecPic P = pics[p];
+    const uint8_t *tab = (const uint8_t *)(P.pic_off + P.header_size);
+    const uint64_t data0 = P.header_size + 2ull * P.n_slices;  // relative to the picture
+    if (tid == 0) {
+      carry = 0;
+      pic_first_err[p] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < P.n_slices; base += 256) {
+      const uint32_t k = base + tid;
+      const uint32_t size = k < P.n_slices ? ((uint32_t)tab[2 * k] << 8) | tab[2 * k + 1] : 0u;
+      uint32_t incl = size;
+      for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+      }
+      if (lane == 63) wsum[wave] = incl;
+      __syncthreads();
+      uint32_t before = 0;
+      for (uint32_t w = 0; w < wave; w++) before += wsum[w];
+      const uint64_t excl = carry + before + incl - size;
+      if (k < P.n_slices) {
+        const uint32_t row = k / P.row_slices, j = k - row * P.row_slices;
+        const RowSlot rs = rows[P.layout_base + j];
+        SliceRec sr;
+        sr.off = P.pic_off + data0 + excl;
+        sr.coef = P.coef_base + (uint64_t)row * P.row_coefs + rs.coef_prefix;
+        sr.len = size;
+        sr.pic = p;
+        sr.mb_x = rs.mb_x;
+        sr.mb_y = (uint16_t)row;
+        sr.mbs = rs.mbs;
+        slices[P.slice_base + k] = sr;
+        slice_err[P.slice_base + k] = make_uint4(0, 0, 0, 0);
+      }
+      __syncthreads();
+      if (tid == 255) carry += before + incl;
+      __syncthreads();
+    }
+    if (tid == 0) pic_status[p] = (data0 + carry > P.picture_size) ? PIC_TABLE_ERR : PIC_OK;
+    // The picture's IDCT items: every row cut into the same segments.
+    for (uint32_t i = tid; i < P.height_in_mb * P.n_segs; i += 256) {
+      const uint32_t row = i / P.n_segs;
+      IdctItem it = segs[P.seg_base + (i - row * P.n_segs)];
+      it.first += P.slice_base + row * P.row_slices;
+      items[P.item_base + i] = it;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One lane per colour stream; an item is 64 streams of one component (64
+// consecutive slices), one wave's work: item i is component i % 3 of slices
+// 64 (i / 3) ... 64 (i / 3) + 63.
+extern "C" __global__ void __launch_bounds__(256)
+    pd_entropy(DecBatch *__restrict__ B, const DecFrame *__restrict__ frames,
+               const DecPic *__restrict__ pics, const SliceRec *__restrict__ slices,
+               const uint32_t *__restrict__ pic_status, int16_t *__restrict__ coefs,
+               int32_t *__restrict__ coef_wide, uint32_t *__restrict__ slice_err,
+               uint32_t *__restrict__ pic_first_err, uint32_t persistent) {
+  const uint32_t n_items = B->n_entropy_items, n_slices = B->n_slices;
+  const uint32_t lane = threadIdx.x & 63;
+  if (!persistent) {
+    const uint32_t item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t g = (item / 3) * 64 + lane;
+    if (item < n_items && g < n_slices)
+      entropy_stream(g, item % 3, frames, pics, slices, pic_status, coefs, coef_wide, slice_err,
+                     pic_first_err);
+    return;
+  }
+  const uint32_t chunk = B->chunk;
+  for (;;) {
+    const uint32_t first = wave_pop(&B->q[0], chunk);
+    if (first >= n_items) break;
+    const uint32_t last = first + chunk < n_items ? first + chunk : n_items;
+    for (uint32_t item = first; item < last; item++) {
+      const uint32_t g = (item / 3) * 64 + lane;
+      if (g < n_slices)
+        entropy_stream(g, item % 3, frames, pics, slices, pic_status, coefs, coef_wide, slice_err,
+                       pic_first_err);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One wavefront per slice with alpha: scanned_alpha() (src/decode.rs:570-595)
+// and its placement (src/decode.rs:365-389). The wave stages the slice's
+// alpha stream in LDS (big-endian 64-bit words, zero past the end), every
+// lane parses the same codes from it (a broadcast read; the parse is
+// wave-uniform), runs are filled into an LDS copy of the slice's values by
+// the whole wave, and the visible rows are stored coalesced at the end. The
+// parse never waits on a global store.
+// A code is at most 33 bits (a raw difference of 1 + 16, a run of 1 + 4 +
+// 11), so 2048 values read at most 8448 bytes of the stream.
+extern "C" __global__ void __launch_bounds__(64)
+    pd_alpha(DecBatch *__restrict__ B, const DecFrame *__restrict__ frames,
+             const DecPic *__restrict__ pics, const SliceRec *__restrict__ slices,
+             const uint32_t *__restrict__ pic_status, uint32_t *__restrict__ slice_err,
+             uint32_t *__restrict__ pic_first_err, uint32_t persistent) {
+  __shared__ uint64_t words[ALPHA_WORDS];
+  __shared__ __attribute__((aligned(16))) uint16_t vals[2048];
+  const uint32_t n = B->n_slices;
+  if (!persistent) {
+    if (blockIdx.x < n)
+      alpha_slice(blockIdx.x, words, vals, frames, pics, slices, pic_status, slice_err, pic_first_err);
+    return;
+  }
+  const uint32_t chunk = B->chunk;
+  for (;;) {
+    const uint32_t first = wave_pop(&B->q[1], chunk);
+    if (first >= n) break;
+    const uint32_t last = first + chunk < n ? first + chunk : n;
+    for (uint32_t g = first; g < last; g++) {
+      alpha_slice(g, words, vals, frames, pics, slices, pic_status, slice_err, pic_first_err);
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One workgroup per item of consecutive slices of one picture (at most
+// IDCT_MAX_BLOCKS blocks): dequantisation, the separable f32 IDCT in the
+// reference's operation order (src/dct.rs:36-66, src/dsp/scalar.rs:8-12) and
+// clamped planar u16 stores (src/decode.rs:395-438). 32 blocks per pass, 8
+// lanes per block: a row each in the row pass, a column each in the column
+// pass, and a row each again for the 16-byte stores (through LDS). Every sum
+// starts from its first product instead of 0.0 + it, which can change only
+// the sign of a zero, and no nonzero sum or final sample depends on that.
+// Slices with an error are skipped.
+constexpr uint32_t IDCT_MAX_SLICES = 16;
+
+extern "C" __global__ void __launch_bounds__(256)
+    pd_idct(const DevTables *__restrict__ T, DecBatch *__restrict__ B,
             const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
             const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
             const uint32_t *__restrict__ slice_err, const int16_t *__restrict__ coefs,
-            const int32_t *__restrict__ coef_wide) {
-  __shared__ __attribute__((aligned(16))) int16_t sc[96 * 64];
+            const int32_t *__restrict__ coef_wide, const IdctItem *__restrict__ items,
+            uint32_t persistent) {
+  __shared__ __attribute__((aligned(16))) int16_t sc[IDCT_MAX_BLOCKS * 64];
   __shared__ float tmp[32 * 65];
-  const uint32_t g = blockIdx.x, tid = threadIdx.x;
-  const SliceRec S = slices[g];
-  if (pic_status[S.pic] != PIC_OK) return;
-  const uint4 ev = *(const uint4 *)(slice_err + 4 * (size_t)g);
-  if (ev.x | ev.y | ev.z | ev.w) return;
-  const DecPic &P = pics[S.pic];
-  const DecFrame &F = frames[P.frame];
-  const uint32_t chroma = F.chroma, cpm = chroma == 2 ? 2u : 4u, mbs = S.mbs;
-  const uint32_t nbY = 4 * mbs, nbC = cpm * mbs, TB = nbY + 2 * nbC;
-  const uint32_t ncoef = TB * 64;
-  {
-    const uint4 *src = (const uint4 *)(coefs + S.coef);
-    for (uint32_t i = tid; i < ncoef / 8; i += 256) ((uint4 *)sc)[i] = src[i];
-  }
-  const uint32_t q = qscale(((const uint8_t *)S.off)[1]);
-  const uint8_t *scan = T->scan[F.interlaced ? 1 : 0];
-  const float *basis = T->basis;
-  const float osc = F.out_scale, obias = F.out_bias, omax = F.out_max;
-  __syncthreads();
-  const uint32_t r = tid & 7, lb = tid >> 3;
-  for (uint32_t t0 = 0; t0 < TB; t0 += 32) {
-    const uint32_t t = t0 + lb;
-    const bool active = t < TB;
-    uint32_t c = 0, b = t, nb = nbY, cbase = 0;
-    if (t >= nbY + nbC) {
-      c = 2; b = t - nbY - nbC; nb = nbC; cbase = 64 * (nbY + nbC);
-    } else if (t >= nbY) {
-      c = 1; b = t - nbY; nb = nbC; cbase = 64 * nbY;
+  __shared__ __attribute__((aligned(16))) uint16_t outb[32 * 64];
+  __shared__ uint32_t s_start[IDCT_MAX_SLICES + 1], s_q[IDCT_MAX_SLICES], s_mbx[IDCT_MAX_SLICES],
+      s_mby[IDCT_MAX_SLICES], s_mbs[IDCT_MAX_SLICES], s_ok[IDCT_MAX_SLICES];
+  __shared__ uint8_t s_of[IDCT_MAX_BLOCKS], s_w[128], s_scan[64];
+  __shared__ uint32_t s_item;
+  const uint32_t tid = threadIdx.x, n_items = B->n_idct_items;
+  const uint32_t chunk = persistent ? B->chunk : 1u;
+  uint32_t first = blockIdx.x;
+  for (;;) {
+    if (persistent) {
+      if (tid == 0) s_item = atomicAdd(&B->q[2], chunk);
+      __syncthreads();
+      first = s_item;
+      __syncthreads();
     }
-    if (active) {
-      const uint8_t *W = c == 0 ? F.luma_w : F.chroma_w;
-      float fu[8];
-#pragma unroll
-      for (uint32_t u = 0; u < 8; u++) {
-        const uint32_t k = 8 * r + u;
-        const uint32_t idx = cbase + nb * scan[k] + b;
-        int32_t v = sc[idx];
-        if (v == -32768) v = coef_wide[S.coef + idx];
-        fu[u] = (float)v * ((float)((uint32_t)W[k] * q) / 8.0f);
+    if (first >= n_items) break;
+    const uint32_t last = first + chunk < n_items ? first + chunk : n_items;
+    for (uint32_t it = first; it < last; it++) {
+      const IdctItem I = items[it];
+      const SliceRec S0 = slices[I.first];
+      const DecPic &P = pics[S0.pic];
+      const DecFrame &F = frames[P.frame];
+      const bool pic_ok = pic_status[S0.pic] == PIC_OK;
+      const uint32_t chroma = F.chroma, cpm = chroma == 2 ? 2u : 4u;
+      // The item's slices: block offsets, qScale, position, error state; the
+      // frame's weights and scan.
+      if (tid < I.count) {
+        const SliceRec S = slices[I.first + tid];
+        const uint4 ev = *(const uint4 *)(slice_err + 4 * (size_t)(I.first + tid));
+        // A picture whose slice table failed its bounds check has slice
+        // offsets that may point anywhere: nothing of it is read.
+        const bool ok = pic_ok && !(ev.x | ev.y | ev.z | ev.w);
+        s_q[tid] = ok ? qscale(((const uint8_t *)S.off)[1]) : 0u;
+        s_mbx[tid] = S.mb_x;
+        s_mby[tid] = S.mb_y;
+        s_mbs[tid] = S.mbs;
+        s_ok[tid] = ok;
+        s_start[tid + 1] = (uint32_t)((S.coef - S0.coef) / 64) + S.mbs * (4 + 2 * cpm);
+        if (tid == 0) s_start[0] = 0;
       }
-#pragma unroll
-      for (uint32_t x = 0; x < 8; x++) {
-        float acc = 0.0f;
-#pragma unroll
-        for (uint32_t u = 0; u < 8; u++) acc = acc + fu[u] * basis[8 * x + u];
-        tmp[lb * 65 + 8 * r + x] = acc;
+      if (tid >= 64 && tid < 192) s_w[tid - 64] = tid < 128 ? F.luma_w[tid - 64] : F.chroma_w[tid - 128];
+      if (tid >= 192) s_scan[tid - 192] = T->scan[F.interlaced ? 1 : 0][tid - 192];
+      {
+        const uint4 *src = (const uint4 *)(coefs + S0.coef);
+        for (uint32_t i = tid; i < I.blocks * 8; i += 256) ((uint4 *)sc)[i] = src[i];
       }
-    }
-    __syncthreads();
-    if (active) {
-      uint16_t outv[8];
-#pragma unroll
-      for (uint32_t x = 0; x < 8; x++) {
-        float acc = 0.0f;
-#pragma unroll
-        for (uint32_t v = 0; v < 8; v++) acc = acc + basis[8 * r + v] * tmp[lb * 65 + 8 * v + x];
-        float sv = floorf(acc * osc + obias);
-        sv = sv < 0.0f ? 0.0f : (sv > omax ? omax : sv);
-        outv[x] = (uint16_t)(uint32_t)sv;
+      __syncthreads();
+      for (uint32_t t = tid; t < I.blocks; t += 256) {
+        uint32_t j = 0;
+        while (j + 1 < I.count && s_start[j + 1] <= t) j++;
+        s_of[t] = (uint8_t)j;
       }
-      // block_position (src/decode.rs:395-413) and put_block (418-438).
-      const uint32_t per_mb = c == 0 ? 4u : cpm;
-      const uint32_t mbx = S.mb_x + b / per_mb, bb = b % per_mb;
-      uint32_t bx, by;
-      if (c == 0) {
-        bx = 16 * mbx + 8 * (bb & 1); by = 16u * S.mb_y + 8 * (bb >> 1);
-      } else if (chroma == 2) {
-        bx = 8 * mbx; by = 16u * S.mb_y + 8 * bb;
-      } else {
-        bx = 16 * mbx + 8 * (bb >> 1); by = 16u * S.mb_y + 8 * (bb & 1);
-      }
-      const uint32_t pw = F.plane_w[c];
-      const uint32_t prow = by + r;
-      if (bx < pw && prow < P.height) {
-        uint16_t *o = (uint16_t *)F.out_plane[c] + (size_t)(P.first_row + P.row_step * prow) * pw + bx;
-        const uint32_t cols = pw - bx < 8 ? pw - bx : 8u;
-        if (cols == 8 && (((uintptr_t)o) & 15) == 0) {
-          uint4 pk;
-          pk.x = outv[0] | ((uint32_t)outv[1] << 16);
-          pk.y = outv[2] | ((uint32_t)outv[3] << 16);
-          pk.z = outv[4] | ((uint32_t)outv[5] << 16);
-          pk.w = outv[6] | ((uint32_t)outv[7] << 16);
-          *(uint4 *)o = pk;
-        } else {
-          for (uint32_t x = 0; x < cols; x++) o[x] = outv[x];
+      const float osc = F.out_scale, obias = F.out_bias, omax = F.out_max;
+      __syncthreads();
+      const uint32_t r = tid & 7, lb = tid >> 3;
+      for (uint32_t t0 = 0; t0 < I.blocks; t0 += 32) {
+        const uint32_t t = t0 + lb;
+        uint32_t j = 0, c = 0, b = 0, nb = 0, cbase = 0;
+        bool active = t < I.blocks;
+        if (active) {
+          j = s_of[t];
+          active = s_ok[j] != 0;
+          const uint32_t mbs = s_mbs[j];
+          const uint32_t nbY = 4 * mbs, nbC = cpm * mbs, tl = t - s_start[j];
+          cbase = 64 * s_start[j];
+          b = tl;
+          nb = nbY;
+          if (tl >= nbY + nbC) {
+            c = 2; b = tl - nbY - nbC; nb = nbC; cbase += 64 * (nbY + nbC);
+          } else if (tl >= nbY) {
+            c = 1; b = tl - nbY; nb = nbC; cbase += 64 * nbY;
+          }
+        }
+        // Row pass: lane r, row r of the block (F = coefficient x (W q)/8).
+        if (active) {
+          const uint32_t q = s_q[j], wo = c == 0 ? 0u : 64u;
+          float fu[8];
+#pragma unroll
+          for (uint32_t u = 0; u < 8; u++) {
+            const uint32_t k = 8 * r + u;
+            const uint32_t idx = cbase + nb * s_scan[k] + b;
+            int32_t v = sc[idx];
+            if (v == -32768) v = coef_wide[S0.coef + idx];
+            fu[u] = (float)v * ((float)((uint32_t)s_w[wo + k] * q) / 8.0f);
+          }
+#pragma unroll
+          for (uint32_t x = 0; x < 8; x++) {
+            float acc = fu[0] * BASIS[8 * x];
+#pragma unroll
+            for (uint32_t u = 1; u < 8; u++) acc = acc + fu[u] * BASIS[8 * x + u];
+            tmp[lb * 65 + 8 * r + x] = acc;
+          }
+        }
+        __syncthreads();
+        // Column pass: lane r, column r; samples to LDS in rows.
+        if (active) {
+          float tc[8];
+#pragma unroll
+          for (uint32_t v = 0; v < 8; v++) tc[v] = tmp[lb * 65 + 8 * v + r];
+#pragma unroll
+          for (uint32_t y = 0; y < 8; y++) {
+            float acc = BASIS[8 * y] * tc[0];
+#pragma unroll
+            for (uint32_t v = 1; v < 8; v++) acc = acc + BASIS[8 * y + v] * tc[v];
+            float sv = floorf(acc * osc + obias);
+            sv = sv < 0.0f ? 0.0f : (sv > omax ? omax : sv);
+            outb[lb * 64 + 8 * y + r] = (uint16_t)(uint32_t)sv;
+          }
+        }
+        __syncthreads();
+        // Row r of the block to the plane: block_position (src/decode.rs:395-413)
+        // and put_block (418-438).
+        if (active) {
+          const uint32_t per_mb = c == 0 ? 4u : cpm;
+          const uint32_t mbx = s_mbx[j] + b / per_mb, bb = b % per_mb, mby = s_mby[j];
+          uint32_t bx, by;
+          if (c == 0) {
+            bx = 16 * mbx + 8 * (bb & 1); by = 16u * mby + 8 * (bb >> 1);
+          } else if (chroma == 2) {
+            bx = 8 * mbx; by = 16u * mby + 8 * bb;
+          } else {
+            bx = 16 * mbx + 8 * (bb >> 1); by = 16u * mby + 8 * (bb & 1);
+          }
+          const uint32_t pw = F.plane_w[c];
+          const uint32_t prow = by + r;
+          if (bx < pw && prow < P.height) {
+            uint16_t *o = (uint16_t *)F.out_plane[c] + (size_t)(P.first_row + P.row_step * prow) * pw + bx;
+            const uint32_t cols = pw - bx < 8 ? pw - bx : 8u;
+            const uint4 pk = *(const uint4 *)(outb + lb * 64 + 8 * r);
+            if (cols == 8 && (((uintptr_t)o) & 15) == 0) {
+              *(uint4 *)o = pk;
+            } else {
+              const uint32_t wv[4] = {pk.x, pk.y, pk.z, pk.w};
+              for (uint32_t x = 0; x < cols; x++) o[x] = (uint16_t)(wv[x >> 1] >> (16 * (x & 1)));
+            }
+          }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
+    if (!persistent) break;
   }
 }

@@ -61,6 +61,9 @@ struct DecPic {
   uint32_t row_coefs;      // coefficients in one row of slices
   uint32_t frame;          // DecFrame index
   uint32_t height, height_in_mb, first_row, row_step;
+  uint32_t seg_base, n_segs;  // IDCT segments of one row (the same for every row)
+  uint32_t item_base;         // index of the picture's first IDCT item
+  uint32_t pad;
 };
 
 // Where slice j of a row sits: its first macroblock, its size and the
@@ -79,6 +82,28 @@ struct SliceRec {
   uint16_t mb_x, mb_y;
   uint32_t mbs;
 };
+
+// Counts of a batch and the heads of its work queues, in device memory so
+// that the launch arguments stay the same from one batch to the next (a
+// captured graph replays unchanged). pd_slice_table zeroes the queue heads.
+struct DecBatch {
+  uint32_t n_pics, n_slices;
+  uint32_t n_entropy_items;  // 64 streams of one component each: 3 * ceil(n_slices / 64)
+  uint32_t n_idct_items;
+  uint32_t chunk;            // items taken per queue pop by a persistent workgroup
+  uint32_t pad[3];
+  uint32_t q[4];             // queue heads: entropy, alpha, idct
+};
+
+// A run of consecutive slices of one picture transformed by one workgroup
+// (their coefficients are contiguous in the arena). As a row segment,
+// `first` counts from the start of the row; as an item, from the batch's
+// first slice. pd_slice_table expands each picture's row segments into its
+// items (rows x segments).
+struct IdctItem {
+  uint32_t first, count, blocks, pad;
+};
+constexpr uint32_t IDCT_MAX_BLOCKS = 96;  // blocks one workgroup holds in LDS
 
 // Picture status words written by the kernels.
 enum PicStatus : uint32_t { PIC_OK = 0, PIC_TABLE_ERR = 1 };

@@ -9,6 +9,7 @@ GPU the calls raise.
 """
 
 import ctypes as C
+import dataclasses
 import enum
 import os
 
@@ -105,6 +106,11 @@ class _Config(C.Structure):
                 ("threads", C.c_uint32)]
 
 
+class _Tuning(C.Structure):
+    _fields_ = [("persistent", C.c_uint32), ("chunk", C.c_uint32), ("idct_blocks", C.c_uint32),
+                ("entropy_wg", C.c_uint32), ("graph", C.c_uint32)]
+
+
 class _DevDecodeJob(C.Structure):
     _fields_ = [("host_packet", C.c_void_p), ("len", C.c_size_t), ("dev_packet", C.c_void_p),
                 ("dev_data", C.c_void_p), ("dev_alpha", C.c_void_p)]
@@ -123,7 +129,8 @@ EXPORTS = [
     "prores_profile_target_frame_bytes", "prores_profile_fourcc", "prores_profile_from_fourcc",
     "prores_decode_device_batch", "prores_encode_device_batch", "prores_last_batch_timing",
     "prores_last_encode_timing", "prores_version", "prores_device_alloc", "prores_device_free",
-    "prores_memcpy",
+    "prores_memcpy", "prores_decode_tuning_default", "prores_decode_device_batch_tuned",
+    "prores_set_decode_tuning", "prores_get_decode_tuning", "prores_autotune_decode",
 ]
 
 _lib = None
@@ -154,6 +161,13 @@ def lib():
         L.prores_profile_from_fourcc.argtypes = [C.c_char_p]
         L.prores_profile_from_fourcc.restype = C.c_int32
         L.prores_decode_device_batch.argtypes = [C.c_void_p, C.c_size_t, P(_DevDecodeJob), P(_Err)]
+        L.prores_decode_device_batch_tuned.argtypes = [C.c_void_p, C.c_size_t, P(_DevDecodeJob), P(_Tuning),
+                                                       P(_Err)]
+        L.prores_decode_tuning_default.argtypes = [P(_Tuning)]
+        L.prores_set_decode_tuning.argtypes = [P(_Tuning), P(_Err)]
+        L.prores_get_decode_tuning.argtypes = [P(_Tuning)]
+        L.prores_autotune_decode.argtypes = [C.c_void_p, C.c_size_t, P(_DevDecodeJob), C.c_uint32, P(_Tuning),
+                                             P(C.c_double), P(_Err)]
         L.prores_encode_device_batch.argtypes = [C.c_size_t, P(_DevEncodeJob), P(_Err)]
         L.prores_last_batch_timing.argtypes = [P(C.c_double)]
         L.prores_last_encode_timing.argtypes = [P(C.c_double)]
@@ -380,16 +394,31 @@ class Decoder:
                      Metadata(m.aspect_ratio, m.frame_rate_code, m.color_primaries, m.transfer_characteristic,
                               m.matrix_coefficients))
 
-    def decode_device_batch(self, jobs):
+    def decode_device_batch(self, jobs, tuning=None):
         """Decodes device-resident packets: ``jobs`` is a DecodeJobs or a list
         of (host_packet bytes, dev_packet ptr, dev_data ptr, dev_alpha ptr or
-        0). Returns one Error or None per job."""
+        0). ``tuning`` (a DecodeTuning) overrides the device's batch tuning.
+        Returns one Error or None per job."""
         if not isinstance(jobs, DecodeJobs):
             jobs = DecodeJobs(jobs)
-        rc = lib().prores_decode_device_batch(self._h, jobs.n, jobs.arr, jobs.errs)
-        if rc == 4:
+        if tuning is None:
+            rc = lib().prores_decode_device_batch(self._h, jobs.n, jobs.arr, jobs.errs)
+        else:
+            t = tuning._c()
+            rc = lib().prores_decode_device_batch_tuned(self._h, jobs.n, jobs.arr, C.byref(t), jobs.errs)
+        if rc == 4 or (rc and tuning is not None):
             _raise(jobs.errs[0])
         return [None if e.kind == 0 else _KINDS.get(e.kind, Error)(e.msg.decode()) for e in jobs.errs]
+
+    def autotune(self, jobs, reps=3):
+        """Times every candidate DecodeTuning on this batch and makes the
+        fastest the device's batch tuning; returns (tuning, median us)."""
+        if not isinstance(jobs, DecodeJobs):
+            jobs = DecodeJobs(jobs)
+        t, us, err = _Tuning(), C.c_double(), _Err()
+        if lib().prores_autotune_decode(self._h, jobs.n, jobs.arr, reps, C.byref(t), C.byref(us), C.byref(err)):
+            _raise(err)
+        return DecodeTuning._from(t), us.value
 
 
 class DecodeJobs:
@@ -407,6 +436,37 @@ class DecodeJobs:
             self.arr[i].host_packet = C.cast(C.c_char_p(hp), C.c_void_p)
             self.arr[i].len = len(hp)
             self.arr[i].dev_packet, self.arr[i].dev_data, self.arr[i].dev_alpha = dp, dd, da or None
+
+
+@dataclasses.dataclass
+class DecodeTuning:
+    """How a decode batch is launched (prores_decode_tuning): every setting
+    decodes the same samples."""
+    persistent: int = 0
+    chunk: int = 1
+    idct_blocks: int = 96
+    entropy_wg: int = 256
+    graph: int = 0
+
+    def _c(self):
+        return _Tuning(self.persistent, self.chunk, self.idct_blocks, self.entropy_wg, self.graph)
+
+    @classmethod
+    def _from(cls, t):
+        return cls(t.persistent, t.chunk, t.idct_blocks, t.entropy_wg, t.graph)
+
+
+def set_decode_tuning(tuning):
+    err = _Err()
+    t = tuning._c()
+    if lib().prores_set_decode_tuning(C.byref(t), C.byref(err)):
+        _raise(err)
+
+
+def get_decode_tuning():
+    t = _Tuning()
+    lib().prores_get_decode_tuning(C.byref(t))
+    return DecodeTuning._from(t)
 
 
 def last_batch_timing():
