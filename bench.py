@@ -20,6 +20,7 @@ the same packets on this host's cores.
 """
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -113,6 +114,171 @@ def cpu_baseline(O, packets, budget_s):
                       f"(one task per macroblock row), best of {runs} runs"}
 
 
+def timed(fn, steps):
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    return (time.perf_counter() - t0) / steps
+
+
+def leg_mixed_decode(P, synth, steps, warmup, check):
+    """96 mixed frames in one batch: 32 x 720x486 interlaced 422 HQ (top
+    field first, 2-MB slices), 32 x 1280x720 LT (4-MB), 24 x 1920x1080 Proxy
+    (1-MB), 8 x 4096x2160 4444 with 8-bit alpha and loaded matrices (entries
+    4-32); per-frame noise sigma uniform in 2-40 codes. The batch tuning is
+    picked by the autotuner; "plain" decodes the frames one call each."""
+    rng = np.random.default_rng(7)
+    kinds = ([(720, 486, 2, 10, P.Profile.Hq, 1, 1, 0, False)] * 32 +
+             [(1280, 720, 2, 10, P.Profile.Lt, 2, 0, 0, False)] * 32 +
+             [(1920, 1080, 2, 10, P.Profile.Proxy, 0, 0, 0, False)] * 24 +
+             [(4096, 2160, 3, 12, P.Profile.P4444, 3, 0, 1, True)] * 8)
+    frames, items, bufs = [], [], []
+    for i, (w, h, ch, bd, prof, log2, il, atype, mats) in enumerate(kinds):
+        f = synth.frame(w, h, ch, bd, seed=5000 + i, sigma=float(rng.uniform(2, 40)), alpha=atype > 0,
+                        interlace=P.Interlace(il))
+        luma = chroma = None
+        if mats:
+            luma = [int(v) for v in rng.integers(4, 33, 64)]
+            chroma = [int(v) for v in rng.integers(4, 33, 64)]
+        cfg = P.Config(prof, log2_slice_mbs=log2, alpha=P.AlphaType(atype) if atype else P.AlphaType.Bits16,
+                       luma_matrix=luma, chroma_matrix=chroma)
+        dd = P.DeviceBuffer(f.data.nbytes + 64)
+        dd.upload(f.data)
+        da = None
+        if f.alpha is not None:
+            da = P.DeviceBuffer(f.alpha.nbytes + 64)
+            da.upload(f.alpha)
+        frames.append((f, cfg))
+        bufs.append((dd, da))
+        items.append((cfg, f, dd.ptr, da.ptr if da else 0))
+    res = P.encode_device_batch(items)
+    packets, dev_packets, outs = [], [], []
+    for r, (f, cfg) in zip(res, frames):
+        if isinstance(r, Exception):
+            raise r
+        host = P.download_ptr(*r).tobytes()
+        dp = P.DeviceBuffer(len(host) + 128)
+        P.copy_device(dp.ptr, r[0], len(host))
+        packets.append(host)
+        dev_packets.append(dp)
+        n = f.data.size
+        outs.append((P.DeviceBuffer(n * 2 + 64), P.DeviceBuffer(f.width * f.height * 2 + 64) if f.alpha is not None
+                     else None))
+    del bufs
+    n = len(packets)
+    jobs = P.DecodeJobs([(packets[i], dev_packets[i].ptr, outs[i][0].ptr, outs[i][1].ptr if outs[i][1] else 0)
+                         for i in range(n)])
+    singles = [P.DecodeJobs([(packets[i], dev_packets[i].ptr, outs[i][0].ptr,
+                              outs[i][1].ptr if outs[i][1] else 0)]) for i in range(n)]
+    dec = P.Decoder()
+    default = P.DecodeTuning()
+    tuning, tuned_us = dec.autotune(jobs, reps=3)
+
+    def batched():
+        if any(dec.decode_device_batch(jobs)):
+            raise RuntimeError("mixed decode failed")
+
+    def plain():
+        for j in singles:
+            if dec.decode_device_batch(j, default)[0]:
+                raise RuntimeError("mixed decode failed")
+
+    for _ in range(warmup):
+        batched()
+        plain()
+    b_s = timed(batched, steps)
+    p_s = timed(plain, steps)
+    out = {"workload": "decode 96 mixed frames: 32x 720x486i HQ, 32x 1280x720 LT, 24x 1920x1080 Proxy, "
+                       "8x 4096x2160 4444 + 8-bit alpha, loaded matrices",
+           "frames": n, "slices": None, "batched_fps": round(n / b_s, 2), "plain_fps": round(n / p_s, 2),
+           "batched_over_plain": round(p_s / b_s, 3), "ms_per_batch": round(b_s * 1e3, 3),
+           "tuning": dataclasses.asdict(tuning), "autotune_median_us": round(tuned_us, 1),
+           "bytes_per_batch": sum(len(p) for p in packets)}
+    if check:
+        import oracle_lib as O
+        ok = True
+        for i in (0, 32, 64, 88):
+            f, cfg = frames[i]
+            oc = O.config(int(cfg.profile), log2_slice_mbs=cfg.log2_slice_mbs, alpha=int(cfg.alpha),
+                          luma=cfg.luma_matrix, chroma=cfg.chroma_matrix)
+            of = O.Frame(f.width, f.height, int(f.chroma), f.bit_depth, f.data, f.alpha, int(f.interlace))
+            ok &= O.encode(of, oc) == packets[i]
+            od = O.decode(packets[i])
+            ok &= bool(np.array_equal(od.data, outs[i][0].download(f.data.size * 2, dtype=np.uint16)))
+            if f.alpha is not None:
+                ok &= bool(np.array_equal(od.alpha, outs[i][1].download(f.width * f.height * 2, dtype=np.uint16)))
+        out["bit_exact_sample"] = {"frames": [0, 32, 64, 88], "encode_and_decode_equal_oracle": bool(ok)}
+    return out
+
+
+def leg_batch_encode(P, synth, steps, warmup, check):
+    """32 x 3840x2160 422 HQ from 10-bit input, 8-MB slices; frames 16-31
+    use seeded loaded matrices (entries 4-32); every fourth frame adds
+    hard-edged rectangles. "batched" is one encode call for the 32 frames,
+    "plain" one call per frame."""
+    rng = np.random.default_rng(11)
+    frames, items, bufs = [], [], []
+    for i in range(32):
+        f = synth.frame(3840, 2160, P.ChromaFormat.Yuv422, 10, seed=9000 + i, rectangles=(i % 4 == 0))
+        luma = chroma = None
+        if i >= 16:
+            luma = [int(v) for v in rng.integers(4, 33, 64)]
+            chroma = [int(v) for v in rng.integers(4, 33, 64)]
+        cfg = P.Config(P.Profile.Hq, luma_matrix=luma, chroma_matrix=chroma)
+        dd = P.DeviceBuffer(f.data.nbytes + 64)
+        dd.upload(f.data)
+        frames.append((f, cfg))
+        bufs.append(dd)
+        items.append((cfg, f, dd.ptr, 0))
+    last = {}
+
+    def batched():
+        res = P.encode_device_batch(items)
+        for r in res:
+            if isinstance(r, Exception):
+                raise r
+        last["res"] = res
+
+    def plain():
+        for it in items:
+            r = P.encode_device_batch([it])[0]
+            if isinstance(r, Exception):
+                raise r
+
+    for _ in range(warmup):
+        batched()
+        plain()
+    b_s = timed(batched, steps)
+    stage = P.last_encode_timing()
+    batched()
+    sizes = [r[1] for r in last["res"]]
+    # The output arena is reused by the next encode call: keep the sample now.
+    sample = {i: P.download_ptr(*last["res"][i]).tobytes() for i in (0, 17)}
+    target = P.Profile.Hq.target_frame_bytes(3840, 2160)
+    p_s = timed(plain, steps)
+    out = {"workload": "encode 32 x 3840x2160 422 HQ 10-bit, 8-MB slices, half loaded matrices, "
+                       "a quarter with hard-edged rectangles",
+           "frames": 32, "batched_fps": round(32 / b_s, 2), "plain_fps": round(32 / p_s, 2),
+           "batched_over_plain": round(p_s / b_s, 3), "ms_per_batch": round(b_s * 1e3, 3),
+           "stage_us": {"fdct_alpha": round(stage[0], 1), "rate_control": round(stage[1], 1),
+                        "pack": round(stage[2], 1), "all": round(stage[3], 1)},
+           "target_frame_bytes": target, "max_frame_bytes": max(sizes), "min_frame_bytes": min(sizes),
+           "all_at_or_under_target": all(s <= target for s in sizes)}
+    if check:
+        import oracle_lib as O
+        ok = True
+        t0 = time.perf_counter()
+        for i in (0, 17):
+            f, cfg = frames[i]
+            oc = O.config(int(cfg.profile), luma=cfg.luma_matrix, chroma=cfg.chroma_matrix)
+            of = O.Frame(f.width, f.height, int(f.chroma), f.bit_depth, f.data, None, 0)
+            ok &= O.encode(of, oc) == sample[i]
+        cpu_s = (time.perf_counter() - t0) / 2
+        out["bit_exact_sample"] = {"frames": [0, 17], "bytes_equal_oracle": bool(ok)}
+        out["cpu_oracle_encode_s_per_frame"] = round(cpu_s, 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +289,9 @@ def main():
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="mixed,encode",
+                    help="extra legs after the headline decode: mixed (96 mixed frames, batched vs plain "
+                         "decode), encode (32-frame batched encode), comma-separated, or none")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="after the timed steps, one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, for scaling profiler byte counters)")
@@ -226,6 +395,14 @@ def main():
         "encode_stage_us": {"fdct_alpha": round(enc_stage[0], 1), "rate_control": round(enc_stage[1], 1),
                             "pack": round(enc_stage[2], 1), "all": round(enc_stage[3], 1)},
     }
+
+    legs = [x for x in args.legs.split(",") if x and x != "none"]
+    if "mixed" in legs:
+        log("bench: mixed decode leg")
+        result["mixed_decode"] = leg_mixed_decode(P, synth, args.steps, args.warmup, not args.no_cpu)
+    if "encode" in legs:
+        log("bench: batch encode leg")
+        result["batch_encode"] = leg_batch_encode(P, synth, max(1, args.steps // 2), 1, not args.no_cpu)
 
     if not args.no_cpu:
         import oracle_lib as O

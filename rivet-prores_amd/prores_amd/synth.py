@@ -10,8 +10,8 @@ The recipe (numpy PCG64 generator, one seed per frame):
 - alpha: soft-edged ellipses, 0 outside and the maximum at the frame's depth
   inside, with 16-px linear ramps.
 
-Sinusoids are evaluated separably (sin(a+b) = sin a cos b + cos a sin b), so a
-3840x2160 plane costs a few outer products.
+Sinusoids are evaluated separably (sin(a+b) = sin a cos b + cos a sin b): the
+six of a plane are two rank-6 matrix products.
 """
 
 import numpy as np
@@ -20,21 +20,27 @@ from . import ChromaFormat, Frame, Interlace
 
 
 def _plane(rng, h, w, xs, centre, amp, sigma, lo, hi):
+    # sum_j a_j sin(kx_j x + ky_j y + phase_j) as two rank-6 products:
+    # sin(p + q) = sin p cos q + cos p sin q with p = ky y + phase, q = kx x.
     y = np.arange(h, dtype=np.float64)[:, None]
-    x = (np.arange(w, dtype=np.float64) * xs)[None, :]
-    v = np.full((h, w), float(centre), dtype=np.float64)
+    x = (np.arange(w, dtype=np.float64) * xs)[:, None]
+    params = []
     for _ in range(6):
         period = rng.uniform(8.0, 512.0)
         a = rng.uniform(20.0, 120.0) * amp
         theta = rng.uniform(0.0, 2 * np.pi)
         phase = rng.uniform(0.0, 2 * np.pi)
-        kx = 2 * np.pi * np.cos(theta) / period
-        ky = 2 * np.pi * np.sin(theta) / period
-        sx, cx = np.sin(kx * x), np.cos(kx * x)
-        sy, cy = np.sin(ky * y + phase), np.cos(ky * y + phase)
-        v += a * (sx * cy + cx * sy)
+        params.append((a, 2 * np.pi * np.cos(theta) / period, 2 * np.pi * np.sin(theta) / period, phase))
+    a = np.array([p[0] for p in params])[None, :]
+    kx = np.array([p[1] for p in params])[None, :]
+    ky = np.array([p[2] for p in params])[None, :]
+    ph = np.array([p[3] for p in params])[None, :]
+    py = ky * y + ph                      # h x 6
+    qx = kx * x                           # w x 6
+    v = (a * np.sin(py)) @ np.cos(qx).T + (a * np.cos(py)) @ np.sin(qx).T
+    v += float(centre)
     if sigma > 0:
-        v += rng.normal(0.0, sigma, size=(h, w))
+        v += sigma * rng.standard_normal(size=(h, w))
     return v
 
 
