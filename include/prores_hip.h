@@ -237,6 +237,62 @@ void prores_last_batch_timing(double *us5);
  * (host sync for the output sizes included). */
 void prores_last_encode_timing(double *us4);
 
+/* ---- Serving loop (no counterpart in the reference) ----
+ *
+ * Decode and encode requests are submitted at any time from any thread. A
+ * request is admitted while its device-memory footprint (its buffers and its
+ * share of the batch's working arenas) fits hbm_budget, first in first out;
+ * each worker (one for decode, one for encode, on separate streams so their
+ * batches overlap on the device) takes every admissible request of its kind
+ * each time it comes round, up to max_batch_frames, so requests that arrive
+ * while a batch runs ride the next one. Inputs are copied at submit. Results
+ * stay resident in device memory (dev_out / dev_alpha) until released. A
+ * request's result is the same alone and in any mix: each frame's bytes and
+ * samples are those of prores_decode / prores_encode. Times are microseconds
+ * on the server's clock. */
+typedef struct {
+  uint64_t hbm_budget;        /* bytes admitted requests may hold; >= 1 MiB */
+  uint32_t max_batch_frames;  /* frames per device batch; >= 1 */
+  uint32_t decode_bit_depth;  /* 0 = natural, else 8..16 */
+} prores_server_config;
+
+typedef struct prores_server prores_server;
+
+typedef struct {
+  uint64_t id;
+  uint32_t kind;        /* 1 decode, 2 encode */
+  prores_error error;   /* kind 0 = Ok */
+  const void *dev_out;  /* decode: u16 planes in Frame layout; encode: the packet */
+  size_t out_bytes;
+  const void *dev_alpha; /* decode: the alpha plane, or NULL */
+  size_t alpha_bytes;
+  prores_frame_header header; /* decode: the packet's frame header */
+  double submit_us, start_us, done_us;
+  uint32_t batch_frames; /* frames in the device batch that carried it */
+} prores_server_result;
+
+typedef struct {
+  uint64_t submitted, completed, failed;
+  uint64_t decode_batches, encode_batches, max_batch_frames;
+  uint64_t held_bytes, peak_held_bytes, hbm_budget, pending;
+} prores_server_stats;
+
+void prores_server_config_default(prores_server_config *cfg);
+prores_status prores_server_create(const prores_server_config *cfg, prores_server **out, prores_error *err);
+/* Waits for the batches in flight, then frees everything. */
+void prores_server_destroy(prores_server *s);
+/* Returns the request id. A request that can never run (a bad header or
+ * Config, or a footprint over the budget) completes at once with its error. */
+uint64_t prores_server_submit_decode(prores_server *s, const uint8_t *packet, size_t len);
+uint64_t prores_server_submit_encode(prores_server *s, const prores_config *cfg, const prores_frame *frame);
+/* Moves up to max completed results to out; waits up to wait_ms (-1:
+ * until one completes) when none is ready. Returns how many. */
+size_t prores_server_poll(prores_server *s, prores_server_result *out, size_t max, int32_t wait_ms);
+/* Frees a polled result's device memory; PRORES_ERR_CONFIG for an unknown or
+ * unpolled id. */
+prores_status prores_server_release(prores_server *s, uint64_t id);
+void prores_server_get_stats(prores_server *s, prores_server_stats *out);
+
 /* Device memory on the current device, for callers without an allocator of
  * their own (the benchmark, the serving loop, the tests). No counterpart in
  * the reference. kind: 1 host->device, 2 device->host, 3 device->device. */

@@ -3,7 +3,7 @@
 - csrc/*_kernels.hip -> code objects (hipcc --genco --offload-arch=gfx950),
   embedded into the shared library with .incbin and loaded at run time with
   hipModuleLoadData;
-- csrc/*.cpp (host runtime, plain C++ over the HIP runtime API) ->
+- csrc/*.cpp (host runtime and serving loop, plain C++ over the HIP runtime API) ->
   prores_amd/libprores_hip.so, the C ABI of include/prores_hip.h.
 
 Every kernel is compiled with -ffp-contract=off (the reference's f32
@@ -28,11 +28,11 @@ KERNEL_FLAGS = [
     "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "--no-gpu-bundle-output",
 ]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wextra", "-Wno-unused-parameter",
               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
 
 MODULES = [("decode", "decode_kernels.hip"), ("encode", "encode_kernels.hip")]
-HOST_SOURCES = ["host_common.cpp", "decode_host.cpp", "encode_host.cpp"]
+HOST_SOURCES = ["host_common.cpp", "decode_host.cpp", "encode_host.cpp", "serve_host.cpp"]
 
 
 def _run(cmd, verbose):
@@ -66,7 +66,8 @@ def build(verbose=False):
         _run(["g++", *HOST_FLAGS, "-c", "-o", o, os.path.join(CSRC, src)], verbose)
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run(["g++", "-shared", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"], verbose)
+    _run(["g++", "-shared", "-pthread", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"],
+         verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -71,15 +71,11 @@ struct Workspace {
   DevBuf frames, pics, rows, slices, status, first_err, slice_err, coefs, wide, packet, planes, hdr, items, segs;
   hipGraphExec_t graph = nullptr;
   GraphKey key;
-  bool tuned = false;
-  prores_decode_tuning tuning;  // the batch path's settings (the autotuner sets them)
 };
-Workspace &workspace(Device &d) {
-  static Workspace ws[64];
-  return ws[d.id];
+Workspace &workspace(Lane &L) {
+  if (!L.dec) L.dec = new Workspace();
+  return *(Workspace *)L.dec;
 }
-
-double g_last_us[5] = {0, 0, 0, 0, 0};
 
 // Lays out one frame's pictures. `dev_packet` is the device address of the
 // packet; `dev_data` / `dev_alpha` the device output planes.
@@ -182,10 +178,10 @@ void plan_frame(const uint8_t *host, size_t len, uint64_t dev_packet, uint32_t w
 }
 
 template <class T>
-Err upload(Device &d, DevBuf &buf, const std::vector<T> &v) {
+Err upload(Lane &L, DevBuf &buf, const std::vector<T> &v) {
   Err e = buf.reserve(v.size() * sizeof(T) + 16);
   if (!e.ok() || v.empty()) return e;
-  return check(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, d.stream),
+  return check(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, L.stream),
                "hipMemcpyAsync(plan)");
 }
 
@@ -234,8 +230,8 @@ Err validate_tuning(const prores_decode_tuning &t) {
 
 // Runs the kernels over a planned batch and turns the device's error records
 // into one Result per frame (`results`, in frame order).
-Err run_batch(Device &d, Batch &b, const prores_decode_tuning &t, std::vector<Err> &results) {
-  Workspace &ws = workspace(d);
+Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::vector<Err> &results) {
+  Workspace &ws = workspace(L);
   Err e;
   if (!(e = validate_tuning(t)).ok()) return e;
   const uint64_t G = b.n_slices;
@@ -251,11 +247,11 @@ Err run_batch(Device &d, Batch &b, const prores_decode_tuning &t, std::vector<Er
   hdr.n_idct_items = (uint32_t)n_items;
   hdr.chunk = t.persistent ? t.chunk : 1;
   std::vector<DecBatch> hv{hdr};
-  if (!(e = upload(d, ws.frames, b.frames)).ok()) return e;
-  if (!(e = upload(d, ws.pics, b.pics)).ok()) return e;
-  if (!(e = upload(d, ws.rows, b.rows)).ok()) return e;
-  if (!(e = upload(d, ws.segs, segs)).ok()) return e;
-  if (!(e = upload(d, ws.hdr, hv)).ok()) return e;
+  if (!(e = upload(L, ws.frames, b.frames)).ok()) return e;
+  if (!(e = upload(L, ws.pics, b.pics)).ok()) return e;
+  if (!(e = upload(L, ws.rows, b.rows)).ok()) return e;
+  if (!(e = upload(L, ws.segs, segs)).ok()) return e;
+  if (!(e = upload(L, ws.hdr, hv)).ok()) return e;
   if (!(e = ws.items.reserve(n_items * sizeof(IdctItem) + 64)).ok()) return e;
   if (!(e = ws.slices.reserve(G * sizeof(SliceRec) + 64)).ok()) return e;
   if (!(e = ws.slice_err.reserve(G * 16 + 64)).ok()) return e;
@@ -283,30 +279,30 @@ Err run_batch(Device &d, Batch &b, const prores_decode_tuning &t, std::vector<Er
   }
   auto enqueue = [&](bool timed) -> Err {
     Err x;
-    if (timed) (void)hipEventRecord(d.ev[0], d.stream);
+    if (timed) (void)hipEventRecord(L.ev[0], L.stream);
     {
       void *args[] = {&B, &pics, &rows, &slices, &status, &serr, &ferr, &segp, &itemp};
-      if (!(x = launch(d, d.k_slice_table, grids[0], 1, 256, args)).ok()) return x;
+      if (!(x = launch(L, d.k_slice_table, grids[0], 1, 256, args)).ok()) return x;
     }
-    if (timed) (void)hipEventRecord(d.ev[1], d.stream);
+    if (timed) (void)hipEventRecord(L.ev[1], L.stream);
     {
       void *args[] = {&B, &frames, &pics, &slices, &status, &coefs, &wide, &serr, &ferr, &pers};
-      if (!(x = launch(d, d.k_entropy, grids[1], 1, t.entropy_wg, args)).ok()) return x;
+      if (!(x = launch(L, d.k_entropy, grids[1], 1, t.entropy_wg, args)).ok()) return x;
     }
-    if (timed) (void)hipEventRecord(d.ev[2], d.stream);
+    if (timed) (void)hipEventRecord(L.ev[2], L.stream);
     if (b.any_alpha) {
       void *args[] = {&B, &frames, &pics, &slices, &status, &serr, &ferr, &pers};
-      if (!(x = launch(d, d.k_alpha, grids[2], 1, 64, args)).ok()) return x;
+      if (!(x = launch(L, d.k_alpha, grids[2], 1, 64, args)).ok()) return x;
     }
-    if (timed) (void)hipEventRecord(d.ev[3], d.stream);
+    if (timed) (void)hipEventRecord(L.ev[3], L.stream);
     {
       void *args[] = {&tables, &B, &frames, &pics, &slices, &status, &serr, &coefs, &wide, &itemp, &pers};
-      if (!(x = launch(d, d.k_idct, grids[3], 1, 256, args)).ok()) return x;
+      if (!(x = launch(L, d.k_idct, grids[3], 1, 256, args)).ok()) return x;
     }
     return Err{};
   };
   if (np == 0) {
-    g_last_us[0] = g_last_us[1] = g_last_us[2] = g_last_us[3] = g_last_us[4] = 0;
+    L.dec_us[0] = L.dec_us[1] = L.dec_us[2] = L.dec_us[3] = L.dec_us[4] = 0;
   } else if (t.graph) {
     GraphKey key;
     memset(&key, 0, sizeof key);
@@ -321,10 +317,10 @@ Err run_batch(Device &d, Batch &b, const prores_decode_tuning &t, std::vector<Er
         ws.graph = nullptr;
       }
       hipGraph_t graph = nullptr;
-      if (!(e = check(hipStreamBeginCapture(d.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture")).ok())
+      if (!(e = check(hipStreamBeginCapture(L.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture")).ok())
         return e;
       Err ce = enqueue(false);
-      Err ee = check(hipStreamEndCapture(d.stream, &graph), "hipStreamEndCapture");
+      Err ee = check(hipStreamEndCapture(L.stream, &graph), "hipStreamEndCapture");
       if (!ce.ok()) {
         if (graph) (void)hipGraphDestroy(graph);
         return ce;
@@ -338,30 +334,30 @@ Err run_batch(Device &d, Batch &b, const prores_decode_tuning &t, std::vector<Er
       }
       ws.key = key;
     }
-    (void)hipEventRecord(d.ev[0], d.stream);
-    if (!(e = check(hipGraphLaunch(ws.graph, d.stream), "hipGraphLaunch")).ok()) return e;
-    (void)hipEventRecord(d.ev[4], d.stream);
+    (void)hipEventRecord(L.ev[0], L.stream);
+    if (!(e = check(hipGraphLaunch(ws.graph, L.stream), "hipGraphLaunch")).ok()) return e;
+    (void)hipEventRecord(L.ev[4], L.stream);
   } else {
     if (!(e = enqueue(true)).ok()) return e;
-    (void)hipEventRecord(d.ev[4], d.stream);
+    (void)hipEventRecord(L.ev[4], L.stream);
   }
   std::vector<uint32_t> st(np), fe(np);
   if (np) {
-    if (!(e = check(hipMemcpyAsync(st.data(), ws.status.p, np * 4, hipMemcpyDeviceToHost, d.stream),
+    if (!(e = check(hipMemcpyAsync(st.data(), ws.status.p, np * 4, hipMemcpyDeviceToHost, L.stream),
                     "status")).ok())
       return e;
-    if (!(e = check(hipMemcpyAsync(fe.data(), ws.first_err.p, np * 4, hipMemcpyDeviceToHost, d.stream),
+    if (!(e = check(hipMemcpyAsync(fe.data(), ws.first_err.p, np * 4, hipMemcpyDeviceToHost, L.stream),
                     "first_err")).ok())
       return e;
   }
-  if (!(e = check(hipStreamSynchronize(d.stream), "hipStreamSynchronize")).ok()) return e;
+  if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
   if (np) {
     float ms[4] = {0, 0, 0, 0}, all = 0;
     if (!t.graph)
-      for (int i = 0; i < 4; i++) (void)hipEventElapsedTime(&ms[i], d.ev[i], d.ev[i + 1]);
-    (void)hipEventElapsedTime(&all, d.ev[0], d.ev[4]);
-    for (int i = 0; i < 4; i++) g_last_us[i] = ms[i] * 1000.0;
-    g_last_us[4] = all * 1000.0;
+      for (int i = 0; i < 4; i++) (void)hipEventElapsedTime(&ms[i], L.ev[i], L.ev[i + 1]);
+    (void)hipEventElapsedTime(&all, L.ev[0], L.ev[4]);
+    for (int i = 0; i < 4; i++) L.dec_us[i] = ms[i] * 1000.0;
+    L.dec_us[4] = all * 1000.0;
   }
 
   results.assign(b.plans.size(), Err{});
@@ -480,16 +476,17 @@ static prores_status decode_common(const prores_decoder *d, const uint8_t *data,
   for (int i = 0; i < 3; i++) out->planes[i] = planes[i];
 
   Device *dev = nullptr;
-  if (!(e = device(dev)).ok()) {
+  Lane *lane0 = nullptr;
+  if (!(e = device(dev)).ok() || !(e = lane(*dev, 0, lane0)).ok()) {
     fill_error(err, e);
     return (prores_status)e.kind;
   }
-  std::lock_guard<std::mutex> g(dev->mu);
-  Workspace &ws = workspace(*dev);
+  std::lock_guard<std::mutex> g(lane0->mu);
+  Workspace &ws = workspace(*lane0);
   const size_t fsz = h.frame_size;
   const size_t plane_bytes = (total * 2 + 255) & ~(size_t)255;
   if ((e = ws.packet.reserve(fsz + 128)).ok() && (e = ws.planes.reserve(plane_bytes + alpha_n * 2 + 64)).ok() &&
-      (e = check(hipMemcpyAsync(ws.packet.p, data, fsz, hipMemcpyHostToDevice, dev->stream), "H2D packet")).ok()) {
+      (e = check(hipMemcpyAsync(ws.packet.p, data, fsz, hipMemcpyHostToDevice, lane0->stream), "H2D packet")).ok()) {
     Batch b;
     uint16_t *dd = ws.planes.as<uint16_t>();
     uint16_t *da = alpha_n ? (uint16_t *)((uint8_t *)ws.planes.p + plane_bytes) : nullptr;
@@ -497,7 +494,7 @@ static prores_status decode_common(const prores_decoder *d, const uint8_t *data,
     std::vector<Err> results;
     prores_decode_tuning t;
     prores_decode_tuning_default(&t);
-    e = run_batch(*dev, b, t, results);
+    e = run_batch(*dev, *lane0, b, t, results);
     if (e.ok()) e = results[0];
     if (e.ok()) {
       e = check(hipMemcpy(out->data, dd, total * 2, hipMemcpyDeviceToHost), "D2H planes");
@@ -519,35 +516,50 @@ prores_status prores_decode_into(const prores_decoder *d, const uint8_t *data, s
   return decode_common(d, data, len, out, err, false);
 }
 
+}  // extern "C"
+
+namespace prores_hip {
+
+Err decode_batch_on(Device &d, Lane &L, uint32_t bit_depth, size_t n, const prores_dev_decode_job *jobs,
+                    const prores_decode_tuning *tuning, prores_error *errs) {
+  prores_decode_tuning t;
+  if (tuning) {
+    t = *tuning;
+  } else {
+    std::lock_guard<std::mutex> g(d.mu);
+    t = d.tuning;
+  }
+  Batch b;
+  for (size_t i = 0; i < n; i++)
+    plan_frame(jobs[i].host_packet, jobs[i].len, (uint64_t)jobs[i].dev_packet, bit_depth, jobs[i].dev_data,
+               jobs[i].dev_alpha, b, nullptr);
+  std::vector<Err> results;
+  Err e = run_batch(d, L, b, t, results);
+  if (!e.ok()) {
+    for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
+    return e;
+  }
+  for (size_t i = 0; i < n; i++) fill_error(&errs[i], results[i]);
+  return Err{};
+}
+
+}  // namespace prores_hip
+
+extern "C" {
+
 static prores_status batch_common(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,
                                   const prores_decode_tuning *tuning, prores_error *errs) {
   Device *dev = nullptr;
+  Lane *lane0 = nullptr;
   Err e = device(dev);
+  if (e.ok()) e = lane(*dev, 0, lane0);
   if (!e.ok()) {
     for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
     return (prores_status)e.kind;
   }
-  std::lock_guard<std::mutex> g(dev->mu);
-  Workspace &ws = workspace(*dev);
-  prores_decode_tuning t;
-  if (tuning)
-    t = *tuning;
-  else if (ws.tuned)
-    t = ws.tuning;
-  else
-    prores_decode_tuning_default(&t);
-  Batch b;
-  for (size_t i = 0; i < n; i++)
-    plan_frame(jobs[i].host_packet, jobs[i].len, (uint64_t)jobs[i].dev_packet, d->bit_depth, jobs[i].dev_data,
-               jobs[i].dev_alpha, b, nullptr);
-  std::vector<Err> results;
-  e = run_batch(*dev, b, t, results);
-  if (!e.ok()) {
-    for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
-    return (prores_status)e.kind;
-  }
-  for (size_t i = 0; i < n; i++) fill_error(&errs[i], results[i]);
-  return PRORES_OK;
+  std::lock_guard<std::mutex> g(lane0->mu);
+  e = decode_batch_on(*dev, *lane0, d->bit_depth, n, jobs, tuning, errs);
+  return e.ok() ? PRORES_OK : (prores_status)e.kind;
 }
 
 prores_status prores_decode_device_batch(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,
@@ -575,8 +587,8 @@ prores_status prores_set_decode_tuning(const prores_decode_tuning *t, prores_err
   if (e.ok()) e = device(dev);
   if (e.ok()) {
     std::lock_guard<std::mutex> g(dev->mu);
-    workspace(*dev).tuning = *t;
-    workspace(*dev).tuned = true;
+    dev->tuning = *t;
+    dev->tuned = true;
   }
   fill_error(err, e);
   return (prores_status)e.kind;
@@ -587,19 +599,21 @@ void prores_get_decode_tuning(prores_decode_tuning *t) {
   prores_decode_tuning_default(t);
   if (!device(dev).ok()) return;
   std::lock_guard<std::mutex> g(dev->mu);
-  if (workspace(*dev).tuned) *t = workspace(*dev).tuning;
+  *t = dev->tuning;
 }
 
 prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,
                                      uint32_t reps, prores_decode_tuning *best, double *best_us,
                                      prores_error *err) {
   Device *dev = nullptr;
+  Lane *lane0 = nullptr;
   Err e = device(dev);
+  if (e.ok()) e = lane(*dev, 0, lane0);
   if (!e.ok()) {
     fill_error(err, e);
     return (prores_status)e.kind;
   }
-  std::lock_guard<std::mutex> g(dev->mu);
+  std::lock_guard<std::mutex> g(lane0->mu);
   Batch b;
   for (size_t i = 0; i < n; i++)
     plan_frame(jobs[i].host_packet, jobs[i].len, (uint64_t)jobs[i].dev_packet, d->bit_depth, jobs[i].dev_data,
@@ -621,11 +635,11 @@ prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const pr
   for (const prores_decode_tuning &c : cands) {
     times.clear();
     for (uint32_t r = 0; r <= reps; r++) {
-      if (!(e = run_batch(*dev, b, c, results)).ok()) {
+      if (!(e = run_batch(*dev, *lane0, b, c, results)).ok()) {
         fill_error(err, e);
         return (prores_status)e.kind;
       }
-      if (r) times.push_back(g_last_us[4]);
+      if (r) times.push_back(lane0->dec_us[4]);
     }
     std::sort(times.begin(), times.end());
     const double med = times[times.size() / 2];
@@ -634,9 +648,11 @@ prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const pr
       best_c = c;
     }
   }
-  Workspace &ws = workspace(*dev);
-  ws.tuning = best_c;
-  ws.tuned = true;
+  {
+    std::lock_guard<std::mutex> g2(dev->mu);
+    dev->tuning = best_c;
+    dev->tuned = true;
+  }
   if (best) *best = best_c;
   if (best_us) *best_us = best_t;
   fill_error(err, Err{});
@@ -644,7 +660,12 @@ prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const pr
 }
 
 void prores_last_batch_timing(double *us5) {
-  for (int i = 0; i < 5; i++) us5[i] = g_last_us[i];
+  Device *dev = nullptr;
+  Lane *lane0 = nullptr;
+  for (int i = 0; i < 5; i++) us5[i] = 0;
+  if (!device(dev).ok() || !lane(*dev, 0, lane0).ok()) return;
+  std::lock_guard<std::mutex> g(lane0->mu);
+  for (int i = 0; i < 5; i++) us5[i] = lane0->dec_us[i];
 }
 
 const char *prores_version(void) { return "prores-hip 0.1 gfx950"; }

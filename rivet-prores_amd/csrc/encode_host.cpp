@@ -238,34 +238,32 @@ struct EncWorkspace {
   DevBuf frames, pics, slices, states, qtabs, hdrs, coefs, alpha_scratch, alpha_len, size_tab, deep, used, qidx,
       slice_out, frame_size, out, planes, alpha_in;
 };
-EncWorkspace &enc_workspace(Device &d) {
-  static EncWorkspace ws[64];
-  return ws[d.id];
+EncWorkspace &enc_workspace(Lane &L) {
+  if (!L.enc) L.enc = new EncWorkspace();
+  return *(EncWorkspace *)L.enc;
 }
 
-double g_enc_us[4] = {0, 0, 0, 0};
-
 template <class T>
-Err upload(Device &d, DevBuf &buf, const std::vector<T> &v) {
+Err upload(Lane &L, DevBuf &buf, const std::vector<T> &v) {
   Err e = buf.reserve(v.size() * sizeof(T) + 16);
   if (!e.ok() || v.empty()) return e;
-  return check(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, d.stream),
+  return check(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, L.stream),
                "hipMemcpyAsync(encode plan)");
 }
 
 // Runs the kernels over a planned batch; on return the packets are in the
 // output arena at `offsets[i]` with `sizes[i]` bytes (batch frame order).
-Err run_encode(Device &d, EncBatch &b, std::vector<uint64_t> &offsets, std::vector<uint64_t> &sizes) {
-  EncWorkspace &ws = enc_workspace(d);
+Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, std::vector<uint64_t> &sizes) {
+  EncWorkspace &ws = enc_workspace(L);
   Err e;
   const uint64_t G = b.slices.size();
   const uint32_t NP = (uint32_t)b.pics.size(), NF = (uint32_t)b.frames.size();
-  if (!(e = upload(d, ws.frames, b.frames)).ok()) return e;
-  if (!(e = upload(d, ws.pics, b.pics)).ok()) return e;
-  if (!(e = upload(d, ws.slices, b.slices)).ok()) return e;
-  if (!(e = upload(d, ws.states, b.states)).ok()) return e;
-  if (!(e = upload(d, ws.qtabs, b.qtabs)).ok()) return e;
-  if (!(e = upload(d, ws.hdrs, b.hdrs)).ok()) return e;
+  if (!(e = upload(L, ws.frames, b.frames)).ok()) return e;
+  if (!(e = upload(L, ws.pics, b.pics)).ok()) return e;
+  if (!(e = upload(L, ws.slices, b.slices)).ok()) return e;
+  if (!(e = upload(L, ws.states, b.states)).ok()) return e;
+  if (!(e = upload(L, ws.qtabs, b.qtabs)).ok()) return e;
+  if (!(e = upload(L, ws.hdrs, b.hdrs)).ok()) return e;
   if (!(e = ws.coefs.reserve(b.n_coefs * 4 + 64)).ok()) return e;
   if (!(e = ws.alpha_scratch.reserve(G * ALPHA_SLICE_CAP + 64)).ok()) return e;
   if (!(e = ws.alpha_len.reserve(G * 4 + 64)).ok()) return e;
@@ -280,42 +278,42 @@ Err run_encode(Device &d, EncBatch &b, std::vector<uint64_t> &offsets, std::vect
   void *alen = ws.alpha_len.p, *stab = ws.size_tab.p, *used = ws.used.p, *qidx = ws.qidx.p;
   void *sout = ws.slice_out.p, *fsize = ws.frame_size.p, *tables = d.tables, *deep = ws.deep.p;
   uint32_t g32 = (uint32_t)G, np32 = NP, window = WINDOW;
-  (void)hipEventRecord(d.ev[0], d.stream);
+  (void)hipEventRecord(L.ev[0], L.stream);
   {
     void *args[] = {&tables, &frames, &pics, &slices, &coefs};
-    if (!(e = launch(d, d.k_fdct, g32, 1, 256, args)).ok()) return e;
+    if (!(e = launch(L, d.k_fdct, g32, 1, 256, args)).ok()) return e;
   }
   {
     void *args[] = {&frames, &pics, &slices, &g32, &ascr, &alen, &states};
-    if (!(e = launch(d, d.k_alpha_enc, g32, 1, 64, args)).ok()) return e;
+    if (!(e = launch(L, d.k_alpha_enc, g32, 1, 64, args)).ok()) return e;
   }
-  (void)hipEventRecord(d.ev[1], d.stream);
+  (void)hipEventRecord(L.ev[1], L.stream);
   // qi = 224, then at most 8 bisection steps (224 < 2^8).
   for (int probe = 0; probe < 9; probe++) {
     void *a1[] = {&frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states};
-    if (!(e = launch(d, d.k_size, (g32 + 3) / 4, 1, 256, a1)).ok()) return e;
+    if (!(e = launch(L, d.k_size, (g32 + 3) / 4, 1, 256, a1)).ok()) return e;
     void *a2[] = {&pics, &states, &np32, &window};
-    if (!(e = launch(d, d.k_search, (np32 + 63) / 64, 1, 64, a2)).ok()) return e;
+    if (!(e = launch(L, d.k_search, (np32 + 63) / 64, 1, 64, a2)).ok()) return e;
   }
   {
     void *args[] = {&frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &stab};
-    if (!(e = launch(d, d.k_window, (g32 + 3) / 4, 1, 256, args)).ok()) return e;
+    if (!(e = launch(L, d.k_window, (g32 + 3) / 4, 1, 256, args)).ok()) return e;
   }
-  if (G && !(e = check(hipMemsetAsync(ws.deep.p, 0, G * ENC_DEEP * 4, d.stream), "memset")).ok()) return e;
+  if (G && !(e = check(hipMemsetAsync(ws.deep.p, 0, G * ENC_DEEP * 4, L.stream), "memset")).ok()) return e;
   {
     void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &states, &stab, &deep, &used, &qidx};
-    if (!(e = launch(d, d.k_shareout, np32, 1, 64, args)).ok()) return e;
+    if (!(e = launch(L, d.k_shareout, np32, 1, 64, args)).ok()) return e;
   }
   {
     void *args[] = {&frames, &pics, &used, &sout, &fsize};
-    if (!(e = launch(d, d.k_offsets, NF, 1, 256, args)).ok()) return e;
+    if (!(e = launch(L, d.k_offsets, NF, 1, 256, args)).ok()) return e;
   }
-  (void)hipEventRecord(d.ev[2], d.stream);
+  (void)hipEventRecord(L.ev[2], L.stream);
   sizes.assign(NF, 0);
-  if (NF && !(e = check(hipMemcpyAsync(sizes.data(), ws.frame_size.p, NF * 8, hipMemcpyDeviceToHost, d.stream),
+  if (NF && !(e = check(hipMemcpyAsync(sizes.data(), ws.frame_size.p, NF * 8, hipMemcpyDeviceToHost, L.stream),
                         "frame sizes")).ok())
     return e;
-  if (!(e = check(hipStreamSynchronize(d.stream), "hipStreamSynchronize")).ok()) return e;
+  if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
   offsets.assign(NF, 0);
   uint64_t total = 0;
   for (uint32_t f = 0; f < NF; f++) {
@@ -325,28 +323,68 @@ Err run_encode(Device &d, EncBatch &b, std::vector<uint64_t> &offsets, std::vect
   }
   if (!(e = ws.out.reserve(total + 256)).ok()) return e;
   for (uint32_t f = 0; f < NF; f++) b.frames[f].out = (uint64_t)ws.out.p + offsets[f];
-  if (!(e = upload(d, ws.frames, b.frames)).ok()) return e;
-  (void)hipEventRecord(d.ev[3], d.stream);
+  if (!(e = upload(L, ws.frames, b.frames)).ok()) return e;
+  (void)hipEventRecord(L.ev[3], L.stream);
   {
     void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &ascr, &used, &qidx, &sout, &fsize, &hdrs};
-    if (!(e = launch(d, d.k_pack, g32, 1, 64, args)).ok()) return e;
+    if (!(e = launch(L, d.k_pack, g32, 1, 64, args)).ok()) return e;
   }
-  hipEvent_t done = d.ev[4];
-  (void)hipEventRecord(done, d.stream);
-  if (!(e = check(hipStreamSynchronize(d.stream), "hipStreamSynchronize")).ok()) return e;
+  hipEvent_t done = L.ev[4];
+  (void)hipEventRecord(done, L.stream);
+  if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
   float a = 0, r = 0, p = 0, all = 0;
-  (void)hipEventElapsedTime(&a, d.ev[0], d.ev[1]);
-  (void)hipEventElapsedTime(&r, d.ev[1], d.ev[2]);
-  (void)hipEventElapsedTime(&p, d.ev[3], done);
-  (void)hipEventElapsedTime(&all, d.ev[0], done);
-  g_enc_us[0] = a * 1000.0;
-  g_enc_us[1] = r * 1000.0;
-  g_enc_us[2] = p * 1000.0;
-  g_enc_us[3] = all * 1000.0;
+  (void)hipEventElapsedTime(&a, L.ev[0], L.ev[1]);
+  (void)hipEventElapsedTime(&r, L.ev[1], L.ev[2]);
+  (void)hipEventElapsedTime(&p, L.ev[3], done);
+  (void)hipEventElapsedTime(&all, L.ev[0], done);
+  L.enc_us[0] = a * 1000.0;
+  L.enc_us[1] = r * 1000.0;
+  L.enc_us[2] = p * 1000.0;
+  L.enc_us[3] = all * 1000.0;
   return Err{};
 }
 
 }  // namespace
+}  // namespace prores_hip
+
+namespace prores_hip {
+
+Err encode_check(const prores_config &cfg, const prores_frame &frame) {
+  EncBatch probe;
+  return plan_encode(cfg, frame, 0, 0, probe);
+}
+
+Err encode_batch_on(Device &d, Lane &L, size_t n, prores_dev_encode_job *jobs, prores_error *errs) {
+  Err e = ensure_encoder(d);
+  if (!e.ok()) {
+    for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
+    return e;
+  }
+  EncBatch b;
+  std::vector<int> frame_of(n, -1);
+  for (size_t i = 0; i < n; i++) {
+    jobs[i].dev_out = nullptr;
+    jobs[i].out_len = 0;
+    const uint32_t before = (uint32_t)b.frames.size();
+    Err je = plan_encode(*jobs[i].cfg, jobs[i].frame, (uint64_t)jobs[i].frame.data, (uint64_t)jobs[i].frame.alpha, b);
+    fill_error(&errs[i], je);
+    if (je.ok()) frame_of[i] = (int)before;
+  }
+  std::vector<uint64_t> offsets, sizes;
+  if (!b.frames.empty()) e = run_encode(d, L, b, offsets, sizes);
+  if (!e.ok()) {
+    for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
+    return e;
+  }
+  EncWorkspace &ws = enc_workspace(L);
+  for (size_t i = 0; i < n; i++)
+    if (frame_of[i] >= 0) {
+      jobs[i].dev_out = (const uint8_t *)ws.out.p + offsets[frame_of[i]];
+      jobs[i].out_len = sizes[frame_of[i]];
+    }
+  return Err{};
+}
+
 }  // namespace prores_hip
 
 using namespace prores_hip;
@@ -389,25 +427,26 @@ prores_status prores_encode(const prores_config *cfg, const prores_frame *frame,
     return (prores_status)e.kind;
   }
   Device *dev = nullptr;
-  if (!(e = device(dev)).ok()) {
+  Lane *lane0 = nullptr;
+  if (!(e = device(dev)).ok() || !(e = lane(*dev, 0, lane0)).ok()) {
     fill_error(err, e);
     return (prores_status)e.kind;
   }
-  std::lock_guard<std::mutex> g(dev->mu);
+  std::lock_guard<std::mutex> g(lane0->mu);
   if (!(e = ensure_encoder(*dev)).ok()) {
     fill_error(err, e);
     return (prores_status)e.kind;
   }
-  EncWorkspace &ws = enc_workspace(*dev);
+  EncWorkspace &ws = enc_workspace(*lane0);
   const size_t nsamp = frame->data_len, nalpha = frame->alpha ? frame->alpha_len : 0;
   if ((e = ws.planes.reserve(nsamp * 2 + 64)).ok() && (e = ws.alpha_in.reserve(nalpha * 2 + 64)).ok() &&
-      (e = check(hipMemcpyAsync(ws.planes.p, frame->data, nsamp * 2, hipMemcpyHostToDevice, dev->stream), "H2D frame")).ok() &&
-      (!nalpha || (e = check(hipMemcpyAsync(ws.alpha_in.p, frame->alpha, nalpha * 2, hipMemcpyHostToDevice, dev->stream),
+      (e = check(hipMemcpyAsync(ws.planes.p, frame->data, nsamp * 2, hipMemcpyHostToDevice, lane0->stream), "H2D frame")).ok() &&
+      (!nalpha || (e = check(hipMemcpyAsync(ws.alpha_in.p, frame->alpha, nalpha * 2, hipMemcpyHostToDevice, lane0->stream),
                              "H2D alpha")).ok())) {
     EncBatch b;
     e = plan_encode(*cfg, *frame, (uint64_t)ws.planes.p, (uint64_t)ws.alpha_in.p, b);
     std::vector<uint64_t> offsets, sizes;
-    if (e.ok()) e = run_encode(*dev, b, offsets, sizes);
+    if (e.ok()) e = run_encode(*dev, *lane0, b, offsets, sizes);
     if (e.ok()) {
       *out = (uint8_t *)malloc(sizes[0] ? sizes[0] : 1);
       if (!*out) e = Err{PRORES_ERR_DEVICE, "host allocation failed"};
@@ -426,40 +465,25 @@ prores_status prores_encode(const prores_config *cfg, const prores_frame *frame,
 
 prores_status prores_encode_device_batch(size_t n, prores_dev_encode_job *jobs, prores_error *errs) {
   Device *dev = nullptr;
+  Lane *lane0 = nullptr;
   Err e = device(dev);
+  if (e.ok()) e = lane(*dev, 0, lane0);
   if (e.ok()) {
-    std::lock_guard<std::mutex> g(dev->mu);
-    e = ensure_encoder(*dev);
-    if (e.ok()) {
-      EncBatch b;
-      std::vector<int> frame_of(n, -1);
-      for (size_t i = 0; i < n; i++) {
-        jobs[i].dev_out = nullptr;
-        jobs[i].out_len = 0;
-        const uint32_t before = (uint32_t)b.frames.size();
-        Err je = plan_encode(*jobs[i].cfg, jobs[i].frame, (uint64_t)jobs[i].frame.data, (uint64_t)jobs[i].frame.alpha, b);
-        fill_error(&errs[i], je);
-        if (je.ok()) frame_of[i] = (int)before;
-      }
-      std::vector<uint64_t> offsets, sizes;
-      if (!b.frames.empty()) e = run_encode(*dev, b, offsets, sizes);
-      if (e.ok()) {
-        EncWorkspace &ws = enc_workspace(*dev);
-        for (size_t i = 0; i < n; i++)
-          if (frame_of[i] >= 0) {
-            jobs[i].dev_out = (const uint8_t *)ws.out.p + offsets[frame_of[i]];
-            jobs[i].out_len = sizes[frame_of[i]];
-          }
-        return PRORES_OK;
-      }
-    }
+    std::lock_guard<std::mutex> g(lane0->mu);
+    e = encode_batch_on(*dev, *lane0, n, jobs, errs);
+    return e.ok() ? PRORES_OK : (prores_status)e.kind;
   }
   for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
   return (prores_status)e.kind;
 }
 
 void prores_last_encode_timing(double *us4) {
-  for (int i = 0; i < 4; i++) us4[i] = g_enc_us[i];
+  Device *dev = nullptr;
+  Lane *lane0 = nullptr;
+  for (int i = 0; i < 4; i++) us4[i] = 0;
+  if (!device(dev).ok() || !lane(*dev, 0, lane0).ok()) return;
+  std::lock_guard<std::mutex> g(lane0->mu);
+  for (int i = 0; i < 4; i++) us4[i] = lane0->enc_us[i];
 }
 
 }  // extern "C"

@@ -238,9 +238,9 @@ Err DevBuf::reserve(size_t bytes) {
   return Err{};
 }
 
-Err launch(Device &d, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args) {
+Err launch(Lane &L, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args) {
   if (gx == 0 || gy == 0) return Err{};
-  return check(hipModuleLaunchKernel(f, gx, gy, 1, bx, 1, 1, 0, d.stream, args, nullptr),
+  return check(hipModuleLaunchKernel(f, gx, gy, 1, bx, 1, 1, 0, L.stream, args, nullptr),
                "hipModuleLaunchKernel");
 }
 
@@ -266,15 +266,35 @@ static Err init_device(Device &d, int id) {
   if (!(e = check(hipMalloc(&d.tables, sizeof(DevTables)), "hipMalloc(tables)")).ok()) return e;
   if (!(e = check(hipMemcpy(d.tables, &host, sizeof host, hipMemcpyHostToDevice), "tables")).ok())
     return e;
-  if (!(e = check(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking), "hipStreamCreate")).ok())
-    return e;
-  for (auto &ev : d.ev)
-    if (!(e = check(hipEventCreate(&ev), "hipEventCreate")).ok()) return e;
+  prores_decode_tuning_default(&d.tuning);
+  return Err{};
+}
+
+Err lane(Device &d, int index, Lane *&out) {
+  if (index < 0 || index >= MAX_LANES) return Err{PRORES_ERR_DEVICE, "lane index out of range"};
+  std::lock_guard<std::mutex> g(d.mu);
+  if (!d.lanes[index]) {
+    Lane *L = new Lane();
+    L->index = index;
+    Err e = check(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto &ev : L->ev)
+      if (e.ok()) e = check(hipEventCreate(&ev), "hipEventCreate");
+    if (!e.ok()) {
+      for (auto &ev : L->ev)
+        if (ev) (void)hipEventDestroy(ev);
+      if (L->stream) (void)hipStreamDestroy(L->stream);
+      delete L;
+      return e;
+    }
+    d.lanes[index] = L;
+  }
+  out = d.lanes[index];
   return Err{};
 }
 
 Err ensure_encoder(Device &d) {
-  // The encoder's code object is loaded on the first encode (callers hold d.mu).
+  // The encoder's code object is loaded on the first encode.
+  std::lock_guard<std::mutex> g(d.mu);
   if (d.enc_mod) return Err{};
   Err e = check(hipModuleLoadData(&d.enc_mod, prores_encode_co), "hipModuleLoadData(encode)");
   if (!e.ok()) return e;

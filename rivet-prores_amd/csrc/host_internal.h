@@ -66,7 +66,24 @@ struct PicHeader {
 };
 Err parse_picture_header(const uint8_t *d, size_t len, PicHeader &out);
 
-// The per-device context: modules, kernels, constant tables, one stream.
+// One line of work on a device: a stream, its timing events, the decode and
+// encode workspaces, and the lock that makes its batches one at a time.
+// Lane 0 serves the per-call API; the serving loop's workers take other
+// lanes, so their batches overlap on the device. Streams keep the default
+// priority.
+struct Lane {
+  int index = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  std::mutex mu;
+  void *dec = nullptr;  // decode workspace (decode_host.cpp)
+  void *enc = nullptr;  // encode workspace (encode_host.cpp)
+  double dec_us[5] = {0, 0, 0, 0, 0};  // last decode batch by stage
+  double enc_us[4] = {0, 0, 0, 0};     // last encode batch by stage
+};
+constexpr int MAX_LANES = 8;
+
+// The per-device context: modules, kernels, constant tables, lanes.
 struct Device {
   int id = -1;
   hipModule_t dec_mod = nullptr, enc_mod = nullptr;
@@ -75,13 +92,16 @@ struct Device {
                 k_shareout = nullptr, k_offsets = nullptr, k_pack = nullptr,
                 k_window = nullptr;
   DevTables *tables = nullptr;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  std::mutex mu;  // one batch at a time per device
+  std::mutex mu;  // lane creation, encoder loading, the batch tuning
+  Lane *lanes[MAX_LANES] = {nullptr};
+  prores_decode_tuning tuning;
+  bool tuned = false;
 };
 // The context of the current device, created on first use.
 Err device(Device *&out);
-// Loads the encoder's kernels into `d` (once).
+// Lane `index` of `d`, created on first use.
+Err lane(Device &d, int index, Lane *&out);
+// Loads the encoder's kernels into `d` (once; takes d.mu).
 Err ensure_encoder(Device &d);
 
 // A grow-only device allocation.
@@ -95,9 +115,17 @@ struct DevBuf {
   }
 };
 
-Err launch(Device &d, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args);
+Err launch(Lane &L, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args);
 Err check(hipError_t e, const char *where);
 
 void build_tables(DevTables &t);
+
+// Batches on a given lane (the caller holds L.mu): the bodies of
+// prores_decode_device_batch and prores_encode_device_batch.
+Err decode_batch_on(Device &d, Lane &L, uint32_t bit_depth, size_t n, const prores_dev_decode_job *jobs,
+                    const prores_decode_tuning *t, prores_error *errs);
+Err encode_batch_on(Device &d, Lane &L, size_t n, prores_dev_encode_job *jobs, prores_error *errs);
+// The Config and Frame checks of Encoder::encode, without device work.
+Err encode_check(const prores_config &cfg, const prores_frame &frame);
 
 }  // namespace prores_hip

@@ -111,6 +111,23 @@ class _Tuning(C.Structure):
                 ("entropy_wg", C.c_uint32), ("graph", C.c_uint32)]
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_batch_frames", C.c_uint32), ("decode_bit_depth", C.c_uint32)]
+
+
+class _ServerResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("kind", C.c_uint32), ("error", _Err), ("dev_out", C.c_void_p),
+                ("out_bytes", C.c_size_t), ("dev_alpha", C.c_void_p), ("alpha_bytes", C.c_size_t),
+                ("header", _Header), ("submit_us", C.c_double), ("start_us", C.c_double),
+                ("done_us", C.c_double), ("batch_frames", C.c_uint32)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "failed", "decode_batches", "encode_batches",
+                                           "max_batch_frames", "held_bytes", "peak_held_bytes", "hbm_budget",
+                                           "pending")]
+
+
 class _DevDecodeJob(C.Structure):
     _fields_ = [("host_packet", C.c_void_p), ("len", C.c_size_t), ("dev_packet", C.c_void_p),
                 ("dev_data", C.c_void_p), ("dev_alpha", C.c_void_p)]
@@ -131,6 +148,9 @@ EXPORTS = [
     "prores_last_encode_timing", "prores_version", "prores_device_alloc", "prores_device_free",
     "prores_memcpy", "prores_decode_tuning_default", "prores_decode_device_batch_tuned",
     "prores_set_decode_tuning", "prores_get_decode_tuning", "prores_autotune_decode",
+    "prores_server_config_default", "prores_server_create", "prores_server_destroy",
+    "prores_server_submit_decode", "prores_server_submit_encode", "prores_server_poll",
+    "prores_server_release", "prores_server_get_stats",
 ]
 
 _lib = None
@@ -166,6 +186,17 @@ def lib():
         L.prores_decode_tuning_default.argtypes = [P(_Tuning)]
         L.prores_set_decode_tuning.argtypes = [P(_Tuning), P(_Err)]
         L.prores_get_decode_tuning.argtypes = [P(_Tuning)]
+        L.prores_server_config_default.argtypes = [P(_ServerConfig)]
+        L.prores_server_create.argtypes = [P(_ServerConfig), P(C.c_void_p), P(_Err)]
+        L.prores_server_destroy.argtypes = [C.c_void_p]
+        L.prores_server_submit_decode.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]
+        L.prores_server_submit_decode.restype = C.c_uint64
+        L.prores_server_submit_encode.argtypes = [C.c_void_p, P(_Config), P(_Frame)]
+        L.prores_server_submit_encode.restype = C.c_uint64
+        L.prores_server_poll.argtypes = [C.c_void_p, P(_ServerResult), C.c_size_t, C.c_int32]
+        L.prores_server_poll.restype = C.c_size_t
+        L.prores_server_release.argtypes = [C.c_void_p, C.c_uint64]
+        L.prores_server_get_stats.argtypes = [C.c_void_p, P(_ServerStats)]
         L.prores_autotune_decode.argtypes = [C.c_void_p, C.c_size_t, P(_DevDecodeJob), C.c_uint32, P(_Tuning),
                                              P(C.c_double), P(_Err)]
         L.prores_encode_device_batch.argtypes = [C.c_size_t, P(_DevEncodeJob), P(_Err)]
@@ -602,6 +633,90 @@ def encode_device_batch(items):
         else:
             out.append((jobs[i].dev_out, jobs[i].out_len))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Serving loop (prores_server_*).
+
+
+class ServerResult:
+    """A completed request. Decode results hold device planes in Frame layout
+    (and alpha); encode results the device packet. They stay on the device
+    until Server.release(id)."""
+
+    def __init__(self, r):
+        self.id = r.id
+        self.kind = "decode" if r.kind == 1 else "encode"
+        self.error = None if r.error.kind == 0 else _KINDS.get(r.error.kind, Error)(r.error.msg.decode())
+        self.dev_out, self.out_bytes = r.dev_out, r.out_bytes
+        self.dev_alpha, self.alpha_bytes = r.dev_alpha, r.alpha_bytes
+        self.width, self.height, self.chroma = r.header.width, r.header.height, r.header.chroma
+        self.submit_us, self.start_us, self.done_us = r.submit_us, r.start_us, r.done_us
+        self.batch_frames = r.batch_frames
+
+    @property
+    def latency_us(self):
+        return self.done_us - self.submit_us
+
+    def samples(self):
+        return download_ptr(self.dev_out, self.out_bytes).view(np.uint16)
+
+    def alpha(self):
+        return None if not self.dev_alpha else download_ptr(self.dev_alpha, self.alpha_bytes).view(np.uint16)
+
+    def packet(self):
+        return download_ptr(self.dev_out, self.out_bytes).tobytes()
+
+
+class Server:
+    """The serving loop: submit decode / encode requests from any thread,
+    poll results, release them (prores_server_* in include/prores_hip.h)."""
+
+    def __init__(self, hbm_budget=64 << 30, max_batch_frames=64, decode_bit_depth=0):
+        c = _ServerConfig(hbm_budget, max_batch_frames, decode_bit_depth)
+        h, err = C.c_void_p(), _Err()
+        lib().prores_server_create(C.byref(c), C.byref(h), C.byref(err))
+        _raise(err)
+        self._h = h
+
+    def submit_decode(self, packet):
+        packet = bytes(packet)
+        return lib().prores_server_submit_decode(self._h, packet, len(packet))
+
+    def submit_encode(self, config, frame):
+        c, f = config._c(), frame._c()
+        return lib().prores_server_submit_encode(self._h, C.byref(c), C.byref(f))
+
+    def poll(self, max_results=256, wait_ms=0):
+        arr = (_ServerResult * max_results)()
+        n = lib().prores_server_poll(self._h, arr, max_results, wait_ms)
+        return [ServerResult(arr[i]) for i in range(n)]
+
+    def release(self, request_id):
+        if lib().prores_server_release(self._h, request_id):
+            raise ConfigError(f"request {request_id} is unknown or not polled")
+
+    def stats(self):
+        st = _ServerStats()
+        lib().prores_server_get_stats(self._h, C.byref(st))
+        return {n: getattr(st, n) for n, _ in _ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().prores_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def version():

@@ -147,3 +147,12 @@ def test_metadata_frame_rates():  # src/frame.rs:370-382
         assert P.Metadata.frame_rate_code_for(n, d) == c
     assert P.Metadata.frame_rate_code_for(48, 2) == 2
     assert P.Metadata.frame_rate_code_for(48, 1) == 0
+
+
+def test_server_config_is_checked_before_the_device():
+    """A serving loop with a budget under 1 MiB or no batch room is refused
+    with a Config error before any device work."""
+    with pytest.raises(P.ConfigError):
+        P.Server(hbm_budget=1000)
+    with pytest.raises(P.ConfigError):
+        P.Server(max_batch_frames=0)
