@@ -279,6 +279,129 @@ def leg_batch_encode(P, synth, steps, warmup, check):
     return out
 
 
+SERVE_SIZES = [(720, 486, 1), (1280, 720, 0), (1920, 1080, 0), (2048, 1080, 0), (3840, 2160, 0), (4096, 2160, 0)]
+
+
+def leg_serving(P, synth, seconds, rate, check):
+    """The serving loop under Poisson arrivals at `rate` requests/s for
+    `seconds`: 70% decode, 30% encode; sizes from SERVE_SIZES, profiles from
+    all six (alpha on half of the 4444 requests), 1-8 macroblock slices,
+    seeded sinusoid+noise content. Requests draw from a pool of 24 decode
+    packets and 12 encode frames made up front. A poller thread records each
+    result's latency (submit to done, on the server's clock) and releases it;
+    a 5% sample is checked against the oracle."""
+    import threading
+    rng = np.random.default_rng(23)
+
+    def spec():
+        w, h, il = SERVE_SIZES[int(rng.integers(0, len(SERVE_SIZES)))]
+        prof = P.Profile(int(rng.integers(0, 6)))
+        ch = P.ChromaFormat.Yuv444 if prof >= P.Profile.P4444 else P.ChromaFormat.Yuv422
+        bd = 12 if ch == P.ChromaFormat.Yuv444 else 10
+        alpha = ch == P.ChromaFormat.Yuv444 and rng.random() < 0.5
+        atype = P.AlphaType(int(rng.integers(1, 3))) if alpha else P.AlphaType.Bits16
+        log2 = int(rng.integers(0, 4))
+        f = synth.frame(w, h, ch, bd, seed=int(rng.integers(1 << 30)), sigma=float(rng.uniform(2, 40)),
+                        alpha=alpha, interlace=P.Interlace(il))
+        return f, P.Config(prof, log2_slice_mbs=log2, alpha=atype)
+
+    log("bench: serving pool")
+    dec_pool = [spec() for _ in range(24)]
+    enc_pool = [spec() for _ in range(12)]
+    packets = [P.Encoder(cfg).encode(f) for f, cfg in dec_pool]
+    n_req = max(1, int(rate * seconds))
+    kinds = rng.random(n_req) < 0.7
+    picks = [int(rng.integers(0, 24 if k else 12)) for k in kinds]
+    gaps = rng.exponential(1.0 / rate, n_req)
+    sample = set(int(i) for i in np.flatnonzero(rng.random(n_req) < 0.05))
+
+    srv = P.Server()
+    results, sampled, lock = {}, {}, threading.Lock()
+    id_to_req = {}
+    stop = threading.Event()
+
+    def poller():
+        while not stop.is_set() or len(results) < n_req:
+            for r in srv.poll(256, 50):
+                with lock:
+                    k = id_to_req.get(r.id)
+                while k is None:  # submitted but not yet recorded by the submitter
+                    time.sleep(0)
+                    with lock:
+                        k = id_to_req.get(r.id)
+                if k in sample and r.error is None:
+                    sampled[k] = r.samples().copy() if r.kind == "decode" else r.packet()
+                    if r.kind == "decode" and r.dev_alpha:
+                        sampled[k] = (sampled[k], r.alpha().copy())
+                results[k] = (r.kind, r.latency_us, r.batch_frames, r.error)
+                srv.release(r.id)
+            if stop.is_set() and len(results) >= n_req:
+                break
+
+    th = threading.Thread(target=poller, daemon=True)
+    th.start()
+    log(f"bench: serving {n_req} requests at {rate}/s")
+    t0 = time.perf_counter()
+    t_next = t0
+    for i in range(n_req):
+        t_next += gaps[i]
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        if kinds[i]:
+            rid = srv.submit_decode(packets[picks[i]])
+        else:
+            f, cfg = enc_pool[picks[i]]
+            rid = srv.submit_encode(cfg, f)
+        with lock:
+            id_to_req[rid] = i
+    stop.set()
+    th.join(timeout=120)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([v[1] for v in results.values()])
+    lat_d = np.array([v[1] for v in results.values() if v[0] == "decode"])
+    lat_e = np.array([v[1] for v in results.values() if v[0] == "encode"])
+    errors = sum(1 for v in results.values() if v[3] is not None)
+    pct = lambda a, q: round(float(np.percentile(a, q)) / 1e3, 3) if a.size else None
+    out = {"workload": "serving: Poisson arrivals, 70% decode / 30% encode, sizes 720x486i-4096x2160, "
+                       "all six profiles (alpha on half of 4444), 1-8 MB slices",
+           "duration_s": round(wall, 2), "offered_rps": rate, "requests": n_req, "completed": len(results),
+           "errors": errors, "frames_per_s": round(len(results) / wall, 2),
+           "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": round(float(lat.max()) / 1e3, 3) if lat.size else None,
+                          "decode_p50": pct(lat_d, 50), "decode_p99": pct(lat_d, 99),
+                          "encode_p50": pct(lat_e, 50), "encode_p99": pct(lat_e, 99)},
+           "mean_batch_frames": round(float(np.mean([v[2] for v in results.values()])), 2) if results else None,
+           "decode_batches": st["decode_batches"], "encode_batches": st["encode_batches"],
+           "peak_held_bytes": st["peak_held_bytes"], "hbm_budget": st["hbm_budget"]}
+    if check:
+        import oracle_lib as O
+        ok, n, want = True, 0, {}
+        for k, got in sampled.items():
+            key = (bool(kinds[k]), picks[k])
+            if key not in want:  # the oracle runs once per pool item
+                if kinds[k]:
+                    want[key] = O.decode(packets[picks[k]])
+                else:
+                    f, cfg = enc_pool[picks[k]]
+                    oc = O.config(int(cfg.profile), log2_slice_mbs=cfg.log2_slice_mbs, alpha=int(cfg.alpha))
+                    of = O.Frame(f.width, f.height, int(f.chroma), f.bit_depth, f.data, f.alpha, int(f.interlace))
+                    want[key] = O.encode(of, oc)
+            w = want[key]
+            if kinds[k]:
+                if isinstance(got, tuple):
+                    ok &= bool(np.array_equal(got[0][:w.data.size], w.data) and
+                               np.array_equal(got[1][:w.alpha.size], w.alpha))
+                else:
+                    ok &= bool(np.array_equal(got[:w.data.size], w.data))
+            else:
+                ok &= w == got
+            n += 1
+        out["bit_exact_sample"] = {"requests": n, "equal_oracle": bool(ok)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -289,9 +412,12 @@ def main():
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="mixed,encode",
+    ap.add_argument("--legs", default="mixed,encode,serve",
                     help="extra legs after the headline decode: mixed (96 mixed frames, batched vs plain "
-                         "decode), encode (32-frame batched encode), comma-separated, or none")
+                         "decode), encode (32-frame batched encode), serve (the serving loop), "
+                         "comma-separated, or none")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=300.0, help="offered requests per second")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="after the timed steps, one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, for scaling profiler byte counters)")
@@ -403,6 +529,10 @@ def main():
     if "encode" in legs:
         log("bench: batch encode leg")
         result["batch_encode"] = leg_batch_encode(P, synth, max(1, args.steps // 2), 1, not args.no_cpu)
+
+    if "serve" in legs:
+        log("bench: serving leg")
+        result["serving"] = leg_serving(P, synth, args.serve_seconds, args.serve_rate, not args.no_cpu)
 
     if not args.no_cpu:
         import oracle_lib as O
