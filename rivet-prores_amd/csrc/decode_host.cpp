@@ -60,7 +60,7 @@ struct Batch {
 // arguments are workspace addresses and launch shapes, so it replays while
 // those stay the same.
 struct GraphKey {
-  void *ptrs[13];
+  void *ptrs[14];
   unsigned grids[4];
   prores_decode_tuning t;
   bool alpha;
@@ -68,7 +68,8 @@ struct GraphKey {
 };
 
 struct Workspace {
-  DevBuf frames, pics, rows, slices, status, first_err, slice_err, coefs, wide, packet, planes, hdr, items, segs;
+  DevBuf frames, pics, rows, slices, status, first_err, slice_err, coefs, wide, packet, planes, hdr, items, segs,
+      alpha_order;
   hipGraphExec_t graph = nullptr;
   GraphKey key;
 };
@@ -259,10 +260,12 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   if (!(e = ws.first_err.reserve(np * 4 + 64)).ok()) return e;
   if (!(e = ws.coefs.reserve(b.n_coefs * 2 + 64)).ok()) return e;
   if (!(e = ws.wide.reserve(b.n_coefs * 4 + 64)).ok()) return e;
+  if (!(e = ws.alpha_order.reserve(G * 4 + 64)).ok()) return e;
 
   void *B = ws.hdr.p, *pics = ws.pics.p, *rows = ws.rows.p, *slices = ws.slices.p, *status = ws.status.p;
   void *frames = ws.frames.p, *tables = d.tables, *coefs = ws.coefs.p, *wide = ws.wide.p;
   void *serr = ws.slice_err.p, *ferr = ws.first_err.p, *itemp = ws.items.p, *segp = ws.segs.p;
+  void *aord = ws.alpha_order.p;
   uint32_t pers = t.persistent;
   const unsigned ewaves = t.entropy_wg / 64;
   unsigned grids[4];
@@ -281,7 +284,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     Err x;
     if (timed) (void)hipEventRecord(L.ev[0], L.stream);
     {
-      void *args[] = {&B, &pics, &rows, &slices, &status, &serr, &ferr, &segp, &itemp};
+      void *args[] = {&B, &frames, &pics, &rows, &slices, &status, &serr, &ferr, &segp, &itemp, &aord};
       if (!(x = launch(L, d.k_slice_table, grids[0], 1, 256, args)).ok()) return x;
     }
     if (timed) (void)hipEventRecord(L.ev[1], L.stream);
@@ -291,7 +294,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     }
     if (timed) (void)hipEventRecord(L.ev[2], L.stream);
     if (b.any_alpha) {
-      void *args[] = {&B, &frames, &pics, &slices, &status, &serr, &ferr, &pers};
+      void *args[] = {&B, &frames, &pics, &slices, &status, &serr, &ferr, &aord, &pers};
       if (!(x = launch(L, d.k_alpha, grids[2], 1, 64, args)).ok()) return x;
     }
     if (timed) (void)hipEventRecord(L.ev[3], L.stream);
@@ -306,7 +309,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   } else if (t.graph) {
     GraphKey key;
     memset(&key, 0, sizeof key);
-    void *ptrs[13] = {B, pics, rows, slices, status, frames, tables, coefs, wide, serr, ferr, itemp, segp};
+    void *ptrs[14] = {B, pics, rows, slices, status, frames, tables, coefs, wide, serr, ferr, itemp, segp, aord};
     memcpy(key.ptrs, ptrs, sizeof ptrs);
     memcpy(key.grids, grids, sizeof grids);
     key.t = t;

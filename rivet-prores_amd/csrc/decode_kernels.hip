@@ -388,22 +388,30 @@ __device__ __forceinline__ void entropy_stream(
 // One slice's alpha (see pd_alpha).
 constexpr uint32_t ALPHA_WORDS = (2048 * 33 / 8) / 8 + 6;
 
+// A value the whole wave holds, as a scalar.
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __device__ __forceinline__ void alpha_slice(
     uint32_t g, uint64_t *__restrict__ words, uint16_t *__restrict__ vals,
     const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
     const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
     uint32_t *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err) {
   const uint32_t L = threadIdx.x;
+  g = uni(g);
   const SliceRec S = slices[g];
-  if (pic_status[S.pic] != PIC_OK) return;
+  if (uni(pic_status[S.pic]) != PIC_OK) return;
   const DecPic &P = pics[S.pic];
   const DecFrame &F = frames[P.frame];
-  const uint32_t bits = F.alpha_bits;
+  const uint32_t bits = uni(F.alpha_bits);
   if (bits == 0) return;
   const uint8_t *s = (const uint8_t *)S.off;
-  const uint32_t slen = S.len;
+  const uint32_t slen = uni(S.len);
   uint32_t hs, ysz, cbsz, crsz;
-  uint32_t err = slice_header(s, slen, bits, hs, ysz, cbsz, crsz);
+  uint32_t err = uni(slice_header(s, slen, bits, hs, ysz, cbsz, crsz));
+  hs = uni(hs);
+  ysz = uni(ysz);
+  cbsz = uni(cbsz);
+  crsz = uni(crsz);
   if (err) {
     if (L == 0) {
       slice_err[4 * (size_t)g + 3] = err;
@@ -412,14 +420,16 @@ __device__ __forceinline__ void alpha_slice(
     return;
   }
   // Stage the stream: aligned words from base - m, bytes outside the stream
-  // zeroed.
+  // zeroed, and two zero words after it (the reader holds the word after its
+  // current one, and never passes the end of the stream).
   const uint32_t used = hs + ysz + cbsz + crsz, len = slen - used;
   const uintptr_t a = (uintptr_t)(s + used);
-  const uint32_t m = (uint32_t)(a & 7);
+  const uint32_t m = uni((uint32_t)(a & 7));
   const uint64_t *src = (const uint64_t *)(a - m);
   const uint32_t end_byte = m + len;  // bytes of stream data end here
   const uint32_t have = (end_byte + 7) / 8;
-  for (uint32_t j = L; j < ALPHA_WORDS; j += 64) {
+  const uint32_t stage = have + 2 < ALPHA_WORDS ? have + 2 : ALPHA_WORDS;
+  for (uint32_t j = L; j < stage; j += 64) {
     uint64_t w = 0;
     if (j < have) {
       w = __builtin_bswap64(src[j]);
@@ -431,18 +441,27 @@ __device__ __forceinline__ void alpha_slice(
   }
   __syncthreads();
 
+  // The parse is the same in every lane: its state lives in scalar
+  // registers (each staged word is read as a wave-uniform value), and the
+  // vector lanes only fill runs.
   const uint32_t mag = bits == 8 ? 3u : 6u, mask = (1u << bits) - 1;
-  const uint32_t log2c = 4 + (31 - __builtin_clz(S.mbs));
+  const uint32_t log2c = uni(4 + (31 - __builtin_clz(S.mbs)));
   const uint32_t cols = 1u << log2c, count = cols * 16;
   const uint32_t rows =
-      (uint32_t)S.mb_y + 1 < P.height_in_mb ? 16u : P.height - 16u * (P.height_in_mb - 1);
+      uni((uint32_t)S.mb_y + 1 < P.height_in_mb ? 16u : P.height - 16u * (P.height_in_mb - 1));
   const uint32_t needed = cols * rows;
-  const uint32_t ident = F.rescale_identity;
-  const uint64_t rm = F.rescale_m;
+  const uint32_t ident = uni(F.rescale_identity);
+  const uint64_t rm = ((uint64_t)uni((uint32_t)(F.rescale_m >> 32)) << 32) | uni((uint32_t)F.rescale_m);
+  auto word = [&](uint32_t j) -> uint64_t {
+    const uint64_t w = words[j < ALPHA_WORDS ? j : ALPHA_WORDS - 1];
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)w);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(w >> 32));
+    return ((uint64_t)hi << 32) | lo;
+  };
   // The reader: bit `pos` of the stream is bit `off` of word `wj`.
   uint32_t wj = 0, off = 8 * m, pos = 0;
   const uint32_t lenbits = 8 * len;
-  uint64_t A = words[0], B = words[1];
+  uint64_t A = word(0), B = word(1);
   auto window = [&]() -> uint64_t { return off ? (A << off) | (B >> (64 - off)) : A; };
   auto consume = [&](uint32_t n) {  // n <= 64
     pos += n;
@@ -451,7 +470,7 @@ __device__ __forceinline__ void alpha_slice(
       off -= 64;
       wj++;
       A = B;
-      B = words[wj + 1 < ALPHA_WORDS ? wj + 1 : ALPHA_WORDS - 1];
+      B = word(wj + 1);
     }
   };
   auto take = [&](uint32_t n) -> uint32_t {  // n <= 32
@@ -523,7 +542,18 @@ __device__ __forceinline__ void alpha_slice(
     prev = (int32_t)value;
     if (n + run > count) { err = E_ALPHA_RUN; break; }
     const uint16_t out = rescale(value, ident, rm);
-    for (uint32_t i = n + L; i < n + run; i += 64) vals[i] = out;
+    if (run <= 64) {
+      if (L < run) vals[n + L] = out;
+    } else {
+      // A long run: to a 4-value boundary, then 8 bytes per lane.
+      const uint32_t head = (4 - (n & 3)) & 3;
+      if (L < head) vals[n + L] = out;
+      const uint32_t a4 = n + head, quads = (run - head) / 4;
+      const uint64_t v4 = (uint64_t)out * 0x0001000100010001ull;
+      for (uint32_t k = L; k < quads; k += 64) ((uint64_t *)vals)[a4 / 4 + k] = v4;
+      const uint32_t t0 = a4 + 4 * quads;
+      if (L < n + run - t0) vals[t0 + L] = out;
+    }
     n += run;
   }
   if (err) {
@@ -566,19 +596,21 @@ __device__ __forceinline__ void alpha_slice(
 // first-error slots, so the launch sequence needs no memset, and writes the
 // picture's IDCT items.
 extern "C" __global__ void __launch_bounds__(256)
-    pd_slice_table(DecBatch *__restrict__ B, const DecPic *__restrict__ pics,
-                   const RowSlot *__restrict__ rows, SliceRec *__restrict__ slices,
-                   uint32_t *__restrict__ pic_status, uint4 *__restrict__ slice_err,
-                   uint32_t *__restrict__ pic_first_err, const IdctItem *__restrict__ segs,
-                   IdctItem *__restrict__ items) {
+    pd_slice_table(DecBatch *__restrict__ B, const DecFrame *__restrict__ frames,
+                   const DecPic *__restrict__ pics, const RowSlot *__restrict__ rows,
+                   SliceRec *__restrict__ slices, uint32_t *__restrict__ pic_status,
+                   uint4 *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err,
+                   const IdctItem *__restrict__ segs, IdctItem *__restrict__ items,
+                   uint32_t *__restrict__ alpha_order) {
   const uint32_t tid = threadIdx.x;
   if (blockIdx.x == 0 && tid < 4) B->q[tid] = 0;
-  const uint32_t np = B->n_pics;
+  const uint32_t np = B->n_pics, G = B->n_slices;
   __shared__ uint32_t wsum[4];
   __shared__ uint64_t carry;
   const uint32_t lane = tid & 63, wave = tid >> 6;
   for (uint32_t p = blockIdx.x; p < np; p += gridDim.x) {
     const DecPic P = pics[p];
+    const bool has_alpha = frames[P.frame].alpha_bits != 0;
     const uint8_t *tab = (const uint8_t *)(P.pic_off + P.header_size);
     const uint64_t data0 = P.header_size + 2ull * P.n_slices;  // relative to the picture
     if (tid == 0) {
@@ -612,6 +644,32 @@ extern "C" __global__ void __launch_bounds__(256)
         sr.mbs = rs.mbs;
         slices[P.slice_base + k] = sr;
         slice_err[P.slice_base + k] = make_uint4(0, 0, 0, 0);
+      }
+      // The alpha work order: slices with a long alpha stream at the front,
+      // the rest from the back (a slice outside the picture reads nothing).
+      bool heavy = false;
+      if (k < P.n_slices && has_alpha && size >= 8 && data0 + excl + size <= P.picture_size) {
+        const uint8_t *sl = (const uint8_t *)(P.pic_off + data0 + excl);
+        const uint32_t hs = sl[0] >> 3;
+        const uint32_t comp = ((uint32_t)sl[2] << 8 | sl[3]) + ((uint32_t)sl[4] << 8 | sl[5]) +
+                              ((uint32_t)sl[6] << 8 | sl[7]);
+        heavy = hs <= size && comp <= size - hs && size - hs - comp > ALPHA_HEAVY_BYTES;
+      }
+      const uint64_t hmask = __ballot(k < P.n_slices && heavy);
+      const uint64_t lmask = __ballot(k < P.n_slices && !heavy);
+      const uint64_t below = (1ull << lane) - 1;
+      uint32_t hb = 0, lb = 0;
+      if (lane == 0) {
+        if (hmask) hb = atomicAdd(&B->heavy, (uint32_t)__popcll(hmask));
+        if (lmask) lb = atomicAdd(&B->light, (uint32_t)__popcll(lmask));
+      }
+      hb = __shfl(hb, 0, 64);
+      lb = __shfl(lb, 0, 64);
+      if (k < P.n_slices) {
+        if (heavy)
+          alpha_order[hb + __popcll(hmask & below)] = P.slice_base + k;
+        else
+          alpha_order[G - 1 - (lb + __popcll(lmask & below))] = P.slice_base + k;
       }
       __syncthreads();
       if (tid == 255) carry += before + incl;
@@ -664,8 +722,9 @@ extern "C" __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// One wavefront per slice with alpha: scanned_alpha() (src/decode.rs:570-595)
-// and its placement (src/decode.rs:365-389). The wave stages the slice's
+// One wavefront per slice with alpha, in the work order pd_slice_table made
+// (long streams first): scanned_alpha() (src/decode.rs:570-595) and its
+// placement (src/decode.rs:365-389). The wave stages the slice's
 // alpha stream in LDS (big-endian 64-bit words, zero past the end), every
 // lane parses the same codes from it (a broadcast read; the parse is
 // wave-uniform), runs are filled into an LDS copy of the slice's values by
@@ -677,22 +736,23 @@ extern "C" __global__ void __launch_bounds__(64)
     pd_alpha(DecBatch *__restrict__ B, const DecFrame *__restrict__ frames,
              const DecPic *__restrict__ pics, const SliceRec *__restrict__ slices,
              const uint32_t *__restrict__ pic_status, uint32_t *__restrict__ slice_err,
-             uint32_t *__restrict__ pic_first_err, uint32_t persistent) {
+             uint32_t *__restrict__ pic_first_err, const uint32_t *__restrict__ order,
+             uint32_t persistent) {
   __shared__ uint64_t words[ALPHA_WORDS];
   __shared__ __attribute__((aligned(16))) uint16_t vals[2048];
   const uint32_t n = B->n_slices;
   if (!persistent) {
     if (blockIdx.x < n)
-      alpha_slice(blockIdx.x, words, vals, frames, pics, slices, pic_status, slice_err, pic_first_err);
+      alpha_slice(order[blockIdx.x], words, vals, frames, pics, slices, pic_status, slice_err, pic_first_err);
     return;
   }
   const uint32_t chunk = B->chunk;
   for (;;) {
-    const uint32_t first = wave_pop(&B->q[1], chunk);
+    const uint32_t first = uni(wave_pop(&B->q[1], chunk));
     if (first >= n) break;
     const uint32_t last = first + chunk < n ? first + chunk : n;
-    for (uint32_t g = first; g < last; g++) {
-      alpha_slice(g, words, vals, frames, pics, slices, pic_status, slice_err, pic_first_err);
+    for (uint32_t i = first; i < last; i++) {
+      alpha_slice(order[i], words, vals, frames, pics, slices, pic_status, slice_err, pic_first_err);
       __syncthreads();
     }
   }

@@ -85,13 +85,15 @@ struct SliceRec {
 
 // Counts of a batch and the heads of its work queues, in device memory so
 // that the launch arguments stay the same from one batch to the next (a
-// captured graph replays unchanged). pd_slice_table zeroes the queue heads.
+// captured graph replays unchanged). The host uploads it before every
+// launch sequence, so every counter starts at zero.
 struct DecBatch {
   uint32_t n_pics, n_slices;
   uint32_t n_entropy_items;  // 64 streams of one component each: 3 * ceil(n_slices / 64)
   uint32_t n_idct_items;
   uint32_t chunk;            // items taken per queue pop by a persistent workgroup
-  uint32_t pad[3];
+  uint32_t heavy, light;     // alpha work order fill counts (zero at upload)
+  uint32_t pad;
   uint32_t q[4];             // queue heads: entropy, alpha, idct
 };
 
@@ -104,6 +106,10 @@ struct IdctItem {
   uint32_t first, count, blocks, pad;
 };
 constexpr uint32_t IDCT_MAX_BLOCKS = 96;  // blocks one workgroup holds in LDS
+
+// Slices whose alpha stream is longer than this many bytes go first in the
+// alpha work order: their decode is one long serial chain.
+constexpr uint32_t ALPHA_HEAVY_BYTES = 256;
 
 // Picture status words written by the kernels.
 enum PicStatus : uint32_t { PIC_OK = 0, PIC_TABLE_ERR = 1 };
