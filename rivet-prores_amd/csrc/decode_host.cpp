@@ -187,9 +187,9 @@ Err upload(Lane &L, DevBuf &buf, const std::vector<T> &v) {
 }
 
 // Workgroups of kernel `f` that fit on the device at once (persistent grids).
-unsigned resident_grid(Device &d, hipFunction_t f, unsigned block) {
+unsigned resident_grid(Device &d, hipFunction_t f, unsigned block, size_t lds = 0) {
   int per_cu = 0, cus = 0;
-  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, (int)block, 0) != hipSuccess || per_cu < 1)
+  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, (int)block, lds) != hipSuccess || per_cu < 1)
     per_cu = 1;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d.id) != hipSuccess || cus < 1) cus = 1;
   return (unsigned)(per_cu * cus);
@@ -271,7 +271,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   unsigned grids[4];
   if (t.persistent) {
     grids[0] = resident_grid(d, d.k_slice_table, 256);
-    grids[1] = resident_grid(d, d.k_entropy, t.entropy_wg);
+    grids[1] = resident_grid(d, d.k_entropy, t.entropy_wg, t.entropy_wg * LINE_WORDS * 4);
     grids[2] = resident_grid(d, d.k_alpha, 64);
     grids[3] = resident_grid(d, d.k_idct, 256);
   } else {
@@ -290,7 +290,8 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     if (timed) (void)hipEventRecord(L.ev[1], L.stream);
     {
       void *args[] = {&B, &frames, &pics, &slices, &status, &coefs, &wide, &serr, &ferr, &pers};
-      if (!(x = launch(L, d.k_entropy, grids[1], 1, t.entropy_wg, args)).ok()) return x;
+      if (!(x = launch(L, d.k_entropy, grids[1], 1, t.entropy_wg, args, t.entropy_wg * LINE_WORDS * 4)).ok())
+        return x;
     }
     if (timed) (void)hipEventRecord(L.ev[2], L.stream);
     if (b.any_alpha) {

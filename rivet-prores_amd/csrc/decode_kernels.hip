@@ -157,40 +157,51 @@ __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t
 }
 
 // Writes one component's scanned coefficients as i16, in increasing index
-// order, 16 bytes at a time; the zeros between are written too, so the arena
-// needs no clearing. A value outside +-32767 is stored as -32768 with the
-// full value beside it in the i32 side arena.
+// order, through a 64-coefficient line of LDS per lane: a coefficient is one
+// LDS store, and each finished line goes to the arena as one whole 128-byte
+// line (eight 16-byte stores); the zeros between are written too, so the
+// arena needs no clearing. A value outside +-32767 is stored as -32768 with
+// the full value beside it in the i32 side arena.
 struct CoefOut {
   int16_t *base;
   int32_t *wide;
-  uint32_t cur, w0, w1, w2, w3;
-  __device__ __forceinline__ void store(uint32_t grp, uint32_t a, uint32_t b, uint32_t c,
-                                        uint32_t d) {
-    *(uint4 *)(base + 8 * grp) = make_uint4(a, b, c, d);
+  uint32_t *line;  // this lane's LINE_WORDS dwords of LDS
+  uint32_t cur;
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (uint32_t i = 0; i < 16; i++) ((uint2 *)line)[i] = make_uint2(0, 0);
+  }
+  __device__ __forceinline__ void flush(uint32_t grp) {
+    uint2 v[16];
+#pragma unroll
+    for (uint32_t i = 0; i < 16; i++) v[i] = ((const uint2 *)line)[i];
+    uint4 *o = (uint4 *)(base + 64 * grp);
+#pragma unroll
+    for (uint32_t i = 0; i < 8; i++) o[i] = make_uint4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
+    clear();
+  }
+  __device__ __forceinline__ void zero_line(uint32_t grp) {
+    uint4 *o = (uint4 *)(base + 64 * grp);
+#pragma unroll
+    for (uint32_t i = 0; i < 8; i++) o[i] = make_uint4(0, 0, 0, 0);
   }
   __device__ __forceinline__ void put(uint32_t n, int32_t v) {
-    const uint32_t grp = n >> 3;
+    const uint32_t grp = n >> 6;
     if (grp != cur) {
-      store(cur, w0, w1, w2, w3);
-      for (uint32_t g = cur + 1; g < grp; g++) store(g, 0, 0, 0, 0);
+      flush(cur);
+      for (uint32_t g = cur + 1; g < grp; g++) zero_line(g);
       cur = grp;
-      w0 = w1 = w2 = w3 = 0;
     }
     int32_t h = v;
     if (v > 32767 || v < -32767) {
       h = -32768;
       wide[n] = v;
     }
-    const uint32_t word = (uint32_t)(uint16_t)(int16_t)h << (16 * (n & 1));
-    const uint32_t wi = (n >> 1) & 3;
-    w0 |= wi == 0 ? word : 0u;
-    w1 |= wi == 1 ? word : 0u;
-    w2 |= wi == 2 ? word : 0u;
-    w3 |= wi == 3 ? word : 0u;
+    ((uint16_t *)line)[n & 63] = (uint16_t)(int16_t)h;
   }
   __device__ __forceinline__ void finish(uint32_t total) {
-    store(cur, w0, w1, w2, w3);
-    for (uint32_t g = cur + 1; g < total / 8; g++) store(g, 0, 0, 0, 0);
+    flush(cur);
+    for (uint32_t g = cur + 1; g < total / 64; g++) zero_line(g);
   }
 };
 
@@ -354,7 +365,7 @@ __device__ __forceinline__ uint32_t wave_pop(uint32_t *head, uint32_t chunk) {
 
 // One colour stream: component `comp` of slice `g`.
 __device__ __forceinline__ void entropy_stream(
-    uint32_t g, uint32_t comp, const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
+    uint32_t g, uint32_t comp, uint32_t *line, const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
     const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
     int16_t *__restrict__ coefs, int32_t *__restrict__ coef_wide, uint32_t *__restrict__ slice_err,
     uint32_t *__restrict__ pic_first_err) {
@@ -376,7 +387,8 @@ __device__ __forceinline__ void entropy_stream(
     const uint32_t len = comp == 0 ? ysz : (comp == 1 ? cbsz : crsz);
     const uint64_t cofs = S.coef + (comp == 0 ? 0ull : (comp == 1 ? 256ull * S.mbs : (256ull + 64ull * cpm) * S.mbs));
     r.init(s + off, len);
-    CoefOut out{coefs + cofs, coef_wide + cofs, 0, 0, 0, 0, 0};
+    CoefOut out{coefs + cofs, coef_wide + cofs, line, 0};
+    out.clear();
     err = decode_component(r, nb, out);
   }
   if (err) {
@@ -690,20 +702,22 @@ extern "C" __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // One lane per colour stream; an item is 64 streams of one component (64
 // consecutive slices), one wave's work: item i is component i % 3 of slices
-// 64 (i / 3) ... 64 (i / 3) + 63.
+// 64 (i / 3) ... 64 (i / 3) + 63. Dynamic LDS: LINE_WORDS dwords per thread.
 extern "C" __global__ void __launch_bounds__(256)
     pd_entropy(DecBatch *__restrict__ B, const DecFrame *__restrict__ frames,
                const DecPic *__restrict__ pics, const SliceRec *__restrict__ slices,
                const uint32_t *__restrict__ pic_status, int16_t *__restrict__ coefs,
                int32_t *__restrict__ coef_wide, uint32_t *__restrict__ slice_err,
                uint32_t *__restrict__ pic_first_err, uint32_t persistent) {
+  extern __shared__ uint32_t lines[];  // LINE_WORDS per thread
+  uint32_t *line = lines + LINE_WORDS * threadIdx.x;
   const uint32_t n_items = B->n_entropy_items, n_slices = B->n_slices;
   const uint32_t lane = threadIdx.x & 63;
   if (!persistent) {
     const uint32_t item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t g = (item / 3) * 64 + lane;
     if (item < n_items && g < n_slices)
-      entropy_stream(g, item % 3, frames, pics, slices, pic_status, coefs, coef_wide, slice_err,
+      entropy_stream(g, item % 3, line, frames, pics, slices, pic_status, coefs, coef_wide, slice_err,
                      pic_first_err);
     return;
   }
@@ -715,7 +729,7 @@ extern "C" __global__ void __launch_bounds__(256)
     for (uint32_t item = first; item < last; item++) {
       const uint32_t g = (item / 3) * 64 + lane;
       if (g < n_slices)
-        entropy_stream(g, item % 3, frames, pics, slices, pic_status, coefs, coef_wide, slice_err,
+        entropy_stream(g, item % 3, line, frames, pics, slices, pic_status, coefs, coef_wide, slice_err,
                        pic_first_err);
     }
   }

@@ -238,9 +238,9 @@ Err DevBuf::reserve(size_t bytes) {
   return Err{};
 }
 
-Err launch(Lane &L, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args) {
+Err launch(Lane &L, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args, unsigned lds) {
   if (gx == 0 || gy == 0) return Err{};
-  return check(hipModuleLaunchKernel(f, gx, gy, 1, bx, 1, 1, 0, L.stream, args, nullptr),
+  return check(hipModuleLaunchKernel(f, gx, gy, 1, bx, 1, 1, lds, L.stream, args, nullptr),
                "hipModuleLaunchKernel");
 }
 

@@ -115,7 +115,8 @@ struct DevBuf {
   }
 };
 
-Err launch(Lane &L, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args);
+// `lds`: bytes of dynamic LDS per workgroup.
+Err launch(Lane &L, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args, unsigned lds = 0);
 Err check(hipError_t e, const char *where);
 
 void build_tables(DevTables &t);

@@ -106,6 +106,7 @@ struct IdctItem {
   uint32_t first, count, blocks, pad;
 };
 constexpr uint32_t IDCT_MAX_BLOCKS = 96;  // blocks one workgroup holds in LDS
+constexpr uint32_t LINE_WORDS = 34;  // entropy: LDS dwords per lane (a 64-coefficient line, padded)
 
 // Slices whose alpha stream is longer than this many bytes go first in the
 // alpha work order: their decode is one long serial chain.
