@@ -97,6 +97,21 @@ def alpha_stream_len(packet):
     return total
 
 
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload`, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", "decode_c2_hbm_traffic.md")
+    try:
+        text = open(path).read()
+        block = text[text.index("```json") + 7:]
+        d = json.loads(block[:block.index("```")])
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != workload:
+        return None
+    return d.get("traffic_bytes", {}).get(kernel)
+
+
 def cpu_baseline(O, packets, budget_s):
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     best = None
@@ -491,6 +506,7 @@ def main():
     achieved = kernel_bytes[dom] / (stage[dom] * 1e-6) / 1e9
     pipeline_gbs = (packet_bytes + colour_bytes + alpha_bytes) / (stage[4] * 1e-6) / 1e9
 
+    workload = f"decode {n} x {w}x{h} ProRes 4444 XQ, 12-bit 4:4:4 + 16-bit alpha, 8-MB slices"
     result = {
         "metric": METRIC,
         "value": round(n / batched_s, 3),
@@ -504,7 +520,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"decode {n} x {w}x{h} ProRes 4444 XQ, 12-bit 4:4:4 + 16-bit alpha, 8-MB slices",
+        "config": {"workload": workload,
                    "frames": n, "profile": "ap4x", "bit_depth": 12, "alpha_bits": 16, "log2_slice_mbs": 3},
         "batched_fps": round(n / batched_s, 3),
         "plain_fps": round(n / plain_s, 3),
@@ -512,7 +528,9 @@ def main():
         "stage_us": {"slice_table": round(stage[0], 2), "entropy": round(stage[1], 2), "alpha": round(stage[2], 2),
                      "idct": round(stage[3], 2), "kernels_total": round(stage[4], 2)},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(workload, names[dom]),
+                     "algorithmic_bytes": int(kernel_bytes[dom])},
         "pipeline_roofline": {"achieved": round(pipeline_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(pipeline_gbs / HBM_PEAK_GBS, 4),
                               "bytes_per_frame": round((packet_bytes + colour_bytes + alpha_bytes) / n)},
