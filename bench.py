@@ -540,6 +540,13 @@ def main():
                             "pack": round(enc_stage[2], 1), "all": round(enc_stage[3], 1)},
     }
 
+    # The host-buffer entry point (prores_decode): packet from host memory,
+    # planes back to host memory, one frame per call; PCIe transfers included.
+    t0 = time.perf_counter()
+    for i in range(4):
+        dec.decode(packets[i])
+    result["host_api_fps"] = round(4 / (time.perf_counter() - t0), 2)
+
     legs = [x for x in args.legs.split(",") if x and x != "none"]
     if "mixed" in legs:
         log("bench: mixed decode leg")
