@@ -1,0 +1,129 @@
+"""The kernels' closed forms and lane logic, restated on the CPU and checked
+against bit-serial readers and the oracle. The GPU tests check the kernels
+themselves end to end; these pin the two places where a kernel departs from
+the reference's step-by-step structure.
+
+- fast_code (csrc/decode_kernels.hip): a combination code (lastRiceQ, kRice,
+  kExp) read in closed form from the top 32 bits of the window, against a
+  reader that follows Codebook::read bit by bit (src/vlc.rs:206-230).
+- pe_alpha (csrc/encode_kernels.hip): 64 lanes each code the runs that start
+  in their 1/64 of a slice's values, a run ending at the next lane's first run
+  start (an inclusive suffix minimum shifted by one lane); the bits must equal
+  the oracle's encode_alpha (src/encode.rs:457-470).
+"""
+
+import ctypes as C
+import random
+
+import numpy as np
+
+import oracle_lib as O
+
+BOOKS = [(2, 0, 1), (1, 0, 1), (0, 0, 1), (1, 1, 2), (0, 1, 2), (0, 2, 3), (2, 0, 2), (0, 3, 4), (1, 2, 3),
+         (0, 5, 6)]
+
+
+def reference_read(bits, lq, kr, ke):
+    """(symbol, length), 'range' or None (prefix over 32 or out of bits)."""
+    q = 0
+    while q < len(bits) and bits[q] == "0":
+        q += 1
+    if q > 32 or q >= len(bits):
+        return None
+    if q <= lq:
+        n = 1 + kr
+        if q + n > len(bits):
+            return None
+        return ((q << kr) | (int(bits[q:q + n], 2) & ((1 << kr) - 1)), q + n)
+    n = q - (lq + 1) + ke + 1
+    if q + n > len(bits):
+        return None
+    val = int(bits[q:q + n], 2) - (1 << ke) + ((lq + 1) << kr)
+    return "range" if val > (1 << 30) else (val, q + n)
+
+
+def fast_code(top, lq, kr, ke):
+    """The kernel's closed form, in 32-bit arithmetic."""
+    q = 32 - top.bit_length() if top else 32
+    rice = q <= lq
+    L = q + 1 + kr if rice else 2 * q - lq + ke
+    p = (top >> ((32 - L) & 31)) & 0xFFFFFFFF
+    k = kr if rice else ke
+    qq = q if rice else lq + 1
+    sym = (p - (1 << k) + (qq << kr)) & 0xFFFFFFFF
+    return (L <= 32 and sym <= (1 << 30)), sym, L
+
+
+def test_fast_code_equals_the_bit_serial_reader():
+    rng = random.Random(1)
+    n = 0
+    for lq, kr, ke in BOOKS:
+        tops = list(range(0, 1 << 14)) + [rng.getrandbits(32) >> rng.randrange(32) for _ in range(60000)]
+        for t in tops:
+            ok, sym, L = fast_code(t, lq, kr, ke)
+            r = reference_read(format(t, "032b") + "0" * 40, lq, kr, ke)
+            if ok:
+                assert r not in (None, "range") and r == (sym, L), (lq, kr, ke, hex(t))
+                n += 1
+            else:
+                assert r is None or r == "range" or r[1] > 32, (lq, kr, ke, hex(t), r)
+    assert n > 100000
+
+
+def lane_alpha(vals, bits):
+    """pe_alpha's lane logic: bytes of the coded slice."""
+    count = len(vals)
+    seg = count // 64
+    mag, maxd = (3, 8) if bits == 8 else (6, 64)
+    first = [count] * 64
+    for L in range(64):
+        for i in range(L * seg, L * seg + seg):
+            if i == 0 or vals[i] != vals[i - 1]:
+                first[L] = i
+                break
+    incl = first[:]
+    d = 1
+    while d < 64:
+        incl = [min(incl[L], incl[L + d]) if L + d < 64 else incl[L] for L in range(64)]
+        d *= 2
+    nxt = [incl[L + 1] if L < 63 else count for L in range(64)]
+    codes = []
+    for L in range(64):
+        hi = L * seg + seg
+        i = first[L]
+        while i < hi:
+            j = i + 1
+            while j < hi and vals[j] == vals[i]:
+                j += 1
+            run = (j if j < hi else nxt[L]) - i
+            dd = int(vals[i]) - (int(vals[i - 1]) if i else -1)
+            if 1 <= abs(dd) <= maxd:
+                codes.append((((abs(dd) - 1) << 1) | (dd < 0), 2 + mag))
+            else:
+                codes.append(((1 << bits) | (dd & ((1 << bits) - 1)), 1 + bits))
+            codes.append((1, 1) if run == 1 else ((run - 1, 5) if run <= 16 else (run - 1, 16)))
+            i = j
+    s = "".join(format(v, f"0{k}b") for v, k in codes)
+    s += "0" * (-len(s) % 8)
+    return bytes(int(s[k:k + 8], 2) for k in range(0, len(s), 8))
+
+
+def test_alpha_lane_logic_equals_the_oracle():
+    rng = np.random.default_rng(5)
+    buf = C.create_string_buffer(8192)
+    for trial in range(160):
+        bits = 8 if trial % 3 == 0 else 16
+        count = 256 * int(rng.choice([1, 2, 4, 8]))
+        kind = trial % 4
+        if kind == 0:
+            v = rng.integers(0, 1 << bits, count)
+        elif kind == 1:
+            v = np.repeat(rng.integers(0, 1 << bits, count // 7 + 1), rng.integers(1, 40, count // 7 + 1))[:count]
+            v = np.concatenate([v, np.full(count - len(v), v[-1])]) if len(v) < count else v
+        elif kind == 2:
+            v = np.full(count, rng.integers(0, 1 << bits))
+        else:
+            v = np.clip(np.cumsum(rng.integers(-3, 4, count)) + 100, 0, (1 << bits) - 1)
+        v = np.ascontiguousarray(v, dtype=np.uint16)
+        n = O.lib().oracle_encode_alpha(v.ctypes.data, len(v), bits, buf, 8192)
+        assert lane_alpha(v, bits) == buf.raw[:n], trial
