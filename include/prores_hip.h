@@ -237,6 +237,12 @@ void prores_last_batch_timing(double *us5);
  * (host sync for the output sizes included). */
 void prores_last_encode_timing(double *us4);
 
+/* Counters of the last encode batch on the per-call lane: [0] sizes the
+ * share-out computed past its precomputed window, [1] the most for one
+ * picture, [2] pictures, [3] mean uniform quantiser index. No counterpart in
+ * the reference (rate-control diagnostics). */
+void prores_last_encode_counters(uint64_t *out4);
+
 /* ---- Serving loop (no counterpart in the reference) ----
  *
  * Decode and encode requests are submitted at any time from any thread. A

@@ -332,6 +332,22 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   hipEvent_t done = L.ev[4];
   (void)hipEventRecord(done, L.stream);
   if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
+  {
+    // Diagnostics: sizes the share-out computed past the window.
+    std::vector<EncState> st(NP);
+    if (NP && hipMemcpy(st.data(), ws.states.p, NP * sizeof(EncState), hipMemcpyDeviceToHost) == hipSuccess) {
+      uint64_t total = 0, most = 0, uni = 0;
+      for (const EncState &x : st) {
+        total += x.fallback;
+        most = x.fallback > most ? x.fallback : most;
+        uni += x.uniform;
+      }
+      L.enc_counters[0] = total;
+      L.enc_counters[1] = most;
+      L.enc_counters[2] = NP;
+      L.enc_counters[3] = NP ? uni / NP : 0;
+    }
+  }
   float a = 0, r = 0, p = 0, all = 0;
   (void)hipEventElapsedTime(&a, L.ev[0], L.ev[1]);
   (void)hipEventElapsedTime(&r, L.ev[1], L.ev[2]);
@@ -475,6 +491,15 @@ prores_status prores_encode_device_batch(size_t n, prores_dev_encode_job *jobs, 
   }
   for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
   return (prores_status)e.kind;
+}
+
+void prores_last_encode_counters(uint64_t *out4) {
+  Device *dev = nullptr;
+  Lane *lane0 = nullptr;
+  for (int i = 0; i < 4; i++) out4[i] = 0;
+  if (!device(dev).ok() || !lane(*dev, 0, lane0).ok()) return;
+  std::lock_guard<std::mutex> g(lane0->mu);
+  for (int i = 0; i < 4; i++) out4[i] = lane0->enc_counters[i];
 }
 
 void prores_last_encode_timing(double *us4) {

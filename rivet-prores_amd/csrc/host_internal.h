@@ -80,6 +80,7 @@ struct Lane {
   void *enc = nullptr;  // encode workspace (encode_host.cpp)
   double dec_us[5] = {0, 0, 0, 0, 0};  // last decode batch by stage
   double enc_us[4] = {0, 0, 0, 0};     // last encode batch by stage
+  uint64_t enc_counters[4] = {0, 0, 0, 0};  // last encode batch: see prores_last_encode_counters
 };
 constexpr int MAX_LANES = 8;
 

@@ -150,7 +150,7 @@ EXPORTS = [
     "prores_set_decode_tuning", "prores_get_decode_tuning", "prores_autotune_decode",
     "prores_server_config_default", "prores_server_create", "prores_server_destroy",
     "prores_server_submit_decode", "prores_server_submit_encode", "prores_server_poll",
-    "prores_server_release", "prores_server_get_stats",
+    "prores_server_release", "prores_server_get_stats", "prores_last_encode_counters",
 ]
 
 _lib = None
@@ -202,6 +202,7 @@ def lib():
         L.prores_encode_device_batch.argtypes = [C.c_size_t, P(_DevEncodeJob), P(_Err)]
         L.prores_last_batch_timing.argtypes = [P(C.c_double)]
         L.prores_last_encode_timing.argtypes = [P(C.c_double)]
+        L.prores_last_encode_counters.argtypes = [P(C.c_uint64)]
         L.prores_version.restype = C.c_char_p
         L.prores_device_alloc.argtypes = [C.c_size_t]
         L.prores_device_alloc.restype = C.c_void_p
@@ -505,6 +506,14 @@ def last_batch_timing():
     us = (C.c_double * 5)()
     lib().prores_last_batch_timing(us)
     return list(us)
+
+
+def last_encode_counters():
+    """[share-out sizes past the window, most for one picture, pictures,
+    mean uniform qi] of the last encode batch."""
+    out = (C.c_uint64 * 4)()
+    lib().prores_last_encode_counters(out)
+    return list(out)
 
 
 def last_encode_timing():
