@@ -5,7 +5,9 @@
 namespace prores_hip {
 
 constexpr uint32_t ENC_MAXK = 8;         // window columns of the size table
-constexpr uint32_t ENC_DEEP = 24;        // cached columns past the window, per slice
+constexpr uint32_t ENC_DEEP = 224;       // size cache columns per slice (column t: qi = uniform - t)
+constexpr uint32_t SHARE_WAVES = 8;      // waves of a pe_shareout workgroup
+constexpr uint32_t DEEP_STEP = 4;        // quantisers coded per cache fill
 constexpr uint32_t ALPHA_SLICE_CAP = 4608;  // bytes: 2048 values x 18 bits, the worst case
 
 struct EncFrame {

@@ -302,7 +302,7 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   if (G && !(e = check(hipMemsetAsync(ws.deep.p, 0, G * ENC_DEEP * 4, L.stream), "memset")).ok()) return e;
   {
     void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &states, &stab, &deep, &used, &qidx};
-    if (!(e = launch(L, d.k_shareout, np32, 1, 64, args)).ok()) return e;
+    if (!(e = launch(L, d.k_shareout, np32, 1, 64 * SHARE_WAVES, args)).ok()) return e;
   }
   {
     void *args[] = {&frames, &pics, &used, &sout, &fsize};

@@ -13,10 +13,11 @@
 //                (src/encode.rs:685-713).
 //   pe_window    one wavefront per slice: sizes at uniform, uniform-1, ... in
 //                one pass over the coefficients.
-//   pe_shareout  one wavefront per picture: the sequential slice-by-slice
+//   pe_shareout  one workgroup per picture: the sequential slice-by-slice
 //                share-out (src/encode.rs:769-793) solved 64 slices at a time
-//                by fixed-point iteration of its triangular recurrence; sizes
-//                past the table are coded by the whole wave and cached.
+//                by fixed-point iteration of its triangular recurrence (one
+//                wave); sizes past the table are coded by all the waves, four
+//                quantisers a request, one per wave, and cached.
 //   pe_offsets   one workgroup per frame: slice and picture offsets.
 //   pe_pack      one wavefront per slice: quantise at the chosen qi and place
 //                every codeword with a wave prefix sum over their lengths.
@@ -528,18 +529,20 @@ extern "C" __global__ void __launch_bounds__(256)
 // every lane decides from a guess of its slack, the guesses are replaced by
 // the prefix sums of the decisions, and the loop stops at the fixed point,
 // which is the sequential answer (lane j is exact after j rounds).
-extern "C" __global__ void __launch_bounds__(64)
+extern "C" __global__ void __launch_bounds__(64 * SHARE_WAVES)
     pe_shareout(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
                 const EncSlice *__restrict__ slices, const float *__restrict__ coefs,
                 const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
                 EncState *__restrict__ state, const uint32_t *__restrict__ size_tab,
                 uint32_t *__restrict__ deep, uint32_t *__restrict__ used, uint32_t *__restrict__ qidx) {
-  const uint32_t p = blockIdx.x, L = threadIdx.x;
+  __shared__ uint32_t req_g[64], req_t[64];
+  __shared__ uint32_t n_req, done;
+  const uint32_t p = blockIdx.x, tid = threadIdx.x, L = tid & 63, wave = tid >> 6;
   const EncPic &P = pics[p];
   EncState &st = state[p];
   const uint32_t n = P.n_slices, g0 = P.slice_base, U = st.uniform, K = st.window_k;
   if (!(U > 1 && st.fits_at_all)) {
-    for (uint32_t i = L; i < n; i += 64) {
+    for (uint32_t i = tid; i < n; i += 64 * SHARE_WAVES) {
       used[g0 + i] = size_tab[(size_t)(g0 + i) * ENC_MAXK] & 0x7FFFFFFFu;
       qidx[g0 + i] = U;
     }
@@ -548,94 +551,146 @@ extern "C" __global__ void __launch_bounds__(64)
   const EncFrame &F = frames[P.frame];
   const uint32_t cpm = F.chroma == 2 ? 2u : 4u;
   const float *qt = qtabs + (size_t)P.qtab * 225 * 128;
-  uint64_t sum_base = 0, total_mbs = 0;
-  for (uint32_t i = L; i < n; i += 64) {
-    sum_base += size_tab[(size_t)(g0 + i) * ENC_MAXK] & 0x7FFFFFFFu;
-    total_mbs += slices[g0 + i].mbs;
+  // Wave 0 runs the share-out; its state lives across the service rounds.
+  int64_t slack = 0, s_i = 0, extra = 0;
+  uint64_t total_mbs = 0, mbs_before = 0, M = 0, allowed = 0;
+  uint32_t c0 = 0, round = 0, phase = 0;  // 0 new chunk, 1 new round, 2 walking
+  uint32_t g = 0, base = 0, m = 0, qi = 0, u = 0, served = 0;
+  uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0, w6 = 0, w7 = 0;  // the slice's window columns
+  uint4 na = make_uint4(0, 0, 0, 0), nb = na;  // the next chunk's, loaded a chunk ahead
+  uint32_t nm_next = 0;
+  bool valid = false, walking = false;
+  auto prefetch = [&](uint32_t c) {
+    const uint32_t i = c + L;
+    const uint32_t gg = g0 + (i < n ? i : 0);
+    na = ((const uint4 *)(size_tab + (size_t)gg * ENC_MAXK))[0];
+    nb = ((const uint4 *)(size_tab + (size_t)gg * ENC_MAXK))[1];
+    nm_next = i < n ? slices[gg].mbs : 0u;
+  };
+  if (wave == 0) {
+    uint64_t sum_base = 0;
+    for (uint32_t i = L; i < n; i += 64) {
+      sum_base += size_tab[(size_t)(g0 + i) * ENC_MAXK] & 0x7FFFFFFFu;
+      total_mbs += slices[g0 + i].mbs;
+    }
+    sum_base = wave_sum64(sum_base);
+    total_mbs = wave_sum64(total_mbs);
+    slack = (int64_t)(st.budget - sum_base);
+    prefetch(0);
   }
-  sum_base = wave_sum64(sum_base);
-  total_mbs = wave_sum64(total_mbs);
-  int64_t slack = (int64_t)(st.budget - sum_base);
-  uint64_t mbs_before = 0;
-  uint32_t computed = 0;
-  for (uint32_t c0 = 0; c0 < n; c0 += 64) {
-    const uint32_t i = c0 + L;
-    const bool valid = i < n;
-    const uint32_t g = g0 + (valid ? i : 0);
-    const uint32_t base = valid ? (size_tab[(size_t)g * ENC_MAXK] & 0x7FFFFFFFu) : 0u;
-    const uint32_t m = valid ? slices[g].mbs : 0u;
-    const uint64_t M = total_mbs - mbs_before - excl_scan(m, L);
-    int64_t s_i = slack, extra = 0;
-    uint32_t q_i = U, u_i = base;
-    for (uint32_t round = 0; round <= 64; round++) {
-      // Slice i's own steps for a given slack (src/encode.rs:773-789): it
-      // steps to finer quantisers while the next size fits its share.
-      const uint64_t sc = s_i < 0 ? 0ull : (uint64_t)s_i;
-      const uint64_t allowed = valid ? base + sc * m / M : 0ull;
-      uint32_t qi = U, u = base, pv = 0;
-      bool walking = valid, have_pv = false;
-      while (__ballot(walking)) {
-        bool need = false;
-        if (walking) {
-          const uint32_t t = U - qi + 1;  // the column of qi - 1
-          uint32_t v = 0;
-          if (t < K)
-            v = size_tab[(size_t)g * ENC_MAXK + t];
-          else if (have_pv)
-            v = pv;
-          else if (t - K < ENC_DEEP)
-            v = deep[(size_t)g * ENC_DEEP + (t - K)];
-          have_pv = false;
-          if (v == 0) {
-            need = true;
-          } else if ((v >> 31) || (v & 0x7FFFFFFFu) > allowed) {
-            walking = false;
-          } else {
-            qi--;
-            u = v & 0x7FFFFFFFu;
-            if (qi <= 1) walking = false;
-          }
-        }
-        // Sizes past the table: the whole wave codes them, one slice at a time.
-        uint64_t mk = __ballot(need);
-        while (mk) {
-          const int pl = __builtin_ctzll(mk);
-          mk &= mk - 1;
-          const uint32_t gp = __shfl(g, pl, 64), qp = __shfl(qi, pl, 64) - 1;
-          uint32_t sz[1];
-          bool ft[1];
-          slice_sizes<1>(coefs + slices[gp].coef, slices[gp].mbs, cpm, qt, &qp, 1, P.header_bytes,
-                         alpha_len[gp], sz, ft);
-          computed++;
-          if ((int)L == pl) {
-            pv = sz[0] | (ft[0] ? 0u : 0x80000000u);
-            have_pv = true;
-            const uint32_t t = U - qp;
-            if (t - K < ENC_DEEP) deep[(size_t)gp * ENC_DEEP + (t - K)] = pv;
-          }
-        }
+  for (;;) {
+    if (wave == 0) {
+      if (L == 0) {
+        n_req = 0;
+        done = 0;
       }
-      q_i = qi;
-      u_i = u;
-      extra = valid ? (int64_t)u - (int64_t)base : 0;
-      const int64_t s_new = slack - excl_scan64(extra, L);
-      if (!__ballot(valid && s_new != s_i)) break;
-      s_i = s_new;
+      for (;;) {
+        if (phase == 0) {  // the next 64 slices
+          if (c0 >= n) {
+            if (L == 0) done = 1;
+            break;
+          }
+          const uint32_t i = c0 + L;
+          valid = i < n;
+          g = g0 + (valid ? i : 0);
+          w0 = na.x; w1 = na.y; w2 = na.z; w3 = na.w; w4 = nb.x; w5 = nb.y; w6 = nb.z; w7 = nb.w;
+          m = nm_next;
+          if (c0 + 64 < n) prefetch(c0 + 64);
+          base = valid ? (w0 & 0x7FFFFFFFu) : 0u;
+          M = total_mbs - mbs_before - excl_scan(m, L);
+          s_i = slack;
+          round = 0;
+          phase = 1;
+        }
+        if (phase == 1) {
+          // Slice i's own steps for a given slack (src/encode.rs:773-789): it
+          // steps to finer quantisers while the next size fits its share.
+          const uint64_t sc = s_i < 0 ? 0ull : (uint64_t)s_i;
+          allowed = valid ? base + sc * m / M : 0ull;
+          qi = U;
+          u = base;
+          walking = valid;
+          phase = 2;
+        }
+        bool need = false;
+        while (__ballot(walking)) {
+          if (walking) {
+            const uint32_t t = U - qi + 1;  // the column of qi - 1
+            uint32_t v;
+            if (t < K) {
+              v = t == 1 ? w1 : t == 2 ? w2 : t == 3 ? w3 : t == 4 ? w4 : t == 5 ? w5 : t == 6 ? w6 : w7;
+            } else {
+              v = deep[(size_t)g * ENC_DEEP + t];
+            }
+            if (v == 0) {
+              need = true;
+            } else if ((v >> 31) || (v & 0x7FFFFFFFu) > allowed) {
+              walking = false;
+            } else {
+              qi--;
+              u = v & 0x7FFFFFFFu;
+              if (qi <= 1) walking = false;
+            }
+          }
+          if (__ballot(need)) break;
+        }
+        const uint64_t nm = __ballot(need);
+        if (nm) {
+          // Sizes past the cache: the whole workgroup codes them.
+          if (need) {
+            const uint32_t slot = (uint32_t)__popcll(nm & ((1ull << L) - 1));
+            req_g[slot] = g;
+            req_t[slot] = U - qi + 1;
+          }
+          if (L == 0) n_req = (uint32_t)__popcll(nm);
+          break;
+        }
+        // Round done: replace the guessed slacks by the prefix of the
+        // decisions they imply; stop at the fixed point.
+        extra = valid ? (int64_t)u - (int64_t)base : 0;
+        const int64_t s_new = slack - excl_scan64(extra, L);
+        if (__ballot(valid && s_new != s_i) && round < 64) {
+          s_i = s_new;
+          round++;
+          phase = 1;
+          continue;
+        }
+        if (valid) {
+          used[g] = u;
+          qidx[g] = qi;
+        }
+        int64_t ext = extra;
+        for (int o = 32; o > 0; o >>= 1) {
+          const uint32_t lo = __shfl_xor((uint32_t)(uint64_t)ext, o, 64);
+          const uint32_t hi = __shfl_xor((uint32_t)((uint64_t)ext >> 32), o, 64);
+          ext += (int64_t)(((uint64_t)hi << 32) | lo);
+        }
+        slack -= ext;
+        mbs_before += wave_sum64(m);
+        c0 += 64;
+        phase = 0;
+      }
     }
-    if (valid) {
-      used[g] = u_i;
-      qidx[g] = q_i;
+    __syncthreads();
+    if (done) break;
+    // Each request: quantisers U - t ... U - t - (DEEP_STEP - 1) (down to 1)
+    // of one slice, one quantiser per wave.
+    const uint32_t nr = n_req;
+    for (uint32_t it = wave; it < nr * DEEP_STEP; it += SHARE_WAVES) {
+      const uint32_t r = it / DEEP_STEP, k = it - r * DEEP_STEP;
+      const uint32_t gr = req_g[r], t = req_t[r];
+      if (k >= U - t) continue;  // quantisers stop at 1
+      const uint32_t q = U - t - k;
+      uint32_t sz[1];
+      bool ft[1];
+      slice_sizes<1>(coefs + slices[gr].coef, slices[gr].mbs, cpm, qt, &q, 1, P.header_bytes, alpha_len[gr],
+                     sz, ft);
+      if (L == 0) deep[(size_t)gr * ENC_DEEP + t + k] = sz[0] | (ft[0] ? 0u : 0x80000000u);
     }
-    int64_t ext = extra;
-    for (int o = 32; o > 0; o >>= 1) {
-      uint32_t lo = __shfl_xor((uint32_t)(uint64_t)ext, o, 64);
-      uint32_t hi = __shfl_xor((uint32_t)((uint64_t)ext >> 32), o, 64);
-      ext += (int64_t)(((uint64_t)hi << 32) | lo);
-    }
-    slack -= ext;
-    mbs_before += wave_sum64(m);
+    served += nr;
+    __syncthreads();
   }
-  if (L == 0) st.fallback = computed;
+  if (tid == 0) st.fallback = served;
 }
 
 // ---------------------------------------------------------------------------
