@@ -502,7 +502,10 @@ def main():
     # alpha streams read and the alpha plane written; the colour planes written.
     names = ["pd_slice_table", "pd_entropy", "pd_alpha", "pd_idct"]
     kernel_bytes = [2 * n * 4050, packet_bytes - alpha_stream_bytes, alpha_stream_bytes + alpha_bytes, colour_bytes]
-    dom = int(np.argmax(stage[:4]))
+    # The dominant kernel is the longest on the main stream (slice table,
+    # colour entropy, IDCT); alpha runs beside them on a second stream and its
+    # stage is the span from the end of the slice table to its own end.
+    dom = max((0, 1, 3), key=lambda i: stage[i])
     achieved = kernel_bytes[dom] / (stage[dom] * 1e-6) / 1e9
     pipeline_gbs = (packet_bytes + colour_bytes + alpha_bytes) / (stage[4] * 1e-6) / 1e9
 

@@ -288,22 +288,41 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
       if (!(x = launch(L, d.k_slice_table, grids[0], 1, 256, args)).ok()) return x;
     }
     if (timed) (void)hipEventRecord(L.ev[1], L.stream);
+    // Alpha needs only the slice table: it runs on the second stream beside
+    // the colour entropy decode and the IDCT, which leave most of the
+    // machine's wave slots free, and joins the main stream at the end.
+    bool forked = false;
+    if (b.any_alpha) {
+      if (!(x = check(hipEventRecord(L.fork, L.stream), "hipEventRecord")).ok()) return x;
+      if (!(x = check(hipStreamWaitEvent(L.aux, L.fork, 0), "hipStreamWaitEvent")).ok()) return x;
+      forked = true;
+      void *args[] = {&B, &frames, &pics, &slices, &status, &serr, &ferr, &aord, &pers};
+      x = launch_on(L.aux, d.k_alpha, grids[2], 1, 64, args);
+      if (timed) (void)hipEventRecord(L.ev[5], L.aux);
+      Err j = check(hipEventRecord(L.join, L.aux), "hipEventRecord");
+      if (!j.ok()) return j;  // the stream joins nothing: fail before more work
+      if (!x.ok()) {
+        (void)hipStreamWaitEvent(L.stream, L.join, 0);
+        return x;
+      }
+    }
     {
       void *args[] = {&B, &frames, &pics, &slices, &status, &coefs, &wide, &serr, &ferr, &pers};
-      if (!(x = launch(L, d.k_entropy, grids[1], 1, t.entropy_wg, args, t.entropy_wg * LINE_WORDS * 4)).ok())
-        return x;
+      x = launch(L, d.k_entropy, grids[1], 1, t.entropy_wg, args, t.entropy_wg * LINE_WORDS * 4);
     }
-    if (timed) (void)hipEventRecord(L.ev[2], L.stream);
-    if (b.any_alpha) {
-      void *args[] = {&B, &frames, &pics, &slices, &status, &serr, &ferr, &aord, &pers};
-      if (!(x = launch(L, d.k_alpha, grids[2], 1, 64, args)).ok()) return x;
-    }
-    if (timed) (void)hipEventRecord(L.ev[3], L.stream);
-    {
+    if (x.ok()) {
+      if (timed) (void)hipEventRecord(L.ev[2], L.stream);
       void *args[] = {&tables, &B, &frames, &pics, &slices, &status, &serr, &coefs, &wide, &itemp, &pers};
-      if (!(x = launch(L, d.k_idct, grids[3], 1, 256, args)).ok()) return x;
+      x = launch(L, d.k_idct, grids[3], 1, 256, args);
+      if (timed) (void)hipEventRecord(L.ev[3], L.stream);
     }
-    return Err{};
+    // Whatever happened, the main stream waits for the alpha work before
+    // anything else on it can reuse the workspace.
+    if (forked) {
+      Err j = check(hipStreamWaitEvent(L.stream, L.join, 0), "hipStreamWaitEvent");
+      if (x.ok()) x = j;
+    }
+    return x;
   };
   if (np == 0) {
     L.dec_us[0] = L.dec_us[1] = L.dec_us[2] = L.dec_us[3] = L.dec_us[4] = 0;
@@ -356,9 +375,15 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   }
   if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
   if (np) {
+    // Stages: slice table, colour entropy, alpha (from the end of the slice
+    // table to its own end, beside the other two), IDCT, whole batch.
     float ms[4] = {0, 0, 0, 0}, all = 0;
-    if (!t.graph)
-      for (int i = 0; i < 4; i++) (void)hipEventElapsedTime(&ms[i], L.ev[i], L.ev[i + 1]);
+    if (!t.graph) {
+      (void)hipEventElapsedTime(&ms[0], L.ev[0], L.ev[1]);
+      (void)hipEventElapsedTime(&ms[1], L.ev[1], L.ev[2]);
+      if (b.any_alpha) (void)hipEventElapsedTime(&ms[2], L.ev[1], L.ev[5]);
+      (void)hipEventElapsedTime(&ms[3], L.ev[2], L.ev[3]);
+    }
     (void)hipEventElapsedTime(&all, L.ev[0], L.ev[4]);
     for (int i = 0; i < 4; i++) L.dec_us[i] = ms[i] * 1000.0;
     L.dec_us[4] = all * 1000.0;

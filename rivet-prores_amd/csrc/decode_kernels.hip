@@ -823,8 +823,9 @@ extern "C" __global__ void __launch_bounds__(256)
         const SliceRec S = slices[I.first + tid];
         const uint4 ev = *(const uint4 *)(slice_err + 4 * (size_t)(I.first + tid));
         // A picture whose slice table failed its bounds check has slice
-        // offsets that may point anywhere: nothing of it is read.
-        const bool ok = pic_ok && !(ev.x | ev.y | ev.z | ev.w);
+        // offsets that may point anywhere: nothing of it is read. The alpha
+        // word is not looked at: alpha is decoded beside this kernel.
+        const bool ok = pic_ok && !(ev.x | ev.y | ev.z);
         s_q[tid] = ok ? qscale(((const uint8_t *)S.off)[1]) : 0u;
         s_mbx[tid] = S.mb_x;
         s_mby[tid] = S.mb_y;

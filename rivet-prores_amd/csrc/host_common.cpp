@@ -238,10 +238,14 @@ Err DevBuf::reserve(size_t bytes) {
   return Err{};
 }
 
-Err launch(Lane &L, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args, unsigned lds) {
+Err launch_on(hipStream_t s, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args,
+              unsigned lds) {
   if (gx == 0 || gy == 0) return Err{};
-  return check(hipModuleLaunchKernel(f, gx, gy, 1, bx, 1, 1, lds, L.stream, args, nullptr),
-               "hipModuleLaunchKernel");
+  return check(hipModuleLaunchKernel(f, gx, gy, 1, bx, 1, 1, lds, s, args, nullptr), "hipModuleLaunchKernel");
+}
+
+Err launch(Lane &L, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args, unsigned lds) {
+  return launch_on(L.stream, f, gx, gy, bx, args, lds);
 }
 
 static Err load_functions(hipModule_t m, const char *const *names, hipFunction_t *const *out,
@@ -277,11 +281,17 @@ Err lane(Device &d, int index, Lane *&out) {
     Lane *L = new Lane();
     L->index = index;
     Err e = check(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (e.ok()) e = check(hipStreamCreateWithFlags(&L->aux, hipStreamNonBlocking), "hipStreamCreate");
     for (auto &ev : L->ev)
       if (e.ok()) e = check(hipEventCreate(&ev), "hipEventCreate");
+    if (e.ok()) e = check(hipEventCreateWithFlags(&L->fork, hipEventDisableTiming), "hipEventCreate");
+    if (e.ok()) e = check(hipEventCreateWithFlags(&L->join, hipEventDisableTiming), "hipEventCreate");
     if (!e.ok()) {
       for (auto &ev : L->ev)
         if (ev) (void)hipEventDestroy(ev);
+      if (L->fork) (void)hipEventDestroy(L->fork);
+      if (L->join) (void)hipEventDestroy(L->join);
+      if (L->aux) (void)hipStreamDestroy(L->aux);
       if (L->stream) (void)hipStreamDestroy(L->stream);
       delete L;
       return e;

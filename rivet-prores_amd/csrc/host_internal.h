@@ -74,6 +74,10 @@ Err parse_picture_header(const uint8_t *d, size_t len, PicHeader &out);
 struct Lane {
   int index = 0;
   hipStream_t stream = nullptr;
+  // A second stream for work that runs beside the main one (alpha decode),
+  // forked from and joined back into `stream` with the two untimed events.
+  hipStream_t aux = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   std::mutex mu;
   void *dec = nullptr;  // decode workspace (decode_host.cpp)
@@ -118,6 +122,8 @@ struct DevBuf {
 
 // `lds`: bytes of dynamic LDS per workgroup.
 Err launch(Lane &L, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args, unsigned lds = 0);
+Err launch_on(hipStream_t s, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args,
+              unsigned lds = 0);
 Err check(hipError_t e, const char *where);
 
 void build_tables(DevTables &t);
