@@ -52,23 +52,21 @@ __device__ __forceinline__ int32_t symbol_to_signed(uint32_t s) {
 // q0/q1, and the two after them are in flight as one raw 16-byte load that is
 // byte-swapped only when q0/q1 are refilled, a 16-byte stretch of the stream
 // after it was issued: the load's latency hides behind that decode, and no
-// register holding a pending load is ever copied. Bits past the end of the
-// stream read as zero, as the reference's 64-bit peek does
-// (src/bits.rs:30-40).
+// register holding a pending load is ever copied.
+//
+// window() holds whatever bytes follow the stream in memory. A code read
+// from it is accepted only when it ends inside the stream, so those bytes
+// never reach a result; the general reader uses exact_window(), in which
+// bits past the end of the stream read as zero, as the reference's 64-bit
+// peek does (src/bits.rs:30-40).
 typedef const __attribute__((address_space(1))) uint64_t gword_t;
 
 struct Reader {
   gword_t *src;
   uint64_t A, B, q0, q1, r0, r1;  // r0/r1: raw words rj, rj + 1
   uint32_t off, half, rj, maxw, pos, lenbits;
-  int32_t end_bit;
 
-  __device__ __forceinline__ uint64_t fix(uint64_t raw, uint32_t j) const {
-    uint64_t w = __builtin_bswap64(raw);
-    const int32_t valid = end_bit - 64 * (int32_t)j;
-    if (valid < 64) w = valid <= 0 ? 0ull : (w & (~0ull << (64 - valid)));
-    return w;
-  }
+  __device__ __forceinline__ static uint64_t fix(uint64_t raw) { return __builtin_bswap64(raw); }
   // Words j and j + 1 (j clamped to the last word holding stream bytes; the
   // caller's buffer has 64 readable bytes past every packet).
   __device__ __forceinline__ void issue(uint32_t j) {
@@ -80,15 +78,14 @@ struct Reader {
     const uintptr_t a = (uintptr_t)base;
     const uint32_t m = (uint32_t)(a & 7);
     src = (gword_t *)(a - m);
-    end_bit = (int32_t)(8 * (m + len));
     maxw = (m + len + 7) / 8;
     maxw = maxw ? maxw - 1 : 0;
     issue(0);
-    A = fix(r0, 0);
-    B = fix(r1, 1);
+    A = fix(r0);
+    B = fix(r1);
     issue(2);
-    q0 = fix(r0, 2);
-    q1 = fix(r1, 3);
+    q0 = fix(r0);
+    q1 = fix(r1);
     rj = 4;
     issue(4);
     half = 0;
@@ -98,6 +95,11 @@ struct Reader {
   }
   __device__ __forceinline__ uint64_t window() const {
     return off ? (A << off) | (B >> (64 - off)) : A;
+  }
+  __device__ __forceinline__ uint64_t exact_window() const {
+    const uint32_t r = remaining();
+    const uint64_t w = window();
+    return r >= 64 ? w : (r == 0 ? 0ull : w & (~0ull << (64 - r)));
   }
   __device__ __forceinline__ uint32_t remaining() const { return lenbits - pos; }
   __device__ __forceinline__ void consume(uint32_t n) {  // n <= 64
@@ -111,8 +113,8 @@ struct Reader {
         half = 1;
       } else {
         B = q1;
-        q0 = fix(r0, rj);
-        q1 = fix(r1, rj + 1);
+        q0 = fix(r0);
+        q1 = fix(r1);
         rj += 2;
         issue(rj);
         half = 0;
@@ -120,11 +122,12 @@ struct Reader {
     }
   }
   __device__ __forceinline__ uint32_t take(uint32_t n) {  // n <= 32
-    const uint32_t v = n ? (uint32_t)(window() >> (64 - n)) : 0u;
+    const uint32_t v = n ? (uint32_t)(exact_window() >> (64 - n)) : 0u;
     consume(n);
     return v;
   }
-  // RDD 36 endOfData(): 31 or fewer bits remain and all are zero.
+  // RDD 36 endOfData(): 31 or fewer bits remain and all are zero (the top r
+  // bits of the window are the stream's last r bits).
   __device__ __forceinline__ bool end_of_data() const {
     const uint32_t r = remaining();
     return r <= 31 && (r == 0 || (window() >> (64 - r)) == 0);
@@ -135,7 +138,7 @@ struct Reader {
 // (src/vlc.rs:206-230 over src/bits.rs:59-94).
 __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t &sym) {
   const uint32_t last_q = book & 15, k_rice = (book >> 4) & 15, k_exp = book >> 8;
-  const uint32_t lz = clz64(r.window());
+  const uint32_t lz = clz64(r.exact_window());
   if (lz > MAX_PREFIX) return E_PREFIX;
   if (lz >= r.remaining()) return E_PAST_END;
   r.consume(lz);
@@ -148,7 +151,7 @@ __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t
   }
   const uint32_t n = lz - (last_q + 1) + k_exp + 1;  // <= 39
   if (n > r.remaining()) return E_PAST_END;
-  const uint64_t v = r.window() >> (64 - n);
+  const uint64_t v = r.exact_window() >> (64 - n);
   r.consume(n);
   const uint64_t val = v - (1ull << k_exp) + ((uint64_t)(last_q + 1) << k_rice);
   if (val > MAX_SYMBOL) return E_RANGE;
@@ -222,8 +225,10 @@ __device__ __forceinline__ uint32_t level_book_of(uint32_t prev) {
 // bits 0-1, kRice in bits 2-3; kExp = kRice + 1, except level context 0 whose
 // kExp is 2.
 constexpr uint64_t RUN_NIB = 0x8444444555501122ull;  // contexts 0..15
-constexpr uint64_t LVL_NIB = 0x844440212ull;         // contexts 0..8
 constexpr uint32_t DC_NIB = 0xC940u;                 // contexts 0..3
+// The level codebooks with kExp spelled out, 6 bits per context: lastRiceQ
+// in bits 0-1, kRice in bits 2-3, kExp in bits 4-5 (contexts 0..8).
+constexpr uint64_t LVL_BOOKS6 = 0x38924924412462ull;
 
 // A combination code (lq, kr, ke) from the top of a 32-bit window `top`
 // (RDD 36 §7.1.1.1): q leading zeros; if q <= lq, a Rice code of q + 1 + kr
@@ -288,11 +293,11 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out) {
     if (top == 0 && rem <= 31) break;  // endOfData
     const uint32_t rc = prev_run < 15 ? prev_run : 15, lc = prev_lvl < 8 ? prev_lvl : 8;
     const uint32_t rn = (uint32_t)(RUN_NIB >> (4 * rc)) & 15;
-    const uint32_t ln = (uint32_t)(LVL_NIB >> (4 * lc)) & 15;
+    const uint32_t lv = (uint32_t)(LVL_BOOKS6 >> (6 * lc));
     uint32_t run, l1, lsym, l2;
     const bool okr = fast_code(top, rn & 3, rn >> 2, (rn >> 2) + 1, run, l1);
     const uint64_t w2 = w << (l1 & 63);
-    const bool okl = fast_code((uint32_t)(w2 >> 32), ln & 3, ln >> 2, (ln >> 2) + (lc == 0 ? 2u : 1u), lsym, l2);
+    const bool okl = fast_code((uint32_t)(w2 >> 32), lv & 3, (lv >> 2) & 3, (lv >> 4) & 3, lsym, l2);
     const uint32_t T = l1 + l2 + 1;
     if (okr && okl && T <= 64 && T <= rem) {
       n += run;
