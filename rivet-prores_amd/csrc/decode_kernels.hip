@@ -161,40 +161,30 @@ __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t
 
 // Writes one component's scanned coefficients as i16, in increasing index
 // order, through a 64-coefficient line of LDS per lane: a coefficient is one
-// LDS store, and each finished line goes to the arena as one whole 128-byte
-// line (eight 16-byte stores); the zeros between are written too, so the
-// arena needs no clearing. A value outside +-32767 is stored as -32768 with
-// the full value beside it in the i32 side arena.
+// LDS store. A finished line goes to the arena as one coalesced 128-byte
+// store made by half the wave, whichever lane it belongs to: the lines that
+// lanes leave in a step are written by the whole wave together, two at a
+// time, so no lane runs a store sequence of its own. Zeros between are
+// written too, so the arena needs no clearing. A value outside +-32767 is
+// stored as -32768 with the full value beside it in the i32 side arena.
+// The *_all calls are made by every lane of the wave together.
 struct CoefOut {
   int16_t *base;
   int32_t *wide;
-  uint32_t *line;  // this lane's LINE_WORDS dwords of LDS
-  uint32_t cur;
+  uint32_t *line;        // this lane's LINE_WORDS dwords of LDS
+  uint32_t *wave_lines;  // the wave's 64 lines
+  uint32_t cur;          // the line held in LDS
   __device__ __forceinline__ void clear() {
 #pragma unroll
     for (uint32_t i = 0; i < 16; i++) ((uint2 *)line)[i] = make_uint2(0, 0);
-  }
-  __device__ __forceinline__ void flush(uint32_t grp) {
-    uint2 v[16];
-#pragma unroll
-    for (uint32_t i = 0; i < 16; i++) v[i] = ((const uint2 *)line)[i];
-    uint4 *o = (uint4 *)(base + 64 * grp);
-#pragma unroll
-    for (uint32_t i = 0; i < 8; i++) o[i] = make_uint4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
-    clear();
   }
   __device__ __forceinline__ void zero_line(uint32_t grp) {
     uint4 *o = (uint4 *)(base + 64 * grp);
 #pragma unroll
     for (uint32_t i = 0; i < 8; i++) o[i] = make_uint4(0, 0, 0, 0);
   }
-  __device__ __forceinline__ void put(uint32_t n, int32_t v) {
-    const uint32_t grp = n >> 6;
-    if (grp != cur) {
-      flush(cur);
-      for (uint32_t g = cur + 1; g < grp; g++) zero_line(g);
-      cur = grp;
-    }
+  // Coefficient n of the line in LDS.
+  __device__ __forceinline__ void store(uint32_t n, int32_t v) {
     int32_t h = v;
     if (v > 32767 || v < -32767) {
       h = -32768;
@@ -202,9 +192,75 @@ struct CoefOut {
     }
     ((uint16_t *)line)[n & 63] = (uint16_t)(int16_t)h;
   }
-  __device__ __forceinline__ void finish(uint32_t total) {
-    flush(cur);
-    for (uint32_t g = cur + 1; g < total / 64; g++) zero_line(g);
+  // Lanes of `m` (wave-uniform): their line `cur` to the arena, and the LDS
+  // line cleared. Lanes 0-31 copy one line, lanes 32-63 the next.
+  // Lane j's value of a 32-bit lane variable, and of `base` (j uniform).
+  // readlane returns int: zero-extended here, never sign-extended into an
+  // address.
+  __device__ __forceinline__ static uint32_t lane_u32(uint32_t v, uint32_t j) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
+  }
+  __device__ __forceinline__ int16_t *lane_base(uint32_t j) const {
+    const uint64_t lo = lane_u32((uint32_t)(uintptr_t)base, j);
+    const uint64_t hi = lane_u32((uint32_t)((uintptr_t)base >> 32), j);
+    return (int16_t *)(uintptr_t)((hi << 32) | lo);
+  }
+  __device__ __forceinline__ void flush_lines_all(uint64_t m) {
+    const uint32_t L = threadIdx.x & 63;
+    while (m) {
+      const uint32_t j1 = (uint32_t)__builtin_ctzll(m);
+      m &= m - 1;
+      uint32_t j2 = 64;
+      if (m) {
+        j2 = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+      }
+      const uint32_t c1 = lane_u32(cur, j1);
+      int16_t *const b1 = lane_base(j1);
+      uint32_t c2 = 0;
+      int16_t *b2 = nullptr;
+      if (j2 < 64) {
+        c2 = lane_u32(cur, j2);
+        b2 = lane_base(j2);
+      }
+      const bool upper = L >= 32;
+      const uint32_t j = upper ? j2 : j1;
+      if (j < 64) {
+        uint32_t *src = wave_lines + LINE_WORDS * j + (L & 31);
+        const uint32_t v = *src;
+        *src = 0;
+        ((uint32_t *)((upper ? b2 : b1) + 64 * (upper ? c2 : c1)))[L & 31] = v;
+      }
+    }
+  }
+  // One step's coefficient (`have`): a lane that moves to a later line
+  // leaves its line to the wave's flush, writes the lines it skips as
+  // zeros, and stores the coefficient.
+  __device__ __forceinline__ void put_all(bool have, uint32_t n, int32_t v) {
+    const uint32_t grp = n >> 6;
+    const bool moves = have && grp != cur;
+    flush_lines_all(__ballot(moves));
+    if (moves) {
+      for (uint32_t g = cur + 1; g < grp; g++) zero_line(g);
+      cur = grp;
+    }
+    if (have) store(n, v);
+  }
+  // The end of the component for lanes with `done`: the last line, then
+  // zeros up to `nlines`, eight lines a wave store.
+  __device__ __forceinline__ void finish_all(bool done, uint32_t nlines) {
+    const uint32_t L = threadIdx.x & 63;
+    flush_lines_all(__ballot(done));
+    uint64_t m = __ballot(done && cur + 1 < nlines);
+    while (m) {
+      const uint32_t j = (uint32_t)__builtin_ctzll(m);
+      m &= m - 1;
+      const uint32_t first = lane_u32(cur, j) + 1;
+      const uint32_t count = lane_u32(nlines, j) - first;
+      int16_t *const b = lane_base(j);
+      for (uint32_t k = L >> 3; k < count; k += 8)
+        ((uint4 *)(b + 64 * (first + k)))[L & 7] = make_uint4(0, 0, 0, 0);
+    }
   }
 };
 
@@ -250,84 +306,116 @@ __device__ __forceinline__ bool fast_code(uint32_t top, uint32_t lq, uint32_t kr
   return L <= 32 && sym <= MAX_SYMBOL;
 }
 
-// scanned_coefficients() of one colour component (src/decode.rs:484-540).
-// The loop takes the fast decode whenever the codes lie in the window and
-// inside the stream, which gives the reference's result: every check it
-// makes passes. The window reads zero past the end of the stream, so
-// endOfData (at most 31 bits left, all zero) is "top word zero and at most
-// 31 bits left". Every other case goes through the general reader, check for
-// check.
-__device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out) {
-  uint32_t sym;
-  uint32_t e = read_code(r, FIRST_DC_BOOK, sym);
-  if (e) return e;
-  int32_t prev_dc = symbol_to_signed(sym);
-  out.put(0, prev_dc);
-  int32_t prev_diff = 3;
-  for (uint32_t i = 1; i < nb; i++) {
-    const uint32_t a = prev_diff < 0 ? (uint32_t)(-prev_diff) : (uint32_t)prev_diff;
-    const uint32_t c = a < 3 ? a : 3;
-    const uint32_t nib = (DC_NIB >> (4 * c)) & 15;
-    const uint32_t top = (uint32_t)(r.window() >> 32);
-    uint32_t len;
-    if (fast_code(top, nib & 3, nib >> 2, (nib >> 2) + 1, sym, len) && len <= r.remaining()) {
-      r.consume(len);
-    } else {
-      e = read_code(r, DC_BOOK[c], sym);
-      if (e) return e;
-    }
-    const int32_t n = symbol_to_signed(sym);
-    const int32_t diff = prev_diff < 0 ? -n : n;
-    const int64_t s = (int64_t)prev_dc + diff;
-    if (s > INT32_MAX || s < INT32_MIN) return E_DC_OVERFLOW;
-    prev_dc = (int32_t)s;
-    out.put(i, prev_dc);
-    prev_diff = diff;
-  }
+// scanned_coefficients() of one colour component (src/decode.rs:484-540),
+// called by every lane of the wave together; lanes with `act` false (no
+// stream, or a stream already in error) only help with the stores. The loop
+// takes the fast decode whenever the codes lie in the window and inside the
+// stream, which gives the reference's result: every check it makes passes.
+// endOfData (at most 31 bits left, all zero) is caught early when the top
+// word of the window is zero; every other case goes through the general
+// reader, check for check.
+__device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool act) {
+  uint32_t e = E_NONE;
   const uint32_t total = nb * 64;
   uint32_t n = nb, prev_run = 4, prev_lvl = 1;
-  for (;;) {
-    const uint64_t w = r.window();
-    const uint32_t top = (uint32_t)(w >> 32);
-    const uint32_t rem = r.remaining();
-    if (top == 0 && rem <= 31) break;  // endOfData
-    const uint32_t rc = prev_run < 15 ? prev_run : 15, lc = prev_lvl < 8 ? prev_lvl : 8;
-    const uint32_t rn = (uint32_t)(RUN_NIB >> (4 * rc)) & 15;
-    const uint32_t lv = (uint32_t)(LVL_BOOKS6 >> (6 * lc));
-    uint32_t run, l1, lsym, l2;
-    const bool okr = fast_code(top, rn & 3, rn >> 2, (rn >> 2) + 1, run, l1);
-    const uint64_t w2 = w << (l1 & 63);
-    const bool okl = fast_code((uint32_t)(w2 >> 32), lv & 3, (lv >> 2) & 3, (lv >> 4) & 3, lsym, l2);
-    const uint32_t T = l1 + l2 + 1;
-    if (okr && okl && T <= 64 && T <= rem) {
-      n += run;
-      if (n >= total) return E_RUN_PAST;
-      const bool neg = ((w2 >> (63 - l2)) & 1) != 0;
-      r.consume(T);
-      const int32_t level = (int32_t)lsym + 1;
-      out.put(n, neg ? -level : level);
-      n++;
-      prev_run = run;
-      prev_lvl = lsym;
-      continue;
+  if (act) {
+    // The DC coefficients: nb <= 32, all in line 0.
+    uint32_t sym = 0;
+    e = read_code(r, FIRST_DC_BOOK, sym);
+    int32_t prev_dc = symbol_to_signed(sym);
+    if (!e) out.store(0, prev_dc);
+    int32_t prev_diff = 3;
+    for (uint32_t i = 1; i < nb && !e; i++) {
+      const uint32_t a = prev_diff < 0 ? (uint32_t)(-prev_diff) : (uint32_t)prev_diff;
+      const uint32_t c = a < 3 ? a : 3;
+      const uint32_t nib = (DC_NIB >> (4 * c)) & 15;
+      const uint32_t top = (uint32_t)(r.window() >> 32);
+      uint32_t len;
+      if (fast_code(top, nib & 3, nib >> 2, (nib >> 2) + 1, sym, len) && len <= r.remaining()) {
+        r.consume(len);
+      } else {
+        e = read_code(r, DC_BOOK[c], sym);
+        if (e) break;
+      }
+      const int32_t v = symbol_to_signed(sym);
+      const int32_t diff = prev_diff < 0 ? -v : v;
+      const int64_t sum = (int64_t)prev_dc + diff;
+      if (sum > INT32_MAX || sum < INT32_MIN) {
+        e = E_DC_OVERFLOW;
+        break;
+      }
+      prev_dc = (int32_t)sum;
+      out.store(i, prev_dc);
+      prev_diff = diff;
     }
-    if (r.end_of_data()) break;
-    e = read_code(r, run_book_of(prev_run), run);
-    if (e) return e;
-    prev_run = run;
-    n += run;
-    if (n >= total) return E_RUN_PAST;
-    e = read_code(r, level_book_of(prev_lvl), lsym);
-    if (e) return e;
-    prev_lvl = lsym;
-    if (r.remaining() < 1) return E_PAST_END;
-    const uint32_t sgn = r.take(1);
-    const int32_t level = (int32_t)lsym + 1;
-    out.put(n, sgn ? -level : level);
-    n++;
+    act = !e;
   }
-  out.finish(total);
-  return E_NONE;
+  const bool started = act;
+  while (__ballot(act)) {
+    bool have = false;
+    uint32_t at = 0;
+    int32_t val = 0;
+    if (act) {
+      const uint64_t w = r.window();
+      const uint32_t top = (uint32_t)(w >> 32);
+      const uint32_t rem = r.remaining();
+      if (top == 0 && rem <= 31) {  // endOfData
+        act = false;
+      } else {
+        const uint32_t rc = prev_run < 15 ? prev_run : 15, lc = prev_lvl < 8 ? prev_lvl : 8;
+        const uint32_t rn = (uint32_t)(RUN_NIB >> (4 * rc)) & 15;
+        const uint32_t lv = (uint32_t)(LVL_BOOKS6 >> (6 * lc));
+        uint32_t run, l1, lsym, l2;
+        const bool okr = fast_code(top, rn & 3, rn >> 2, (rn >> 2) + 1, run, l1);
+        const uint64_t w2 = w << (l1 & 63);
+        const bool okl = fast_code((uint32_t)(w2 >> 32), lv & 3, (lv >> 2) & 3, (lv >> 4) & 3, lsym, l2);
+        const uint32_t T = l1 + l2 + 1;
+        if (okr && okl && T <= 64 && T <= rem) {
+          n += run;
+          if (n >= total) {
+            e = E_RUN_PAST;
+            act = false;
+          } else {
+            const bool neg = ((w2 >> (63 - l2)) & 1) != 0;
+            r.consume(T);
+            const int32_t level = (int32_t)lsym + 1;
+            have = true;
+            at = n;
+            val = neg ? -level : level;
+            n++;
+            prev_run = run;
+            prev_lvl = lsym;
+          }
+        } else if (r.end_of_data()) {
+          act = false;
+        } else {
+          e = read_code(r, run_book_of(prev_run), run);
+          if (!e) {
+            prev_run = run;
+            n += run;
+            if (n >= total) e = E_RUN_PAST;
+          }
+          if (!e) {
+            e = read_code(r, level_book_of(prev_lvl), lsym);
+            if (!e) prev_lvl = lsym;
+          }
+          if (!e && r.remaining() < 1) e = E_PAST_END;
+          if (!e) {
+            const uint32_t sgn = r.take(1);
+            const int32_t level = (int32_t)lsym + 1;
+            have = true;
+            at = n;
+            val = sgn ? -level : level;
+            n++;
+          }
+          if (e) act = false;
+        }
+      }
+    }
+    out.put_all(have, at, val);
+  }
+  out.finish_all(started && !e, nb);
+  return e;
 }
 
 // The slice header (src/decode.rs:301-338): header size, component sizes
@@ -369,36 +457,47 @@ __device__ __forceinline__ uint32_t wave_pop(uint32_t *head, uint32_t chunk) {
 }
 
 // One colour stream: component `comp` of slice `g`.
+// One colour stream per lane, every lane of the wave together; `valid`
+// lanes hold a slice.
 __device__ __forceinline__ void entropy_stream(
-    uint32_t g, uint32_t comp, uint32_t *line, const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
-    const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
+    bool valid, uint32_t g, uint32_t comp, uint32_t *line, uint32_t *wave_lines, const DecFrame *__restrict__ frames,
+    const DecPic *__restrict__ pics, const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
     int16_t *__restrict__ coefs, int32_t *__restrict__ coef_wide, uint32_t *__restrict__ slice_err,
     uint32_t *__restrict__ pic_first_err) {
-  const SliceRec S = slices[g];
-  if (pic_status[S.pic] != PIC_OK) return;
-  const DecPic &P = pics[S.pic];
-  const DecFrame &F = frames[P.frame];
-  const uint32_t alpha_bits = F.alpha_bits, chroma = F.chroma;
-
-  const uint8_t *s = (const uint8_t *)S.off;
-  const uint32_t slen = S.len;
-  uint32_t hs, ysz, cbsz, crsz;
-  uint32_t err = slice_header(s, slen, alpha_bits, hs, ysz, cbsz, crsz);
-  if (!err) {
-    Reader r;
-    const uint32_t cpm = chroma == 2 ? 2u : 4u;
-    const uint32_t nb = (comp == 0 ? 4u : cpm) * S.mbs;
-    const uint32_t off = comp == 0 ? hs : (comp == 1 ? hs + ysz : hs + ysz + cbsz);
-    const uint32_t len = comp == 0 ? ysz : (comp == 1 ? cbsz : crsz);
-    const uint64_t cofs = S.coef + (comp == 0 ? 0ull : (comp == 1 ? 256ull * S.mbs : (256ull + 64ull * cpm) * S.mbs));
-    r.init(s + off, len);
-    CoefOut out{coefs + cofs, coef_wide + cofs, line, 0};
-    out.clear();
-    err = decode_component(r, nb, out);
+  SliceRec S{};
+  uint32_t err = E_NONE, nb = 0, slice_base = 0;
+  bool act = false;
+  Reader r;
+  CoefOut out{coefs, coef_wide, line, wave_lines, 0};
+  if (valid) {
+    S = slices[g];
+    if (pic_status[S.pic] == PIC_OK) {
+      const DecPic &P = pics[S.pic];
+      const DecFrame &F = frames[P.frame];
+      slice_base = P.slice_base;
+      const uint8_t *s = (const uint8_t *)S.off;
+      uint32_t hs, ysz, cbsz, crsz;
+      err = slice_header(s, S.len, F.alpha_bits, hs, ysz, cbsz, crsz);
+      if (!err) {
+        const uint32_t cpm = F.chroma == 2 ? 2u : 4u;
+        nb = (comp == 0 ? 4u : cpm) * S.mbs;
+        const uint32_t off = comp == 0 ? hs : (comp == 1 ? hs + ysz : hs + ysz + cbsz);
+        const uint32_t len = comp == 0 ? ysz : (comp == 1 ? cbsz : crsz);
+        const uint64_t cofs =
+            S.coef + (comp == 0 ? 0ull : (comp == 1 ? 256ull * S.mbs : (256ull + 64ull * cpm) * S.mbs));
+        r.init(s + off, len);
+        out.base = coefs + cofs;
+        out.wide = coef_wide + cofs;
+        out.clear();
+        act = true;
+      }
+    }
   }
+  const uint32_t e = decode_component(r, nb, out, act);
+  if (act && e) err = e;
   if (err) {
     slice_err[4 * (size_t)g + comp] = err;
-    atomicMin(&pic_first_err[S.pic], g - P.slice_base);
+    atomicMin(&pic_first_err[S.pic], g - slice_base);
   }
 }
 
@@ -716,14 +815,15 @@ extern "C" __global__ void __launch_bounds__(256)
                uint32_t *__restrict__ pic_first_err, uint32_t persistent) {
   extern __shared__ uint32_t lines[];  // LINE_WORDS per thread
   uint32_t *line = lines + LINE_WORDS * threadIdx.x;
+  uint32_t *wave_lines = lines + LINE_WORDS * (threadIdx.x & ~63u);
   const uint32_t n_items = B->n_entropy_items, n_slices = B->n_slices;
   const uint32_t lane = threadIdx.x & 63;
   if (!persistent) {
     const uint32_t item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t g = (item / 3) * 64 + lane;
-    if (item < n_items && g < n_slices)
-      entropy_stream(g, item % 3, line, frames, pics, slices, pic_status, coefs, coef_wide, slice_err,
-                     pic_first_err);
+    if (item < n_items)  // the same for the whole wave
+      entropy_stream(g < n_slices, g, item % 3, line, wave_lines, frames, pics, slices, pic_status, coefs,
+                     coef_wide, slice_err, pic_first_err);
     return;
   }
   const uint32_t chunk = B->chunk;
@@ -733,9 +833,8 @@ extern "C" __global__ void __launch_bounds__(256)
     const uint32_t last = first + chunk < n_items ? first + chunk : n_items;
     for (uint32_t item = first; item < last; item++) {
       const uint32_t g = (item / 3) * 64 + lane;
-      if (g < n_slices)
-        entropy_stream(g, item % 3, line, frames, pics, slices, pic_status, coefs, coef_wide, slice_err,
-                       pic_first_err);
+      entropy_stream(g < n_slices, g, item % 3, line, wave_lines, frames, pics, slices, pic_status, coefs,
+                     coef_wide, slice_err, pic_first_err);
     }
   }
 }
