@@ -271,7 +271,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   unsigned grids[4];
   if (t.persistent) {
     grids[0] = resident_grid(d, d.k_slice_table, 256);
-    grids[1] = resident_grid(d, d.k_entropy, t.entropy_wg, t.entropy_wg * LINE_WORDS * 4);
+    grids[1] = resident_grid(d, d.k_entropy, t.entropy_wg, (t.entropy_wg / 64) * ENTROPY_LDS_PER_WAVE);
     grids[2] = resident_grid(d, d.k_alpha, 64);
     grids[3] = resident_grid(d, d.k_idct, 256);
   } else {
@@ -308,7 +308,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     }
     {
       void *args[] = {&B, &frames, &pics, &slices, &status, &coefs, &wide, &serr, &ferr, &pers};
-      x = launch(L, d.k_entropy, grids[1], 1, t.entropy_wg, args, t.entropy_wg * LINE_WORDS * 4);
+      x = launch(L, d.k_entropy, grids[1], 1, t.entropy_wg, args, (t.entropy_wg / 64) * ENTROPY_LDS_PER_WAVE);
     }
     if (x.ok()) {
       if (timed) (void)hipEventRecord(L.ev[2], L.stream);

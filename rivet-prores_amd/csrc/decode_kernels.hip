@@ -160,40 +160,108 @@ __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t
 }
 
 // Writes one component's scanned coefficients as i16, in increasing index
-// order, through a 64-coefficient line of LDS per lane: a coefficient is one
-// LDS store. A finished line goes to the arena as one coalesced 128-byte
-// store made by half the wave, whichever lane it belongs to: the lines that
-// lanes leave in a step are written by the whole wave together, two at a
-// time, so no lane runs a store sequence of its own. Zeros between are
-// written too, so the arena needs no clearing. A value outside +-32767 is
-// stored as -32768 with the full value beside it in the i32 side arena.
-// The *_all calls are made by every lane of the wave together.
+// order, through two 32-coefficient half-lines of LDS per lane used in
+// turn: a coefficient is one LDS store. A lane that moves to a later
+// half-line leaves the finished one pending and goes on in the other
+// buffer; only when a lane moves while it still has one pending does the
+// wave write out every pending half-line, all lanes' together (about every
+// ten steps): each lane lists its own in the wave's LDS list, then four
+// lanes copy each listed half-line as one 64-byte store, sixteen a store
+// instruction, clearing it in LDS. Zeros between are written too, so the
+// arena needs no clearing. A value outside +-32767 is stored as -32768 with
+// the full value beside it in the i32 side arena. The *_all calls are made
+// by every lane of the wave together.
 struct CoefOut {
+  static constexpr uint32_t NONE = 0xFFFFFFFFu;
   int16_t *base;
   int32_t *wide;
-  uint32_t *line;        // this lane's LINE_WORDS dwords of LDS
-  uint32_t *wave_lines;  // the wave's 64 lines
-  uint32_t cur;          // the line held in LDS
+  uint32_t *lds;   // the workgroup's lines (LINE_WORDS dwords a lane)
+  uint4 *list;     // the wave's 64 list entries
+  uint32_t line;   // this lane's line: dword index into lds
+  uint32_t cur;    // the half-line being filled
+  uint32_t ab;     // its buffer (0 or 1)
+  uint32_t pend;   // the pending half-line, or NONE; its buffer is ab ^ 1
   __device__ __forceinline__ void clear() {
+    uint4 *l = (uint4 *)(lds + line);
 #pragma unroll
-    for (uint32_t i = 0; i < 16; i++) ((uint2 *)line)[i] = make_uint2(0, 0);
+    for (uint32_t i = 0; i < 8; i++) l[i] = make_uint4(0, 0, 0, 0);
   }
-  __device__ __forceinline__ void zero_line(uint32_t grp) {
-    uint4 *o = (uint4 *)(base + 64 * grp);
+  __device__ __forceinline__ void zero_half(uint32_t h) {
+    uint4 *o = (uint4 *)(base + 32 * h);
 #pragma unroll
-    for (uint32_t i = 0; i < 8; i++) o[i] = make_uint4(0, 0, 0, 0);
+    for (uint32_t i = 0; i < 4; i++) o[i] = make_uint4(0, 0, 0, 0);
   }
-  // Coefficient n of the line in LDS.
+  // Coefficient n of the half-line being filled.
   __device__ __forceinline__ void store(uint32_t n, int32_t v) {
     int32_t h = v;
     if (v > 32767 || v < -32767) {
       h = -32768;
       wide[n] = v;
     }
-    ((uint16_t *)line)[n & 63] = (uint16_t)(int16_t)h;
+    ((uint16_t *)(lds + line + 16 * ab))[n & 31] = (uint16_t)(int16_t)h;
   }
-  // Lanes of `m` (wave-uniform): their line `cur` to the arena, and the LDS
-  // line cleared. Lanes 0-31 copy one line, lanes 32-63 the next.
+  // Every lane's pending half-line to the arena; its buffer cleared.
+  __device__ __forceinline__ void flush_pending_all() {
+    const uint32_t L = threadIdx.x & 63;
+    const bool has = pend != NONE;
+    const uint64_t m = __ballot(has);
+    if (!m) return;
+    const uint32_t cnt = (uint32_t)__popcll(m);
+    if (has) {
+      const uint32_t slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      const uint64_t dst = (uint64_t)(uintptr_t)(base + 32 * pend);
+      list[slot] = make_uint4(line + 16 * (ab ^ 1), 0u, (uint32_t)dst, (uint32_t)(dst >> 32));
+      pend = NONE;
+    }
+    for (uint32_t r = 0; r < cnt; r += 16) {
+      const uint32_t e = r + (L >> 2);
+      if (e < cnt) {
+        const uint4 ent = list[e];
+        uint4 *src = (uint4 *)(lds + ent.x) + (L & 3);
+        const uint4 v = *src;
+        *src = make_uint4(0, 0, 0, 0);
+        ((uint4 *)(uintptr_t)(((uint64_t)ent.w << 32) | ent.z))[L & 3] = v;
+      }
+    }
+  }
+  // One step's coefficient (`have`): a lane that moves to a later
+  // half-line writes the half-lines it skips as zeros and leaves the one it
+  // filled pending, first having the wave write out what is pending if it
+  // holds one already; then it stores the coefficient.
+  __device__ __forceinline__ void put_all(bool have, uint32_t n, int32_t v) {
+    const uint32_t h = n >> 5;
+    const bool moves = have && h != cur;
+    if (__ballot(moves && pend != NONE)) flush_pending_all();
+    if (moves) {
+      for (uint32_t g = cur + 1; g < h; g++) zero_half(g);
+      pend = cur;
+      ab ^= 1;
+      cur = h;
+    }
+    if (have) store(n, v);
+  }
+  // The end of the component for lanes with `done`: the pending half-line,
+  // the one being filled, then zeros up to `nhalves`, sixteen half-lines a
+  // wave store.
+  __device__ __forceinline__ void finish_all(bool done, uint32_t nhalves) {
+    const uint32_t L = threadIdx.x & 63;
+    flush_pending_all();
+    if (done) {
+      pend = cur;
+      ab ^= 1;
+    }
+    flush_pending_all();
+    uint64_t m = __ballot(done && cur + 1 < nhalves);
+    while (m) {
+      const uint32_t j = (uint32_t)__builtin_ctzll(m);
+      m &= m - 1;
+      const uint32_t first = lane_u32(cur, j) + 1;
+      const uint32_t count = lane_u32(nhalves, j) - first;
+      int16_t *const b = lane_base(j);
+      for (uint32_t k = L >> 2; k < count; k += 16)
+        ((uint4 *)(b + 32 * (first + k)))[L & 3] = make_uint4(0, 0, 0, 0);
+    }
+  }
   // Lane j's value of a 32-bit lane variable, and of `base` (j uniform).
   // readlane returns int: zero-extended here, never sign-extended into an
   // address.
@@ -204,63 +272,6 @@ struct CoefOut {
     const uint64_t lo = lane_u32((uint32_t)(uintptr_t)base, j);
     const uint64_t hi = lane_u32((uint32_t)((uintptr_t)base >> 32), j);
     return (int16_t *)(uintptr_t)((hi << 32) | lo);
-  }
-  __device__ __forceinline__ void flush_lines_all(uint64_t m) {
-    const uint32_t L = threadIdx.x & 63;
-    while (m) {
-      const uint32_t j1 = (uint32_t)__builtin_ctzll(m);
-      m &= m - 1;
-      uint32_t j2 = 64;
-      if (m) {
-        j2 = (uint32_t)__builtin_ctzll(m);
-        m &= m - 1;
-      }
-      const uint32_t c1 = lane_u32(cur, j1);
-      int16_t *const b1 = lane_base(j1);
-      uint32_t c2 = 0;
-      int16_t *b2 = nullptr;
-      if (j2 < 64) {
-        c2 = lane_u32(cur, j2);
-        b2 = lane_base(j2);
-      }
-      const bool upper = L >= 32;
-      const uint32_t j = upper ? j2 : j1;
-      if (j < 64) {
-        uint32_t *src = wave_lines + LINE_WORDS * j + (L & 31);
-        const uint32_t v = *src;
-        *src = 0;
-        ((uint32_t *)((upper ? b2 : b1) + 64 * (upper ? c2 : c1)))[L & 31] = v;
-      }
-    }
-  }
-  // One step's coefficient (`have`): a lane that moves to a later line
-  // leaves its line to the wave's flush, writes the lines it skips as
-  // zeros, and stores the coefficient.
-  __device__ __forceinline__ void put_all(bool have, uint32_t n, int32_t v) {
-    const uint32_t grp = n >> 6;
-    const bool moves = have && grp != cur;
-    flush_lines_all(__ballot(moves));
-    if (moves) {
-      for (uint32_t g = cur + 1; g < grp; g++) zero_line(g);
-      cur = grp;
-    }
-    if (have) store(n, v);
-  }
-  // The end of the component for lanes with `done`: the last line, then
-  // zeros up to `nlines`, eight lines a wave store.
-  __device__ __forceinline__ void finish_all(bool done, uint32_t nlines) {
-    const uint32_t L = threadIdx.x & 63;
-    flush_lines_all(__ballot(done));
-    uint64_t m = __ballot(done && cur + 1 < nlines);
-    while (m) {
-      const uint32_t j = (uint32_t)__builtin_ctzll(m);
-      m &= m - 1;
-      const uint32_t first = lane_u32(cur, j) + 1;
-      const uint32_t count = lane_u32(nlines, j) - first;
-      int16_t *const b = lane_base(j);
-      for (uint32_t k = L >> 3; k < count; k += 8)
-        ((uint4 *)(b + 64 * (first + k)))[L & 7] = make_uint4(0, 0, 0, 0);
-    }
   }
 };
 
@@ -414,7 +425,7 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
     }
     out.put_all(have, at, val);
   }
-  out.finish_all(started && !e, nb);
+  out.finish_all(started && !e, 2 * nb);
   return e;
 }
 
@@ -460,7 +471,7 @@ __device__ __forceinline__ uint32_t wave_pop(uint32_t *head, uint32_t chunk) {
 // One colour stream per lane, every lane of the wave together; `valid`
 // lanes hold a slice.
 __device__ __forceinline__ void entropy_stream(
-    bool valid, uint32_t g, uint32_t comp, uint32_t *line, uint32_t *wave_lines, const DecFrame *__restrict__ frames,
+    bool valid, uint32_t g, uint32_t comp, uint32_t *lds, uint4 *list, const DecFrame *__restrict__ frames,
     const DecPic *__restrict__ pics, const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
     int16_t *__restrict__ coefs, int32_t *__restrict__ coef_wide, uint32_t *__restrict__ slice_err,
     uint32_t *__restrict__ pic_first_err) {
@@ -468,7 +479,7 @@ __device__ __forceinline__ void entropy_stream(
   uint32_t err = E_NONE, nb = 0, slice_base = 0;
   bool act = false;
   Reader r;
-  CoefOut out{coefs, coef_wide, line, wave_lines, 0};
+  CoefOut out{coefs, coef_wide, lds, list, LINE_WORDS * threadIdx.x, 0, 0, CoefOut::NONE};
   if (valid) {
     S = slices[g];
     if (pic_status[S.pic] == PIC_OK) {
@@ -813,16 +824,16 @@ extern "C" __global__ void __launch_bounds__(256)
                const uint32_t *__restrict__ pic_status, int16_t *__restrict__ coefs,
                int32_t *__restrict__ coef_wide, uint32_t *__restrict__ slice_err,
                uint32_t *__restrict__ pic_first_err, uint32_t persistent) {
-  extern __shared__ uint32_t lines[];  // LINE_WORDS per thread
-  uint32_t *line = lines + LINE_WORDS * threadIdx.x;
-  uint32_t *wave_lines = lines + LINE_WORDS * (threadIdx.x & ~63u);
+  // LINE_WORDS dwords a thread, then a 64-entry list a wave.
+  extern __shared__ uint32_t lines[];
+  uint4 *list = (uint4 *)(lines + LINE_WORDS * blockDim.x) + 64 * (threadIdx.x >> 6);
   const uint32_t n_items = B->n_entropy_items, n_slices = B->n_slices;
   const uint32_t lane = threadIdx.x & 63;
   if (!persistent) {
     const uint32_t item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t g = (item / 3) * 64 + lane;
     if (item < n_items)  // the same for the whole wave
-      entropy_stream(g < n_slices, g, item % 3, line, wave_lines, frames, pics, slices, pic_status, coefs,
+      entropy_stream(g < n_slices, g, item % 3, lines, list, frames, pics, slices, pic_status, coefs,
                      coef_wide, slice_err, pic_first_err);
     return;
   }
@@ -833,7 +844,7 @@ extern "C" __global__ void __launch_bounds__(256)
     const uint32_t last = first + chunk < n_items ? first + chunk : n_items;
     for (uint32_t item = first; item < last; item++) {
       const uint32_t g = (item / 3) * 64 + lane;
-      entropy_stream(g < n_slices, g, item % 3, line, wave_lines, frames, pics, slices, pic_status, coefs,
+      entropy_stream(g < n_slices, g, item % 3, lines, list, frames, pics, slices, pic_status, coefs,
                      coef_wide, slice_err, pic_first_err);
     }
   }

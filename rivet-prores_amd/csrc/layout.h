@@ -106,7 +106,10 @@ struct IdctItem {
   uint32_t first, count, blocks, pad;
 };
 constexpr uint32_t IDCT_MAX_BLOCKS = 96;  // blocks one workgroup holds in LDS
-constexpr uint32_t LINE_WORDS = 34;  // entropy: LDS dwords per lane (a 64-coefficient line, padded)
+// Entropy decode LDS: two 32-coefficient half-lines a lane (16-byte aligned),
+// and a 64-entry list of 16-byte entries a wave.
+constexpr uint32_t LINE_WORDS = 36;
+constexpr uint32_t ENTROPY_LDS_PER_WAVE = 64 * LINE_WORDS * 4 + 64 * 16;
 
 // Slices whose alpha stream is longer than this many bytes go first in the
 // alpha work order: their decode is one long serial chain.
