@@ -363,9 +363,9 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
   }
   const bool started = act;
   while (__ballot(act)) {
-    bool have = false;
-    uint32_t at = 0;
-    int32_t val = 0;
+    bool have = false, have2 = false;
+    uint32_t at = 0, at2 = 0;
+    int32_t val = 0, val2 = 0;
     if (act) {
       const uint64_t w = r.window();
       const uint32_t top = (uint32_t)(w >> 32);
@@ -388,7 +388,6 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
             act = false;
           } else {
             const bool neg = ((w2 >> (63 - l2)) & 1) != 0;
-            r.consume(T);
             const int32_t level = (int32_t)lsym + 1;
             have = true;
             at = n;
@@ -396,6 +395,35 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
             n++;
             prev_run = run;
             prev_lvl = lsym;
+            // A second run/level/sign from the same window when it lies in
+            // it and inside the stream, and its coefficient falls in the
+            // same half-line: the reference decodes it next, with the
+            // contexts just set (its endOfData test fails, since a code
+            // follows).
+            uint32_t T2 = 0;
+            if (T < 64) {
+              const uint64_t w3 = w << T;
+              const uint32_t rc2 = run < 15 ? run : 15, lc2 = lsym < 8 ? lsym : 8;
+              const uint32_t rn2 = (uint32_t)(RUN_NIB >> (4 * rc2)) & 15;
+              const uint32_t lv2 = (uint32_t)(LVL_BOOKS6 >> (6 * lc2));
+              uint32_t run2, m1, lsym2, m2;
+              const bool okr2 = fast_code((uint32_t)(w3 >> 32), rn2 & 3, rn2 >> 2, (rn2 >> 2) + 1, run2, m1);
+              const uint64_t w4 = w3 << (m1 & 63);
+              const bool okl2 =
+                  fast_code((uint32_t)(w4 >> 32), lv2 & 3, (lv2 >> 2) & 3, (lv2 >> 4) & 3, lsym2, m2);
+              const uint32_t U = T + m1 + m2 + 1, n2 = n + run2;
+              if (okr2 && okl2 && U <= 64 && U <= rem && n2 < total && (n2 >> 5) == (at >> 5)) {
+                const int32_t level2 = (int32_t)lsym2 + 1;
+                have2 = true;
+                at2 = n2;
+                val2 = ((w4 >> (63 - m2)) & 1) != 0 ? -level2 : level2;
+                n = n2 + 1;
+                prev_run = run2;
+                prev_lvl = lsym2;
+                T2 = U - T;
+              }
+            }
+            r.consume(T + T2);
           }
         } else if (r.end_of_data()) {
           act = false;
@@ -424,6 +452,7 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
       }
     }
     out.put_all(have, at, val);
+    if (have2) out.store(at2, val2);
   }
   out.finish_all(started && !e, 2 * nb);
   return e;
