@@ -296,6 +296,9 @@ constexpr uint32_t DC_NIB = 0xC940u;                 // contexts 0..3
 // The level codebooks with kExp spelled out, 6 bits per context: lastRiceQ
 // in bits 0-1, kRice in bits 2-3, kExp in bits 4-5 (contexts 0..8).
 constexpr uint64_t LVL_BOOKS6 = 0x38924924412462ull;
+// Run/level codes decoded in one step after the first, when they fit (two
+// measured best: a third costs more than it saves).
+constexpr uint32_t EXTRA_CODES = 2;
 
 // A combination code (lq, kr, ke) from the top of a 32-bit window `top`
 // (RDD 36 §7.1.1.1): q leading zeros; if q <= lq, a Rice code of q + 1 + kr
@@ -363,9 +366,18 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
   }
   const bool started = act;
   while (__ballot(act)) {
-    bool have = false, have2 = false;
-    uint32_t at = 0, at2 = 0;
-    int32_t val = 0, val2 = 0;
+    bool have = false;
+    uint32_t at = 0;
+    int32_t val = 0;
+    bool more[EXTRA_CODES];
+    uint32_t more_at[EXTRA_CODES];
+    int32_t more_val[EXTRA_CODES];
+#pragma unroll
+    for (uint32_t k = 0; k < EXTRA_CODES; k++) {
+      more[k] = false;
+      more_at[k] = 0;
+      more_val[k] = 0;
+    }
     if (act) {
       const uint64_t w = r.window();
       const uint32_t top = (uint32_t)(w >> 32);
@@ -395,15 +407,17 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
             n++;
             prev_run = run;
             prev_lvl = lsym;
-            // A second run/level/sign from the same window when it lies in
-            // it and inside the stream, and its coefficient falls in the
-            // same half-line: the reference decodes it next, with the
+            // Further run/level/sign codes from the same window while they
+            // lie in it and inside the stream and their coefficients fall in
+            // the same half-line: the reference decodes them next, with the
             // contexts just set (its endOfData test fails, since a code
             // follows).
-            uint32_t T2 = 0;
-            if (T < 64) {
-              const uint64_t w3 = w << T;
-              const uint32_t rc2 = run < 15 ? run : 15, lc2 = lsym < 8 ? lsym : 8;
+            uint32_t used = T;
+#pragma unroll
+            for (uint32_t k = 0; k < EXTRA_CODES; k++) {
+              if (used >= 64) break;
+              const uint64_t w3 = w << used;
+              const uint32_t rc2 = prev_run < 15 ? prev_run : 15, lc2 = prev_lvl < 8 ? prev_lvl : 8;
               const uint32_t rn2 = (uint32_t)(RUN_NIB >> (4 * rc2)) & 15;
               const uint32_t lv2 = (uint32_t)(LVL_BOOKS6 >> (6 * lc2));
               uint32_t run2, m1, lsym2, m2;
@@ -411,19 +425,18 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
               const uint64_t w4 = w3 << (m1 & 63);
               const bool okl2 =
                   fast_code((uint32_t)(w4 >> 32), lv2 & 3, (lv2 >> 2) & 3, (lv2 >> 4) & 3, lsym2, m2);
-              const uint32_t U = T + m1 + m2 + 1, n2 = n + run2;
-              if (okr2 && okl2 && U <= 64 && U <= rem && n2 < total && (n2 >> 5) == (at >> 5)) {
-                const int32_t level2 = (int32_t)lsym2 + 1;
-                have2 = true;
-                at2 = n2;
-                val2 = ((w4 >> (63 - m2)) & 1) != 0 ? -level2 : level2;
-                n = n2 + 1;
-                prev_run = run2;
-                prev_lvl = lsym2;
-                T2 = U - T;
-              }
+              const uint32_t U = used + m1 + m2 + 1, n2 = n + run2;
+              if (!(okr2 && okl2 && U <= 64 && U <= rem && n2 < total && (n2 >> 5) == (at >> 5))) break;
+              const int32_t level2 = (int32_t)lsym2 + 1;
+              more[k] = true;
+              more_at[k] = n2;
+              more_val[k] = ((w4 >> (63 - m2)) & 1) != 0 ? -level2 : level2;
+              n = n2 + 1;
+              prev_run = run2;
+              prev_lvl = lsym2;
+              used = U;
             }
-            r.consume(T + T2);
+            r.consume(used);
           }
         } else if (r.end_of_data()) {
           act = false;
@@ -452,7 +465,9 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
       }
     }
     out.put_all(have, at, val);
-    if (have2) out.store(at2, val2);
+#pragma unroll
+    for (uint32_t k = 0; k < EXTRA_CODES; k++)
+      if (more[k]) out.store(more_at[k], more_val[k]);
   }
   out.finish_all(started && !e, 2 * nb);
   return e;
