@@ -208,7 +208,7 @@ uint64_t plan_segments(Batch &b, uint32_t cap, std::vector<IdctItem> &segs) {
     IdctItem cur{0, 0, 0, 0};
     for (uint32_t j = 0; j < P.row_slices; j++) {
       const uint32_t tb = b.rows[P.layout_base + j].mbs * (4 + 2 * cpm);
-      if (cur.count && (cur.blocks + tb > cap || cur.count == 16)) {
+      if (cur.count && (cur.blocks + tb > cap || cur.count == IDCT_MAX_SLICES)) {
         segs.push_back(cur);
         cur = IdctItem{j, 0, 0, 0};
       }

@@ -181,6 +181,7 @@ struct CoefOut {
   uint32_t cur;    // the half-line being filled
   uint32_t ab;     // its buffer (0 or 1)
   uint32_t pend;   // the pending half-line, or NONE; its buffer is ab ^ 1
+  uint32_t *any_wide;  // the batch's flag: set when a value goes to the side arena
   __device__ __forceinline__ void clear() {
     uint4 *l = (uint4 *)(lds + line);
 #pragma unroll
@@ -197,6 +198,7 @@ struct CoefOut {
     if (v > 32767 || v < -32767) {
       h = -32768;
       wide[n] = v;
+      *any_wide = 1;
     }
     ((uint16_t *)(lds + line + 16 * ab))[n & 31] = (uint16_t)(int16_t)h;
   }
@@ -515,7 +517,8 @@ __device__ __forceinline__ uint32_t wave_pop(uint32_t *head, uint32_t chunk) {
 // One colour stream per lane, every lane of the wave together; `valid`
 // lanes hold a slice.
 __device__ __forceinline__ void entropy_stream(
-    bool valid, uint32_t g, uint32_t comp, uint32_t *lds, uint4 *list, const DecFrame *__restrict__ frames,
+    bool valid, uint32_t g, uint32_t comp, uint32_t *lds, uint4 *list, uint32_t *any_wide,
+    const DecFrame *__restrict__ frames,
     const DecPic *__restrict__ pics, const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
     int16_t *__restrict__ coefs, int32_t *__restrict__ coef_wide, uint32_t *__restrict__ slice_err,
     uint32_t *__restrict__ pic_first_err) {
@@ -523,7 +526,7 @@ __device__ __forceinline__ void entropy_stream(
   uint32_t err = E_NONE, nb = 0, slice_base = 0;
   bool act = false;
   Reader r;
-  CoefOut out{coefs, coef_wide, lds, list, LINE_WORDS * threadIdx.x, 0, 0, CoefOut::NONE};
+  CoefOut out{coefs, coef_wide, lds, list, LINE_WORDS * threadIdx.x, 0, 0, CoefOut::NONE, any_wide};
   if (valid) {
     S = slices[g];
     if (pic_status[S.pic] == PIC_OK) {
@@ -877,7 +880,7 @@ extern "C" __global__ void __launch_bounds__(256)
     const uint32_t item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t g = (item / 3) * 64 + lane;
     if (item < n_items)  // the same for the whole wave
-      entropy_stream(g < n_slices, g, item % 3, lines, list, frames, pics, slices, pic_status, coefs,
+      entropy_stream(g < n_slices, g, item % 3, lines, list, &B->wide, frames, pics, slices, pic_status, coefs,
                      coef_wide, slice_err, pic_first_err);
     return;
   }
@@ -888,7 +891,7 @@ extern "C" __global__ void __launch_bounds__(256)
     const uint32_t last = first + chunk < n_items ? first + chunk : n_items;
     for (uint32_t item = first; item < last; item++) {
       const uint32_t g = (item / 3) * 64 + lane;
-      entropy_stream(g < n_slices, g, item % 3, lines, list, frames, pics, slices, pic_status, coefs,
+      entropy_stream(g < n_slices, g, item % 3, lines, list, &B->wide, frames, pics, slices, pic_status, coefs,
                      coef_wide, slice_err, pic_first_err);
     }
   }
@@ -935,13 +938,159 @@ extern "C" __global__ void __launch_bounds__(64)
 // One workgroup per item of consecutive slices of one picture (at most
 // IDCT_MAX_BLOCKS blocks): dequantisation, the separable f32 IDCT in the
 // reference's operation order (src/dct.rs:36-66, src/dsp/scalar.rs:8-12) and
-// clamped planar u16 stores (src/decode.rs:395-438). 32 blocks per pass, 8
-// lanes per block: a row each in the row pass, a column each in the column
-// pass, and a row each again for the 16-byte stores (through LDS). Every sum
-// starts from its first product instead of 0.0 + it, which can change only
-// the sign of a zero, and no nonzero sum or final sample depends on that.
-// Slices with an error are skipped.
-constexpr uint32_t IDCT_MAX_SLICES = 16;
+// clamped planar u16 stores (src/decode.rs:395-438).
+//
+// Set-up, once per item: the slices' quantisers, weights and scan; the
+// dequantisation factors (W q)/8 of every slice and weight matrix in the
+// item; and one descriptor per block (its coefficients, its factors, its
+// place in the output plane). Then 32 blocks per pass, 8 lanes per block
+// (the lanes of a block are in one wave, so the two transposes through LDS
+// need no workgroup barrier): a row each in the row pass, a column each in
+// the column pass, and a row each again for the 16-byte stores.
+//
+// Every sum starts from its first product instead of 0.0 + it, which can
+// change only the sign of a zero, and no nonzero sum or final sample depends
+// on that. Slices with an error are skipped, and so are blocks outside the
+// picture.
+// Per block in LDS, the transpose buffer: 8 rows of 9 floats; the samples
+// (8 rows of 8 u16, 16-byte aligned) reuse the same bytes once the column
+// pass has read them.
+constexpr uint32_t IDCT_TMP = 72;
+constexpr uint32_t IDCT_OUT = 2 * IDCT_TMP;  // the block's stride in u16
+
+// T[7 - x][u] = (-1)^u T[x][u] for every entry of the f32 basis, bit for bit.
+constexpr bool basis_mirrored() {
+  for (int x = 0; x < 8; x++)
+    for (int u = 0; u < 8; u++) {
+      const float a = BASIS[8 * x + u], b = BASIS[8 * (7 - x) + u];
+      if ((u & 1) ? !(a == -b) : !(a == b)) return false;
+    }
+  return true;
+}
+static_assert(basis_mirrored(), "the DCT basis must be mirror-symmetric");
+// Every |T[x][u]| is one of the entries of row 0.
+constexpr bool basis_magnitudes_in_row0() {
+  for (int k = 0; k < 64; k++) {
+    const float m = BASIS[k] < 0.0f ? -BASIS[k] : BASIS[k];
+    bool found = false;
+    for (int i = 0; i < 8; i++) found = found || BASIS[i] == m;
+    if (!found) return false;
+  }
+  return true;
+}
+static_assert(basis_magnitudes_in_row0(), "the DCT basis must take the magnitudes of its first row");
+
+// o[x] = sum_u a[u] T[x][u] for x = 0..7, u ascending from the u = 0 product,
+// every product and sum its own IEEE operation. The products are formed from
+// |T[x][u]| and the sign goes into the sum (s + (-p) and s - p are the same
+// IEEE operation, and a (-c) = -(a c) exactly). Outputs x and 7 - x then use
+// the same products, and equal magnitudes in a column of T (seven distinct
+// ones in all) give the same product: 22 multiplications instead of 64, and
+// every sum bit for bit the reference's.
+// The empty asm keeps each product as it is: without it the compiler turns
+// s - a c back into s + a (-c), a product of its own.
+// (Not volatile: a product no sum uses is still dropped.)
+__device__ __forceinline__ float kept(float p) {
+  asm("" : "+v"(p));
+  return p;
+}
+
+__device__ __forceinline__ void idct8(const float a[8], float o[8]) {
+  constexpr float K[8] = {BASIS[0], BASIS[1], BASIS[2], BASIS[3], BASIS[4], BASIS[5], BASIS[6], BASIS[7]};
+  // p[u][i]: a[u] times the i-th magnitude of column u of T; |T[x][u]| is
+  // one of |T[0][u']|, and the products are made once each.
+  float pr[8][8];
+#pragma unroll
+  for (int u = 0; u < 8; u++)
+#pragma unroll
+    for (int i = 0; i < 8; i++) pr[u][i] = kept(a[u] * K[i]);
+#pragma unroll
+  for (int x = 0; x < 4; x++) {
+    float s = pr[0][0];  // T[x][0] is the same for every x
+    float m = s;         // output 7 - x
+#pragma unroll
+    for (int u = 1; u < 8; u++) {
+      const float c = BASIS[8 * x + u];
+      const float mag = c < 0.0f ? -c : c;
+      int i = 0;
+      while (K[i] != mag) i++;
+      const float p = pr[u][i];
+      const bool neg = c < 0.0f, neg_m = neg != ((u & 1) != 0);
+      s = neg ? s - p : s + p;
+      m = neg_m ? m - p : m + p;
+    }
+    o[x] = s;
+    o[7 - x] = m;
+  }
+}
+
+template <bool WIDE>
+__device__ __forceinline__ void idct_pass(uint32_t t, uint32_t nblocks, uint32_t r, uint32_t lb,
+                                          const uint4 *s_blk, const int16_t *sc, const float *s_scale,
+                                          float *tmp, uint16_t *outb, const uint32_t s2[8],
+                                          const int32_t *__restrict__ wide, const uint64_t *s_plane,
+                                          const uint32_t *s_rstride, float osc, float obias, float omax) {
+  uint4 D = make_uint4(0, 0, 0, 0);
+  if (t < nblocks) D = s_blk[t];
+  const uint32_t rows = D.w & 15;
+  const bool active = rows != 0;
+  float o[8];
+  if (active) {
+    // Row pass, lane r: row r of the block, F = coefficient x (W q)/8.
+    const uint32_t lg = (D.w >> 12) & 15;
+    const float4 *sp = (const float4 *)(s_scale + D.y + 8 * r);
+    const float4 f0 = sp[0], f1 = sp[1];
+    const float fs[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
+    float a[8];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; u++) {
+      // Byte offset of coefficient nb * scan + block (s2 holds 2 scan).
+      const uint32_t at = 2 * D.x + (s2[u] << lg);
+      int32_t v = *(const int16_t *)((const char *)sc + at);
+      if (WIDE && v == -32768) v = wide[at / 2];
+      a[u] = (float)v * fs[u];
+    }
+    idct8(a, o);
+#pragma unroll
+    for (uint32_t x = 0; x < 8; x++) tmp[lb * IDCT_TMP + 9 * r + x] = o[x];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (active) {
+    // Column pass, lane r: column r; clamped samples to LDS in rows, over
+    // the block's transpose buffer (every sample depends on all eight
+    // reads, so every lane of the wave has read before any lane writes).
+    float a[8];
+#pragma unroll
+    for (uint32_t v = 0; v < 8; v++) a[v] = tmp[lb * IDCT_TMP + 9 * v + r];
+    idct8(a, o);
+#pragma unroll
+    for (uint32_t y = 0; y < 8; y++) {
+      const float sv = __builtin_amdgcn_fmed3f(floorf(o[y] * osc + obias), 0.0f, omax);
+      outb[lb * IDCT_OUT + 8 * y + r] = (uint16_t)(uint32_t)sv;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (active && r < rows) {
+    // Row r of the block to its plane (put_block, src/decode.rs:418-438).
+    const uint32_t c = (D.w >> 8) & 3, cols = (D.w >> 4) & 15;
+    uint16_t *o16 = (uint16_t *)(uintptr_t)s_plane[c] + D.z + (size_t)s_rstride[c] * r;
+    const uint4 pk = *(const uint4 *)(outb + lb * IDCT_OUT + 8 * r);
+    if (cols == 8 && (((uintptr_t)o16) & 15) == 0) {
+      *(uint4 *)o16 = pk;
+    } else {
+      const uint32_t wv[4] = {pk.x, pk.y, pk.z, pk.w};
+      for (uint32_t x = 0; x < cols; x++) o16[x] = (uint16_t)(wv[x >> 1] >> (16 * (x & 1)));
+    }
+  }
+  // The next pass reuses tmp and outb: every lane is done with them.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 extern "C" __global__ void __launch_bounds__(256)
     pd_idct(const DevTables *__restrict__ T, DecBatch *__restrict__ B,
@@ -951,14 +1100,21 @@ extern "C" __global__ void __launch_bounds__(256)
             const int32_t *__restrict__ coef_wide, const IdctItem *__restrict__ items,
             uint32_t persistent) {
   __shared__ __attribute__((aligned(16))) int16_t sc[IDCT_MAX_BLOCKS * 64];
-  __shared__ float tmp[32 * 65];
-  __shared__ __attribute__((aligned(16))) uint16_t outb[32 * 64];
+  __shared__ __attribute__((aligned(16))) float s_scale[IDCT_MAX_SLICES * 128];
+  __shared__ __attribute__((aligned(16))) float tmp[32 * IDCT_TMP];
+  uint16_t *const outb = (uint16_t *)tmp;
+  __shared__ __attribute__((aligned(16))) uint4 s_blk[IDCT_MAX_BLOCKS];
   __shared__ uint32_t s_start[IDCT_MAX_SLICES + 1], s_q[IDCT_MAX_SLICES], s_mbx[IDCT_MAX_SLICES],
       s_mby[IDCT_MAX_SLICES], s_mbs[IDCT_MAX_SLICES], s_ok[IDCT_MAX_SLICES];
-  __shared__ uint8_t s_of[IDCT_MAX_BLOCKS], s_w[128], s_scan[64];
+  __shared__ uint64_t s_plane[3];
+  __shared__ uint32_t s_rstride[3];
+  __shared__ uint8_t s_w[128], s_scan[64];
   __shared__ uint32_t s_item;
   const uint32_t tid = threadIdx.x, n_items = B->n_idct_items;
   const uint32_t chunk = persistent ? B->chunk : 1u;
+  // Coefficients outside +-32767 (from the side arena) only when the
+  // entropy decode stored any in this batch.
+  const bool any_wide = B->wide != 0;
   uint32_t first = blockIdx.x;
   for (;;) {
     if (persistent) {
@@ -977,7 +1133,7 @@ extern "C" __global__ void __launch_bounds__(256)
       const bool pic_ok = pic_status[S0.pic] == PIC_OK;
       const uint32_t chroma = F.chroma, cpm = chroma == 2 ? 2u : 4u;
       // The item's slices: block offsets, qScale, position, error state; the
-      // frame's weights and scan.
+      // frame's weights, scan and planes.
       if (tid < I.count) {
         const SliceRec S = slices[I.first + tid];
         const uint4 ev = *(const uint4 *)(slice_err + 4 * (size_t)(I.first + tid));
@@ -993,6 +1149,10 @@ extern "C" __global__ void __launch_bounds__(256)
         s_start[tid + 1] = (uint32_t)((S.coef - S0.coef) / 64) + S.mbs * (4 + 2 * cpm);
         if (tid == 0) s_start[0] = 0;
       }
+      if (tid >= 32 && tid < 35) {
+        s_plane[tid - 32] = F.out_plane[tid - 32];
+        s_rstride[tid - 32] = P.row_step * F.plane_w[tid - 32];
+      }
       if (tid >= 64 && tid < 192) s_w[tid - 64] = tid < 128 ? F.luma_w[tid - 64] : F.chroma_w[tid - 128];
       if (tid >= 192) s_scan[tid - 192] = T->scan[F.interlaced ? 1 : 0][tid - 192];
       {
@@ -1000,96 +1160,59 @@ extern "C" __global__ void __launch_bounds__(256)
         for (uint32_t i = tid; i < I.blocks * 8; i += 256) ((uint4 *)sc)[i] = src[i];
       }
       __syncthreads();
+      // Dequantisation factors, scale[k] = (f32)(W[k] qScale) / 8 for each
+      // slice and matrix (src/decode.rs:356).
+      for (uint32_t e = tid; e < I.count * 128; e += 256)
+        s_scale[e] = (float)((uint32_t)s_w[e & 127] * s_q[e >> 7]) / 8.0f;
+      // Block descriptors: x = its first coefficient in sc (the scanned
+      // index is nb * scan + block), y = its factors in s_scale, z = its
+      // top-left sample in the plane, w = rows | cols << 4 | plane << 8 |
+      // log2(nb) << 12; rows is 0 for a block that is skipped.
       for (uint32_t t = tid; t < I.blocks; t += 256) {
         uint32_t j = 0;
         while (j + 1 < I.count && s_start[j + 1] <= t) j++;
-        s_of[t] = (uint8_t)j;
+        const uint32_t mbs = s_mbs[j], nbY = 4 * mbs, nbC = cpm * mbs, tl = t - s_start[j];
+        uint32_t c = 0, b = tl, nb = nbY, cbase = 64 * s_start[j];
+        if (tl >= nbY + nbC) {
+          c = 2; b = tl - nbY - nbC; nb = nbC; cbase += 64 * (nbY + nbC);
+        } else if (tl >= nbY) {
+          c = 1; b = tl - nbY; nb = nbC; cbase += 64 * nbY;
+        }
+        // block_position (src/decode.rs:395-413).
+        const uint32_t per_mb = c == 0 ? 4u : cpm;
+        const uint32_t mbx = s_mbx[j] + b / per_mb, bb = b % per_mb, mby = s_mby[j];
+        uint32_t bx, by;
+        if (c == 0) {
+          bx = 16 * mbx + 8 * (bb & 1); by = 16u * mby + 8 * (bb >> 1);
+        } else if (chroma == 2) {
+          bx = 8 * mbx; by = 16u * mby + 8 * bb;
+        } else {
+          bx = 16 * mbx + 8 * (bb >> 1); by = 16u * mby + 8 * (bb & 1);
+        }
+        const uint32_t pw = F.plane_w[c];
+        uint32_t rows = 0, cols = 0, out = 0;
+        if (s_ok[j] && bx < pw && by < P.height) {
+          rows = P.height - by < 8 ? P.height - by : 8u;
+          cols = pw - bx < 8 ? pw - bx : 8u;
+          out = (P.first_row + P.row_step * by) * pw + bx;
+        }
+        const uint32_t lg = (uint32_t)__builtin_ctz(nb);  // nb is a power of two
+        s_blk[t] = make_uint4(cbase + b, 128 * j + (c ? 64u : 0u), out, rows | cols << 4 | c << 8 | lg << 12);
       }
-      const float osc = F.out_scale, obias = F.out_bias, omax = F.out_max;
       __syncthreads();
       const uint32_t r = tid & 7, lb = tid >> 3;
+      uint32_t s2[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; u++) s2[u] = 2u * s_scan[8 * r + u];
+      const float osc = F.out_scale, obias = F.out_bias, omax = F.out_max;
+      const int32_t *wide = coef_wide + S0.coef;
       for (uint32_t t0 = 0; t0 < I.blocks; t0 += 32) {
-        const uint32_t t = t0 + lb;
-        uint32_t j = 0, c = 0, b = 0, nb = 0, cbase = 0;
-        bool active = t < I.blocks;
-        if (active) {
-          j = s_of[t];
-          active = s_ok[j] != 0;
-          const uint32_t mbs = s_mbs[j];
-          const uint32_t nbY = 4 * mbs, nbC = cpm * mbs, tl = t - s_start[j];
-          cbase = 64 * s_start[j];
-          b = tl;
-          nb = nbY;
-          if (tl >= nbY + nbC) {
-            c = 2; b = tl - nbY - nbC; nb = nbC; cbase += 64 * (nbY + nbC);
-          } else if (tl >= nbY) {
-            c = 1; b = tl - nbY; nb = nbC; cbase += 64 * nbY;
-          }
-        }
-        // Row pass: lane r, row r of the block (F = coefficient x (W q)/8).
-        if (active) {
-          const uint32_t q = s_q[j], wo = c == 0 ? 0u : 64u;
-          float fu[8];
-#pragma unroll
-          for (uint32_t u = 0; u < 8; u++) {
-            const uint32_t k = 8 * r + u;
-            const uint32_t idx = cbase + nb * s_scan[k] + b;
-            int32_t v = sc[idx];
-            if (v == -32768) v = coef_wide[S0.coef + idx];
-            fu[u] = (float)v * ((float)((uint32_t)s_w[wo + k] * q) / 8.0f);
-          }
-#pragma unroll
-          for (uint32_t x = 0; x < 8; x++) {
-            float acc = fu[0] * BASIS[8 * x];
-#pragma unroll
-            for (uint32_t u = 1; u < 8; u++) acc = acc + fu[u] * BASIS[8 * x + u];
-            tmp[lb * 65 + 8 * r + x] = acc;
-          }
-        }
-        __syncthreads();
-        // Column pass: lane r, column r; samples to LDS in rows.
-        if (active) {
-          float tc[8];
-#pragma unroll
-          for (uint32_t v = 0; v < 8; v++) tc[v] = tmp[lb * 65 + 8 * v + r];
-#pragma unroll
-          for (uint32_t y = 0; y < 8; y++) {
-            float acc = BASIS[8 * y] * tc[0];
-#pragma unroll
-            for (uint32_t v = 1; v < 8; v++) acc = acc + BASIS[8 * y + v] * tc[v];
-            float sv = floorf(acc * osc + obias);
-            sv = sv < 0.0f ? 0.0f : (sv > omax ? omax : sv);
-            outb[lb * 64 + 8 * y + r] = (uint16_t)(uint32_t)sv;
-          }
-        }
-        __syncthreads();
-        // Row r of the block to the plane: block_position (src/decode.rs:395-413)
-        // and put_block (418-438).
-        if (active) {
-          const uint32_t per_mb = c == 0 ? 4u : cpm;
-          const uint32_t mbx = s_mbx[j] + b / per_mb, bb = b % per_mb, mby = s_mby[j];
-          uint32_t bx, by;
-          if (c == 0) {
-            bx = 16 * mbx + 8 * (bb & 1); by = 16u * mby + 8 * (bb >> 1);
-          } else if (chroma == 2) {
-            bx = 8 * mbx; by = 16u * mby + 8 * bb;
-          } else {
-            bx = 16 * mbx + 8 * (bb >> 1); by = 16u * mby + 8 * (bb & 1);
-          }
-          const uint32_t pw = F.plane_w[c];
-          const uint32_t prow = by + r;
-          if (bx < pw && prow < P.height) {
-            uint16_t *o = (uint16_t *)F.out_plane[c] + (size_t)(P.first_row + P.row_step * prow) * pw + bx;
-            const uint32_t cols = pw - bx < 8 ? pw - bx : 8u;
-            const uint4 pk = *(const uint4 *)(outb + lb * 64 + 8 * r);
-            if (cols == 8 && (((uintptr_t)o) & 15) == 0) {
-              *(uint4 *)o = pk;
-            } else {
-              const uint32_t wv[4] = {pk.x, pk.y, pk.z, pk.w};
-              for (uint32_t x = 0; x < cols; x++) o[x] = (uint16_t)(wv[x >> 1] >> (16 * (x & 1)));
-            }
-          }
-        }
+        if (any_wide)
+          idct_pass<true>(t0 + lb, I.blocks, r, lb, s_blk, sc, s_scale, tmp, outb, s2, wide, s_plane,
+                          s_rstride, osc, obias, omax);
+        else
+          idct_pass<false>(t0 + lb, I.blocks, r, lb, s_blk, sc, s_scale, tmp, outb, s2, wide, s_plane,
+                           s_rstride, osc, obias, omax);
       }
       __syncthreads();
     }

@@ -93,7 +93,7 @@ struct DecBatch {
   uint32_t n_idct_items;
   uint32_t chunk;            // items taken per queue pop by a persistent workgroup
   uint32_t heavy, light;     // alpha work order fill counts (zero at upload)
-  uint32_t pad;
+  uint32_t wide;             // nonzero once the entropy decode stores a value outside +-32767
   uint32_t q[4];             // queue heads: entropy, alpha, idct
 };
 
@@ -106,6 +106,7 @@ struct IdctItem {
   uint32_t first, count, blocks, pad;
 };
 constexpr uint32_t IDCT_MAX_BLOCKS = 96;  // blocks one workgroup holds in LDS
+constexpr uint32_t IDCT_MAX_SLICES = 12;  // slices of one item (a slice is at least 8 blocks)
 // Entropy decode LDS: two 32-coefficient half-lines a lane (16-byte aligned),
 // and a 64-entry list of 16-byte entries a wave.
 constexpr uint32_t LINE_WORDS = 36;

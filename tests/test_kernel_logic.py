@@ -127,3 +127,46 @@ def test_alpha_lane_logic_equals_the_oracle():
         v = np.ascontiguousarray(v, dtype=np.uint16)
         n = O.lib().oracle_encode_alpha(v.ctypes.data, len(v), bits, buf, 8192)
         assert lane_alpha(v, bits) == buf.raw[:n], trial
+
+
+def _basis():
+    ob = (C.c_float * 64)()
+    O.lib().oracle_basis(ob)
+    return np.array(ob[:], dtype=np.float32).reshape(8, 8)  # T[x][u]
+
+
+def test_idct_shared_products_equal_the_reference_sums():
+    """pd_idct's idct8: o[x] = sum_u a[u] T[x][u] from products with |T|,
+    each formed once (22 for 8 outputs) and signed inside the sum, must give
+    the reference's sums (src/dct.rs:36-66: from 0.0, u ascending, every
+    product and sum rounded to f32) bit for bit, up to the sign of a zero."""
+    T = _basis()
+    # The symmetries the kernel asserts at compile time.
+    for x in range(8):
+        for u in range(8):
+            want = T[x, u] if u % 2 == 0 else -T[x, u]
+            assert T[7 - x, u] == want
+    mags = np.abs(T[0])
+    rng = np.random.default_rng(5)
+    n = 200_000
+    a = rng.integers(-70000, 70000, size=(n, 8)).astype(np.float32) * rng.choice(
+        np.array([0.125, 1.0, 3.75, 1e-30, 1e6], dtype=np.float32), size=(n, 8))
+    a[rng.random((n, 8)) < 0.3] = 0.0
+    a[: 1000] = rng.standard_normal((1000, 8)).astype(np.float32) * np.float32(1e-39)  # subnormal
+    for x in range(8):
+        ref = np.zeros(n, dtype=np.float32)
+        for u in range(8):
+            ref = (ref + a[:, u] * T[x, u]).astype(np.float32)
+        # The kernel: sum from the first product, |T| products, signs in the sum.
+        xx = x if x < 4 else 7 - x
+        s = (a[:, 0] * mags[0]).astype(np.float32)
+        for u in range(1, 8):
+            c = T[xx, u]
+            i = int(np.nonzero(mags == abs(c))[0][0])
+            p = (a[:, u] * mags[i]).astype(np.float32)
+            neg = (c < 0) != (x >= 4 and u % 2 == 1)
+            s = (s - p if neg else s + p).astype(np.float32)
+        same = (s == ref) & ((s != 0) | (ref == 0))
+        assert same.all(), (x, np.nonzero(~same)[0][:5])
+        nz = ref != 0
+        assert np.array_equal(s[nz].view(np.uint32), ref[nz].view(np.uint32)), x
