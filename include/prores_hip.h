@@ -187,7 +187,9 @@ prores_status prores_decode_device_batch(const prores_decoder *d, size_t n,
  *   chunk        items taken per queue pop (persistent form), 1..64.
  *   idct_blocks  8x8 blocks per IDCT workgroup item: consecutive slices of
  *                one picture are packed up to this many blocks, 8..96.
- *   entropy_wg   threads per entropy workgroup: 64, 128 or 256.
+ *   entropy_wg   threads per entropy workgroup: 64, 128, 256, 512 or 768,
+ *                or 0 to size workgroups from the batch (one a CU when the
+ *                batch has at least 12 streams' waves a CU).
  *   graph        1: the launch sequence is captured once as a hipGraph and
  *                replayed while the workspace and launch shapes stay the
  *                same (with persistent = 1 the shapes never change). */
@@ -206,7 +208,8 @@ void prores_get_decode_tuning(prores_decode_tuning *t);
  * one untimed run) and makes the fastest the device's batch tuning. best /
  * best_us (median microseconds of the whole launch sequence) may be NULL.
  * The candidates: both forms, chunk 1 and 4, idct_blocks 32, 64 and 96,
- * entropy_wg 64 and 256, with and without graph replay. */
+ * entropy_wg 0 (sized from the batch) and 256, with and without graph
+ * replay. */
 prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,
                                      uint32_t reps, prores_decode_tuning *best, double *best_us,
                                      prores_error *err);

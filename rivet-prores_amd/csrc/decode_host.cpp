@@ -195,6 +195,22 @@ unsigned resident_grid(Device &d, hipFunction_t f, unsigned block, size_t lds = 
   return (unsigned)(per_cu * cus);
 }
 
+// Threads per entropy workgroup. A set value is used as it is; 0 picks the
+// size from the batch: enough waves a workgroup that one workgroup per CU
+// holds the batch's waves (up to ENTROPY_MAX_WG threads). Every workgroup
+// copies the decode tables into its LDS once, so fewer, larger workgroups
+// leave LDS for more waves: at 12 waves a CU one workgroup holds them all,
+// where 256-thread workgroups fit only three to a CU.
+unsigned entropy_wg(Device &d, const prores_decode_tuning &t, uint32_t items) {
+  if (t.entropy_wg) return t.entropy_wg;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d.id) != hipSuccess || cus < 1) cus = 1;
+  const uint32_t per_cu = (items + (uint32_t)cus - 1) / (uint32_t)cus;
+  for (unsigned wg : {64u, 128u, 256u, 512u})
+    if (per_cu <= wg / 64) return wg;
+  return ENTROPY_MAX_WG;
+}
+
 // IDCT row segments: runs of consecutive slices of a row holding at most
 // `cap` blocks (a larger slice is a segment of its own) and 16 slices. Every
 // row of a picture is cut the same way; returns the batch's item count.
@@ -224,7 +240,9 @@ uint64_t plan_segments(Batch &b, uint32_t cap, std::vector<IdctItem> &segs) {
 
 Err validate_tuning(const prores_decode_tuning &t) {
   if (t.persistent > 1 || t.graph > 1 || t.chunk < 1 || t.chunk > 64 || t.idct_blocks < 8 ||
-      t.idct_blocks > IDCT_MAX_BLOCKS || (t.entropy_wg != 64 && t.entropy_wg != 128 && t.entropy_wg != 256))
+      t.idct_blocks > IDCT_MAX_BLOCKS ||
+      (t.entropy_wg != 0 && t.entropy_wg != 64 && t.entropy_wg != 128 && t.entropy_wg != 256 &&
+       t.entropy_wg != 512 && t.entropy_wg != ENTROPY_MAX_WG))
     return config_error("decode tuning out of range");
   return Err{};
 }
@@ -267,11 +285,11 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   void *serr = ws.slice_err.p, *ferr = ws.first_err.p, *itemp = ws.items.p, *segp = ws.segs.p;
   void *aord = ws.alpha_order.p;
   uint32_t pers = t.persistent;
-  const unsigned ewaves = t.entropy_wg / 64;
+  const unsigned ewg = entropy_wg(d, t, hdr.n_entropy_items), ewaves = ewg / 64;
   unsigned grids[4];
   if (t.persistent) {
     grids[0] = resident_grid(d, d.k_slice_table, 256);
-    grids[1] = resident_grid(d, d.k_entropy, t.entropy_wg, (t.entropy_wg / 64) * ENTROPY_LDS_PER_WAVE);
+    grids[1] = resident_grid(d, d.k_entropy, ewg, ewaves * ENTROPY_LDS_PER_WAVE);
     grids[2] = resident_grid(d, d.k_alpha, 64);
     grids[3] = resident_grid(d, d.k_idct, 256);
   } else {
@@ -307,8 +325,8 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
       }
     }
     {
-      void *args[] = {&B, &frames, &pics, &slices, &status, &coefs, &wide, &serr, &ferr, &pers};
-      x = launch(L, d.k_entropy, grids[1], 1, t.entropy_wg, args, (t.entropy_wg / 64) * ENTROPY_LDS_PER_WAVE);
+      void *args[] = {&tables, &B, &frames, &pics, &slices, &status, &coefs, &wide, &serr, &ferr, &pers};
+      x = launch(L, d.k_entropy, grids[1], 1, ewg, args, ewaves * ENTROPY_LDS_PER_WAVE);
     }
     if (x.ok()) {
       if (timed) (void)hipEventRecord(L.ev[2], L.stream);
@@ -606,7 +624,7 @@ void prores_decode_tuning_default(prores_decode_tuning *t) {
   t->persistent = 0;
   t->chunk = 1;
   t->idct_blocks = IDCT_MAX_BLOCKS;
-  t->entropy_wg = 256;
+  t->entropy_wg = 0;
   t->graph = 0;
 }
 
@@ -652,7 +670,7 @@ prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const pr
   for (uint32_t pers = 0; pers < 2; pers++)
     for (uint32_t chunk : {1u, 4u})
       for (uint32_t blocks : {32u, 64u, 96u})
-        for (uint32_t wg : {64u, 256u})
+        for (uint32_t wg : {0u, 256u})
           for (uint32_t graph = 0; graph < 2; graph++) {
             if (!pers && chunk != 1) continue;
             cands.push_back(prores_decode_tuning{pers, chunk, blocks, wg, graph});

@@ -32,12 +32,9 @@ constexpr uint32_t MAX_PREFIX = 32;
 
 // Codebook parameters as (last_q | k_rice << 4 | k_exp << 8); an exp-Golomb
 // code of order k is the combination code (0, k, k + 1). Tables 9-11 of
-// RDD 36 (src/vlc.rs:30-68).
+// RDD 36 (src/vlc.rs:30-68): the DC books here, the run and level books in
+// run_book_params / level_book_params and the host's tables.
 __constant__ uint16_t DC_BOOK[4] = {0x100, 0x210, 0x321, 0x430};
-__constant__ uint16_t RUN_BOOK[16] = {0x102, 0x102, 0x101, 0x101, 0x100, 0x211, 0x211, 0x211,
-                                      0x211, 0x210, 0x210, 0x210, 0x210, 0x210, 0x210, 0x320};
-__constant__ uint16_t LEVEL_BOOK[9] = {0x202, 0x101, 0x102, 0x100, 0x210,
-                                       0x210, 0x210, 0x210, 0x320};
 constexpr uint32_t FIRST_DC_BOOK = 0x650;  // EG(5)
 
 __device__ __forceinline__ uint32_t clz64(uint64_t v) { return v ? __builtin_clzll(v) : 64; }
@@ -160,27 +157,26 @@ __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t
 }
 
 // Writes one component's scanned coefficients as i16, in increasing index
-// order, through two 32-coefficient half-lines of LDS per lane used in
-// turn: a coefficient is one LDS store. A lane that moves to a later
-// half-line leaves the finished one pending and goes on in the other
-// buffer; only when a lane moves while it still has one pending does the
-// wave write out every pending half-line, all lanes' together (about every
-// ten steps): each lane lists its own in the wave's LDS list, then four
-// lanes copy each listed half-line as one 64-byte store, sixteen a store
-// instruction, clearing it in LDS. Zeros between are written too, so the
-// arena needs no clearing. A value outside +-32767 is stored as -32768 with
-// the full value beside it in the i32 side arena. The *_all calls are made
-// by every lane of the wave together.
+// order, through a 64-coefficient ring of LDS per lane: half-line h (the 32
+// coefficients from 32 h) is kept in half h & 1 of the lane's line, and a
+// coefficient is one LDS store at its index mod 64. A lane that moves to a
+// later half-line leaves the finished one pending; the wave writes out the
+// pending half-lines of all its lanes together: each lane lists its own in
+// the wave's LDS list, then four lanes copy each listed half-line as one
+// 64-byte store, sixteen a store instruction, clearing it in LDS. Zeros
+// between are written too, so the arena needs no clearing. A value outside
+// +-32767 is stored as -32768 with the full value beside it in the i32 side
+// arena. The *_all calls are made by every lane of the wave together.
 struct CoefOut {
   static constexpr uint32_t NONE = 0xFFFFFFFFu;
+  static constexpr uint32_t SPARE = 64;  // a slot after the ring that is never written out
   int16_t *base;
   int32_t *wide;
   uint32_t *lds;   // the workgroup's lines (LINE_WORDS dwords a lane)
   uint4 *list;     // the wave's 64 list entries
   uint32_t line;   // this lane's line: dword index into lds
   uint32_t cur;    // the half-line being filled
-  uint32_t ab;     // its buffer (0 or 1)
-  uint32_t pend;   // the pending half-line, or NONE; its buffer is ab ^ 1
+  uint32_t pend;   // the pending half-line, or NONE
   uint32_t *any_wide;  // the batch's flag: set when a value goes to the side arena
   __device__ __forceinline__ void clear() {
     uint4 *l = (uint4 *)(lds + line);
@@ -192,7 +188,7 @@ struct CoefOut {
 #pragma unroll
     for (uint32_t i = 0; i < 4; i++) o[i] = make_uint4(0, 0, 0, 0);
   }
-  // Coefficient n of the half-line being filled.
+  // Coefficient n, of the half-line being filled or the next one.
   __device__ __forceinline__ void store(uint32_t n, int32_t v) {
     int32_t h = v;
     if (v > 32767 || v < -32767) {
@@ -200,9 +196,13 @@ struct CoefOut {
       wide[n] = v;
       *any_wide = 1;
     }
-    ((uint16_t *)(lds + line + 16 * ab))[n & 31] = (uint16_t)(int16_t)h;
+    ((uint16_t *)(lds + line))[n & 63] = (uint16_t)(int16_t)h;
   }
-  // Every lane's pending half-line to the arena; its buffer cleared.
+  // A value of magnitude below 32768 at ring slot `slot` (SPARE: nowhere).
+  __device__ __forceinline__ void store_slot(uint32_t slot, int32_t v) {
+    ((uint16_t *)(lds + line))[slot] = (uint16_t)(int16_t)v;
+  }
+  // Every lane's pending half-line to the arena; its half of the ring cleared.
   __device__ __forceinline__ void flush_pending_all() {
     const uint32_t L = threadIdx.x & 63;
     const bool has = pend != NONE;
@@ -212,7 +212,7 @@ struct CoefOut {
     if (has) {
       const uint32_t slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
       const uint64_t dst = (uint64_t)(uintptr_t)(base + 32 * pend);
-      list[slot] = make_uint4(line + 16 * (ab ^ 1), 0u, (uint32_t)dst, (uint32_t)(dst >> 32));
+      list[slot] = make_uint4(line + 16 * (pend & 1), 0u, (uint32_t)dst, (uint32_t)(dst >> 32));
       pend = NONE;
     }
     for (uint32_t r = 0; r < cnt; r += 16) {
@@ -226,19 +226,20 @@ struct CoefOut {
       }
     }
   }
-  // One step's coefficient (`have`): a lane that moves to a later
-  // half-line writes the half-lines it skips as zeros and leaves the one it
-  // filled pending, first having the wave write out what is pending if it
-  // holds one already; then it stores the coefficient.
+  // One coefficient (`have`) of a lane that may move past the next
+  // half-line: the half-lines it leaves go out first, the ones it skips are
+  // written as zeros, then it stores the coefficient.
   __device__ __forceinline__ void put_all(bool have, uint32_t n, int32_t v) {
     const uint32_t h = n >> 5;
     const bool moves = have && h != cur;
-    if (__ballot(moves && pend != NONE)) flush_pending_all();
-    if (moves) {
-      for (uint32_t g = cur + 1; g < h; g++) zero_half(g);
-      pend = cur;
-      ab ^= 1;
-      cur = h;
+    if (__ballot(moves)) {
+      flush_pending_all();
+      if (moves) pend = cur;
+      flush_pending_all();
+      if (moves) {
+        for (uint32_t g = cur + 1; g < h; g++) zero_half(g);
+        cur = h;
+      }
     }
     if (have) store(n, v);
   }
@@ -248,10 +249,7 @@ struct CoefOut {
   __device__ __forceinline__ void finish_all(bool done, uint32_t nhalves) {
     const uint32_t L = threadIdx.x & 63;
     flush_pending_all();
-    if (done) {
-      pend = cur;
-      ab ^= 1;
-    }
+    if (done) pend = cur;
     flush_pending_all();
     uint64_t m = __ballot(done && cur + 1 < nhalves);
     while (m) {
@@ -277,30 +275,9 @@ struct CoefOut {
   }
 };
 
-// Codebook parameters of the run and level contexts, 12 bits each as in
-// RUN_BOOK / LEVEL_BOOK, for the general reader.
-__device__ __forceinline__ uint32_t run_book_of(uint32_t prev) {
-  const uint32_t c = prev < 15 ? prev : 15;
-  // contexts 0-1: (2,0,1); 2-3: (1,0,1); 4: (0,0,1); 5-8: (1,1,2); 9-14: (0,1,2); 15: (0,2,3)
-  return c < 2 ? 0x102u : c < 4 ? 0x101u : c == 4 ? 0x100u : c < 9 ? 0x211u : c < 15 ? 0x210u : 0x320u;
-}
-__device__ __forceinline__ uint32_t level_book_of(uint32_t prev) {
-  const uint32_t c = prev < 8 ? prev : 8;
-  // 0: (2,0,2); 1: (1,0,1); 2: (2,0,1); 3: (0,0,1); 4-7: (0,1,2); 8: (0,2,3)
-  return c == 0 ? 0x202u : c == 1 ? 0x101u : c == 2 ? 0x102u : c == 3 ? 0x100u : c < 8 ? 0x210u : 0x320u;
-}
-
-// The same codebooks for the fast decode, 4 bits per context: lastRiceQ in
-// bits 0-1, kRice in bits 2-3; kExp = kRice + 1, except level context 0 whose
-// kExp is 2.
-constexpr uint64_t RUN_NIB = 0x8444444555501122ull;  // contexts 0..15
-constexpr uint32_t DC_NIB = 0xC940u;                 // contexts 0..3
-// The level codebooks with kExp spelled out, 6 bits per context: lastRiceQ
-// in bits 0-1, kRice in bits 2-3, kExp in bits 4-5 (contexts 0..8).
-constexpr uint64_t LVL_BOOKS6 = 0x38924924412462ull;
-// Run/level codes decoded in one step after the first, when they fit (two
-// measured best: a third costs more than it saves).
-constexpr uint32_t EXTRA_CODES = 2;
+// The DC codebooks for the closed-form decode, 4 bits per context:
+// lastRiceQ in bits 0-1, kRice in bits 2-3; kExp = kRice + 1.
+constexpr uint32_t DC_NIB = 0xC940u;  // contexts 0..3
 
 // A combination code (lq, kr, ke) from the top of a 32-bit window `top`
 // (RDD 36 §7.1.1.1): q leading zeros; if q <= lq, a Rice code of q + 1 + kr
@@ -324,18 +301,51 @@ __device__ __forceinline__ bool fast_code(uint32_t top, uint32_t lq, uint32_t kr
 
 // scanned_coefficients() of one colour component (src/decode.rs:484-540),
 // called by every lane of the wave together; lanes with `act` false (no
-// stream, or a stream already in error) only help with the stores. The loop
-// takes the fast decode whenever the codes lie in the window and inside the
-// stream, which gives the reference's result: every check it makes passes.
-// endOfData (at most 31 bits left, all zero) is caught early when the top
-// word of the window is zero; every other case goes through the general
-// reader, check for check.
-__device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool act) {
+// stream, or a stream already in error) only help with the stores.
+//
+// In each step of the AC loop every lane decodes run/level/sign pairs from
+// its 64-bit window
+// through the LDS tables `ftab` (codes of at most 9 bits; the next 9 bits
+// of the stream index the book of the current context), as long as the
+// pair lies inside the stream, its coefficient falls in the half-line being
+// filled or, once a step, the next one, and the window holds another
+// pair's worth. Such a pair is the reference's next pair: every code holds
+// a 1 bit, so endOfData (at most 31 bits left, all zero) cannot hold before
+// a code that ends inside the stream, and a code of at most 9 bits passes
+// every other check. A lane that decodes no pair from the tables (a longer
+// code, the end of the stream, a run past a half-line, an error) decodes one
+// pair in closed form or with the general reader, check for check, in the
+// same step. The step ends with the wave writing out its pending half-lines.
+constexpr uint32_t FAST_PAIRS = 12;  // table pairs per step at most
+constexpr uint32_t FAST_START = 45;  // a pair needs 9 + 10 bits of the 64-bit window
+// The table book of a run or level context, 4 bits per context: run books
+// sit 1024 bytes apart after FAST_RUN_BASE, level books 2048 bytes apart.
+constexpr uint64_t RUN_CTX_NIB = 0x5444444333321100ull;  // contexts 0..15
+constexpr uint64_t LVL_CTX_NIB = 0x544443210ull;         // contexts 0..8
+__device__ __forceinline__ uint32_t run_book_off(uint32_t prev) {
+  const uint32_t c = prev < 15 ? prev : 15;
+  return (uint32_t)((RUN_CTX_NIB >> (4 * c)) & 15) << 10;
+}
+__device__ __forceinline__ uint32_t level_book_off(uint32_t prev) {
+  const uint32_t c = prev < 8 ? prev : 8;
+  return (uint32_t)((LVL_CTX_NIB >> (4 * c)) & 15) << 11;
+}
+// Codebook parameters (lastRiceQ | kRice << 4 | kExp << 8) of a book.
+__device__ __forceinline__ uint32_t run_book_params(uint32_t off) {
+  const uint32_t b = off >> 10;
+  return b == 0 ? 0x102u : b == 1 ? 0x101u : b == 2 ? 0x100u : b == 3 ? 0x211u : b == 4 ? 0x210u : 0x320u;
+}
+__device__ __forceinline__ uint32_t level_book_params(uint32_t off) {
+  const uint32_t b = off >> 11;
+  return b == 0 ? 0x202u : b == 1 ? 0x101u : b == 2 ? 0x102u : b == 3 ? 0x100u : b == 4 ? 0x210u : 0x320u;
+}
+
+__device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool act, const uint16_t *ftab) {
   uint32_t e = E_NONE;
   const uint32_t total = nb * 64;
-  uint32_t n = nb, prev_run = 4, prev_lvl = 1;
+  uint32_t n = nb;
   if (act) {
-    // The DC coefficients: nb <= 32, all in line 0.
+    // The DC coefficients: nb <= 32, all in half-line 0.
     uint32_t sym = 0;
     e = read_code(r, FIRST_DC_BOOK, sym);
     int32_t prev_dc = symbol_to_signed(sym);
@@ -367,109 +377,113 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
     act = !e;
   }
   const bool started = act;
+  // The books of the current run and level contexts (prev_run 4, prev_lvl 1).
+  uint32_t rbo = run_book_off(4), lbo = level_book_off(1);
   while (__ballot(act)) {
-    bool have = false;
-    uint32_t at = 0;
-    int32_t val = 0;
-    bool more[EXTRA_CODES];
-    uint32_t more_at[EXTRA_CODES];
-    int32_t more_val[EXTRA_CODES];
-#pragma unroll
-    for (uint32_t k = 0; k < EXTRA_CODES; k++) {
-      more[k] = false;
-      more_at[k] = 0;
-      more_val[k] = 0;
-    }
+    // Pairs from the tables. Every lane's half-lines before `cur` are out,
+    // so the pairs may run up to the end of the next half-line.
+    bool moved = false;
     if (act) {
-      const uint64_t w = r.window();
-      const uint32_t top = (uint32_t)(w >> 32);
+      uint64_t w = r.window();
       const uint32_t rem = r.remaining();
-      if (top == 0 && rem <= 31) {  // endOfData
-        act = false;
-      } else {
-        const uint32_t rc = prev_run < 15 ? prev_run : 15, lc = prev_lvl < 8 ? prev_lvl : 8;
-        const uint32_t rn = (uint32_t)(RUN_NIB >> (4 * rc)) & 15;
-        const uint32_t lv = (uint32_t)(LVL_BOOKS6 >> (6 * lc));
-        uint32_t run, l1, lsym, l2;
-        const bool okr = fast_code(top, rn & 3, rn >> 2, (rn >> 2) + 1, run, l1);
-        const uint64_t w2 = w << (l1 & 63);
-        const bool okl = fast_code((uint32_t)(w2 >> 32), lv & 3, (lv >> 2) & 3, (lv >> 4) & 3, lsym, l2);
-        const uint32_t T = l1 + l2 + 1;
-        if (okr && okl && T <= 64 && T <= rem) {
-          n += run;
-          if (n >= total) {
-            e = E_RUN_PAST;
-            act = false;
-          } else {
-            const bool neg = ((w2 >> (63 - l2)) & 1) != 0;
-            const int32_t level = (int32_t)lsym + 1;
-            have = true;
-            at = n;
-            val = neg ? -level : level;
-            n++;
-            prev_run = run;
-            prev_lvl = lsym;
-            // Further run/level/sign codes from the same window while they
-            // lie in it and inside the stream and their coefficients fall in
-            // the same half-line: the reference decodes them next, with the
-            // contexts just set (its endOfData test fails, since a code
-            // follows).
-            uint32_t used = T;
-#pragma unroll
-            for (uint32_t k = 0; k < EXTRA_CODES; k++) {
-              if (used >= 64) break;
-              const uint64_t w3 = w << used;
-              const uint32_t rc2 = prev_run < 15 ? prev_run : 15, lc2 = prev_lvl < 8 ? prev_lvl : 8;
-              const uint32_t rn2 = (uint32_t)(RUN_NIB >> (4 * rc2)) & 15;
-              const uint32_t lv2 = (uint32_t)(LVL_BOOKS6 >> (6 * lc2));
-              uint32_t run2, m1, lsym2, m2;
-              const bool okr2 = fast_code((uint32_t)(w3 >> 32), rn2 & 3, rn2 >> 2, (rn2 >> 2) + 1, run2, m1);
-              const uint64_t w4 = w3 << (m1 & 63);
-              const bool okl2 =
-                  fast_code((uint32_t)(w4 >> 32), lv2 & 3, (lv2 >> 2) & 3, (lv2 >> 4) & 3, lsym2, m2);
-              const uint32_t U = used + m1 + m2 + 1, n2 = n + run2;
-              if (!(okr2 && okl2 && U <= 64 && U <= rem && n2 < total && (n2 >> 5) == (at >> 5))) break;
-              const int32_t level2 = (int32_t)lsym2 + 1;
-              more[k] = true;
-              more_at[k] = n2;
-              more_val[k] = ((w4 >> (63 - m2)) & 1) != 0 ? -level2 : level2;
-              n = n2 + 1;
-              prev_run = run2;
-              prev_lvl = lsym2;
-              used = U;
-            }
-            r.consume(used);
-          }
-        } else if (r.end_of_data()) {
-          act = false;
-        } else {
-          e = read_code(r, run_book_of(prev_run), run);
-          if (!e) {
-            prev_run = run;
-            n += run;
-            if (n >= total) e = E_RUN_PAST;
-          }
-          if (!e) {
-            e = read_code(r, level_book_of(prev_lvl), lsym);
-            if (!e) prev_lvl = lsym;
-          }
-          if (!e && r.remaining() < 1) e = E_PAST_END;
-          if (!e) {
-            const uint32_t sgn = r.take(1);
-            const int32_t level = (int32_t)lsym + 1;
-            have = true;
-            at = n;
-            val = sgn ? -level : level;
-            n++;
-          }
-          if (e) act = false;
+      const uint32_t lim0 = 32 * (out.cur + 2), lim = lim0 < total ? lim0 : total;
+      uint32_t used = 0;
+      bool go = true;
+      for (uint32_t k = 0; k < FAST_PAIRS; k++) {
+        if (!__ballot(go)) break;
+        const uint32_t hi = (uint32_t)(w >> 32), lo = (uint32_t)w;
+        const uint32_t er =
+            *(const uint16_t *)((const char *)ftab + FAST_RUN_BASE + (rbo | ((hi >> 22) & 0x3FE)));
+        const uint32_t lr = er & 15;
+        const uint32_t hi2 = __builtin_amdgcn_alignbit(hi, lo, 32 - lr);  // w << lr, top word
+        const uint32_t el = *(const uint16_t *)((const char *)ftab + (lbo | ((hi2 >> 21) & 0x7FE)));
+        const uint32_t t = lr + (el & 15), m = n + ((er >> 4) & 63);
+        const bool ok = go && (er & el & FAST_VALID) != 0 && used + t <= rem && m < lim;
+        out.store_slot(ok ? (m & 63) : CoefOut::SPARE, ((int32_t)(el << 21)) >> 25);
+        n = ok ? m + 1 : n;
+        used = ok ? used + t : used;
+        w = ok ? w << t : w;
+        rbo = ok ? (er & 0x1C00) : rbo;
+        lbo = ok ? (el & 0x3800) : lbo;
+        moved = moved || ok;
+        go = ok && used <= FAST_START;
+      }
+      if (moved) {
+        r.consume(used);
+        const uint32_t c = (n - 1) >> 5;
+        if (c != out.cur) {
+          out.pend = out.cur;
+          out.cur = c;
         }
       }
     }
-    out.put_all(have, at, val);
-#pragma unroll
-    for (uint32_t k = 0; k < EXTRA_CODES; k++)
-      if (more[k]) out.store(more_at[k], more_val[k]);
+    // One pair the reference's way for lanes the tables did not move.
+    const bool slow = act && !moved;
+    if (__ballot(slow)) {
+      bool have = false;
+      uint32_t at = 0;
+      int32_t val = 0;
+      if (slow) {
+        const uint64_t w = r.window();
+        const uint32_t top = (uint32_t)(w >> 32);
+        const uint32_t rem = r.remaining();
+        if (top == 0 && rem <= 31) {  // endOfData
+          act = false;
+        } else {
+          const uint32_t rb = run_book_params(rbo), lb = level_book_params(lbo);
+          uint32_t run, l1, lsym, l2;
+          const bool okr = fast_code(top, rb & 15, (rb >> 4) & 15, rb >> 8, run, l1);
+          const uint64_t w2 = w << (l1 & 63);
+          const bool okl = fast_code((uint32_t)(w2 >> 32), lb & 15, (lb >> 4) & 15, lb >> 8, lsym, l2);
+          const uint32_t T = l1 + l2 + 1;
+          if (okr && okl && T <= 64 && T <= rem) {
+            n += run;
+            if (n >= total) {
+              e = E_RUN_PAST;
+              act = false;
+            } else {
+              const bool neg = ((w2 >> (63 - l2)) & 1) != 0;
+              const int32_t level = (int32_t)lsym + 1;
+              have = true;
+              at = n;
+              val = neg ? -level : level;
+              n++;
+              rbo = run_book_off(run);
+              lbo = level_book_off(lsym);
+              r.consume(T);
+            }
+          } else if (r.end_of_data()) {
+            act = false;
+          } else {
+            e = read_code(r, rb, run);
+            if (!e) {
+              rbo = run_book_off(run);
+              n += run;
+              if (n >= total) e = E_RUN_PAST;
+            }
+            if (!e) {
+              e = read_code(r, lb, lsym);
+              if (!e) lbo = level_book_off(lsym);
+            }
+            if (!e && r.remaining() < 1) e = E_PAST_END;
+            if (!e) {
+              const uint32_t sgn = r.take(1);
+              const int32_t level = (int32_t)lsym + 1;
+              have = true;
+              at = n;
+              val = sgn ? -level : level;
+              n++;
+            }
+            if (e) act = false;
+          }
+        }
+      }
+      out.put_all(have, at, val);
+    }
+    // The half-lines finished this step go out last: vmcnt counts stores
+    // as well as loads, so a bitstream refill waits for every store issued
+    // before it, and these have the next step's table decode to complete.
+    out.flush_pending_all();
   }
   out.finish_all(started && !e, 2 * nb);
   return e;
@@ -518,7 +532,7 @@ __device__ __forceinline__ uint32_t wave_pop(uint32_t *head, uint32_t chunk) {
 // lanes hold a slice.
 __device__ __forceinline__ void entropy_stream(
     bool valid, uint32_t g, uint32_t comp, uint32_t *lds, uint4 *list, uint32_t *any_wide,
-    const DecFrame *__restrict__ frames,
+    const uint16_t *ftab, const DecFrame *__restrict__ frames,
     const DecPic *__restrict__ pics, const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
     int16_t *__restrict__ coefs, int32_t *__restrict__ coef_wide, uint32_t *__restrict__ slice_err,
     uint32_t *__restrict__ pic_first_err) {
@@ -526,7 +540,7 @@ __device__ __forceinline__ void entropy_stream(
   uint32_t err = E_NONE, nb = 0, slice_base = 0;
   bool act = false;
   Reader r;
-  CoefOut out{coefs, coef_wide, lds, list, LINE_WORDS * threadIdx.x, 0, 0, CoefOut::NONE, any_wide};
+  CoefOut out{coefs, coef_wide, lds, list, LINE_WORDS * threadIdx.x, 0, CoefOut::NONE, any_wide};
   if (valid) {
     S = slices[g];
     if (pic_status[S.pic] == PIC_OK) {
@@ -551,7 +565,7 @@ __device__ __forceinline__ void entropy_stream(
       }
     }
   }
-  const uint32_t e = decode_component(r, nb, out, act);
+  const uint32_t e = decode_component(r, nb, out, act, ftab);
   if (act && e) err = e;
   if (err) {
     slice_err[4 * (size_t)g + comp] = err;
@@ -865,8 +879,8 @@ extern "C" __global__ void __launch_bounds__(256)
 // One lane per colour stream; an item is 64 streams of one component (64
 // consecutive slices), one wave's work: item i is component i % 3 of slices
 // 64 (i / 3) ... 64 (i / 3) + 63. Dynamic LDS: LINE_WORDS dwords per thread.
-extern "C" __global__ void __launch_bounds__(256)
-    pd_entropy(DecBatch *__restrict__ B, const DecFrame *__restrict__ frames,
+extern "C" __global__ void __launch_bounds__(ENTROPY_MAX_WG)
+    pd_entropy(const DevTables *__restrict__ T, DecBatch *__restrict__ B, const DecFrame *__restrict__ frames,
                const DecPic *__restrict__ pics, const SliceRec *__restrict__ slices,
                const uint32_t *__restrict__ pic_status, int16_t *__restrict__ coefs,
                int32_t *__restrict__ coef_wide, uint32_t *__restrict__ slice_err,
@@ -874,13 +888,18 @@ extern "C" __global__ void __launch_bounds__(256)
   // LINE_WORDS dwords a thread, then a 64-entry list a wave.
   extern __shared__ uint32_t lines[];
   uint4 *list = (uint4 *)(lines + LINE_WORDS * blockDim.x) + 64 * (threadIdx.x >> 6);
+  __shared__ __attribute__((aligned(16))) uint16_t ftab[6 * 1024 + 6 * 512];
+  static_assert(sizeof(ftab) == sizeof(T->fast_level) + sizeof(T->fast_run), "the tables are contiguous");
+  for (uint32_t i = threadIdx.x; i < sizeof(ftab) / 16; i += blockDim.x)
+    ((uint4 *)ftab)[i] = ((const uint4 *)T->fast_level)[i];
+  __syncthreads();
   const uint32_t n_items = B->n_entropy_items, n_slices = B->n_slices;
   const uint32_t lane = threadIdx.x & 63;
   if (!persistent) {
     const uint32_t item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t g = (item / 3) * 64 + lane;
     if (item < n_items)  // the same for the whole wave
-      entropy_stream(g < n_slices, g, item % 3, lines, list, &B->wide, frames, pics, slices, pic_status, coefs,
+      entropy_stream(g < n_slices, g, item % 3, lines, list, &B->wide, ftab, frames, pics, slices, pic_status, coefs,
                      coef_wide, slice_err, pic_first_err);
     return;
   }
@@ -891,7 +910,7 @@ extern "C" __global__ void __launch_bounds__(256)
     const uint32_t last = first + chunk < n_items ? first + chunk : n_items;
     for (uint32_t item = first; item < last; item++) {
       const uint32_t g = (item / 3) * 64 + lane;
-      entropy_stream(g < n_slices, g, item % 3, lines, list, &B->wide, frames, pics, slices, pic_status, coefs,
+      entropy_stream(g < n_slices, g, item % 3, lines, list, &B->wide, ftab, frames, pics, slices, pic_status, coefs,
                      coef_wide, slice_err, pic_first_err);
     }
   }

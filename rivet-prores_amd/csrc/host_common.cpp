@@ -157,10 +157,10 @@ namespace {
 struct Book {
   uint32_t lq, kr, ke;
 };
-const Book RUN_BOOKS[16] = {{2, 0, 1}, {2, 0, 1}, {1, 0, 1}, {1, 0, 1}, {0, 0, 1}, {1, 1, 2},
+constexpr Book RUN_BOOKS[16] = {{2, 0, 1}, {2, 0, 1}, {1, 0, 1}, {1, 0, 1}, {0, 0, 1}, {1, 1, 2},
                             {1, 1, 2}, {1, 1, 2}, {1, 1, 2}, {0, 1, 2}, {0, 1, 2}, {0, 1, 2},
                             {0, 1, 2}, {0, 1, 2}, {0, 1, 2}, {0, 2, 3}};
-const Book LEVEL_BOOKS[9] = {{2, 0, 2}, {1, 0, 1}, {2, 0, 1}, {0, 0, 1}, {0, 1, 2},
+constexpr Book LEVEL_BOOKS[9] = {{2, 0, 2}, {1, 0, 1}, {2, 0, 1}, {0, 0, 1}, {0, 1, 2},
                              {0, 1, 2}, {0, 1, 2}, {0, 1, 2}, {0, 2, 3}};
 uint32_t bitlen(uint64_t v) { return v ? 64 - __builtin_clzll(v) : 0; }
 // (bits, length) of symbol n.
@@ -175,26 +175,48 @@ void code_of(Book b, uint32_t n, uint64_t &bits, uint32_t &len) {
     len = b.lq + 1 + 2 * bitlen(v) - b.ke - 1;
   }
 }
-void fill_lut(const Book *books, int nb, uint16_t (*lut)[512]) {
-  for (int c = 0; c < nb; c++) {
-    memset(lut[c], 0, sizeof lut[c]);
-    for (uint32_t n = 0; n < 1024; n++) {
-      uint64_t bits;
-      uint32_t len;
-      code_of(books[c], n, bits, len);
-      if (len > 9) continue;
-      uint32_t first = (uint32_t)bits << (9 - len);
-      for (uint32_t p = 0; p < (1u << (9 - len)); p++) lut[c][first + p] = (uint16_t)((n << 5) | len);
-    }
-  }
+// The distinct codebooks of the run and level contexts (RUN_BOOKS and
+// LEVEL_BOOKS hold each context's), for the entropy decode's tables.
+constexpr Book FAST_RUN_BOOKS[6] = {{2, 0, 1}, {1, 0, 1}, {0, 0, 1}, {1, 1, 2}, {0, 1, 2}, {0, 2, 3}};
+constexpr Book FAST_LEVEL_BOOKS[6] = {{2, 0, 2}, {1, 0, 1}, {2, 0, 1}, {0, 0, 1}, {0, 1, 2}, {0, 2, 3}};
+constexpr uint8_t RUN_CTX_BOOK[16] = {0, 0, 1, 1, 2, 3, 3, 3, 3, 4, 4, 4, 4, 4, 4, 5};
+constexpr uint8_t LEVEL_CTX_BOOK[9] = {0, 1, 2, 3, 4, 4, 4, 4, 5};
+constexpr bool same_book(Book a, Book b) { return a.lq == b.lq && a.kr == b.kr && a.ke == b.ke; }
+constexpr bool fast_books_match() {
+  for (int c = 0; c < 16; c++)
+    if (RUN_CTX_BOOK[c] > 5 || !same_book(RUN_BOOKS[c], FAST_RUN_BOOKS[RUN_CTX_BOOK[c]])) return false;
+  for (int c = 0; c < 9; c++)
+    if (LEVEL_CTX_BOOK[c] > 5 || !same_book(LEVEL_BOOKS[c], FAST_LEVEL_BOOKS[LEVEL_CTX_BOOK[c]])) return false;
+  return true;
 }
-void fill_codes(const Book *books, int nb, uint32_t (*t)[32]) {
-  for (int c = 0; c < nb; c++)
-    for (uint32_t n = 0; n < 32; n++) {
+static_assert(fast_books_match(), "each context's codebook must be its distinct book");
+// Every prefix that starts a code of at most 9 bits (for levels, with its
+// sign bit) holds that code's entry (layout.h).
+void fill_fast(DevTables &t) {
+  memset(t.fast_level, 0, sizeof t.fast_level);
+  memset(t.fast_run, 0, sizeof t.fast_run);
+  for (int b = 0; b < 6; b++)
+    for (uint32_t n = 0; n < 4096; n++) {
       uint64_t bits;
       uint32_t len;
-      code_of(books[c], n, bits, len);
-      t[c][n] = ((uint32_t)bits << 5) | len;
+      code_of(FAST_RUN_BOOKS[b], n, bits, len);
+      if (len <= 9) {
+        if (n >= 64) abort();  // 6 bits of run; no code this short has more
+        const uint32_t first = (uint32_t)bits << (9 - len);
+        for (uint32_t p = 0; p < (1u << (9 - len)); p++)
+          t.fast_run[b][first + p] = (uint16_t)(FAST_VALID | RUN_CTX_BOOK[n < 15 ? n : 15] << 10 | n << 4 | len);
+      }
+      code_of(FAST_LEVEL_BOOKS[b], n, bits, len);
+      if (len <= 9) {
+        if (n + 1 > 63) abort();  // a 7-bit signed level; no code this short has more
+        for (uint32_t sgn = 0; sgn < 2; sgn++) {
+          const int32_t level = sgn ? -(int32_t)(n + 1) : (int32_t)(n + 1);
+          const uint32_t first = (uint32_t)(bits << 1 | sgn) << (9 - len);
+          for (uint32_t p = 0; p < (1u << (9 - len)); p++)
+            t.fast_level[b][first + p] =
+                (uint16_t)(FAST_VALID | LEVEL_CTX_BOOK[n < 8 ? n : 8] << 11 | ((uint32_t)level & 127) << 4 | (len + 1));
+        }
+      }
     }
 }
 const uint8_t SCAN_P[64] = {0,  1,  4,  5,  16, 17, 21, 22, 2,  3,  6,  7,  18, 20, 23, 28,
@@ -219,10 +241,7 @@ void build_tables(DevTables &t) {
       t.scan[0][8 * v + u] = SCAN_P[8 * v + u];
       t.scan[1][8 * u + v] = SCAN_P[8 * v + u];
     }
-  fill_lut(RUN_BOOKS, 16, t.run_lut);
-  fill_lut(LEVEL_BOOKS, 9, t.level_lut);
-  fill_codes(RUN_BOOKS, 16, t.run_codes);
-  fill_codes(LEVEL_BOOKS, 9, t.level_codes);
+  fill_fast(t);
 }
 
 // ---------------------------------------------------------------------------

@@ -28,11 +28,19 @@ enum SliceError : uint32_t {
 struct DevTables {
   float basis[64];              // T[x][u] at 8x+u, C(u)/2 cos((2x+1)u pi/16) rounded to f32
   uint8_t scan[2][64];          // [progressive, interlaced]: scanned index of raster position
-  uint16_t run_lut[16][512];    // (symbol << 5) | length on 9-bit prefixes, 0 = longer
-  uint16_t level_lut[9][512];
-  uint32_t run_codes[16][32];   // (bits << 5) | length, symbols 0..31
-  uint32_t level_codes[9][32];
+  // Run and level codes of at most 9 bits, which the entropy decode reads
+  // from LDS; 0 marks a longer code. Each table is indexed by the next bits
+  // of the stream and `next` is the book of the context the symbol sets.
+  // fast_level[book][next 10 bits]: a level code and its sign bit,
+  //   FAST_VALID | next << 11 | (signed level & 127) << 4 | (length + 1);
+  // fast_run[book][next 9 bits]:
+  //   FAST_VALID | next << 10 | run << 4 | length.
+  // Books are the six distinct level and six distinct run codebooks.
+  uint16_t fast_level[6][1024];
+  uint16_t fast_run[6][512];
 };
+constexpr uint32_t FAST_VALID = 0x8000;
+constexpr uint32_t FAST_RUN_BASE = 6 * 1024 * 2;  // byte offset of fast_run after fast_level
 
 // A frame to decode into (device pointers and per-frame constants).
 struct DecFrame {
@@ -111,6 +119,7 @@ constexpr uint32_t IDCT_MAX_SLICES = 12;  // slices of one item (a slice is at l
 // and a 64-entry list of 16-byte entries a wave.
 constexpr uint32_t LINE_WORDS = 36;
 constexpr uint32_t ENTROPY_LDS_PER_WAVE = 64 * LINE_WORDS * 4 + 64 * 16;
+constexpr uint32_t ENTROPY_MAX_WG = 768;  // 12 waves: 120 KB of lines and lists plus 12 KB of tables
 
 // Slices whose alpha stream is longer than this many bytes go first in the
 // alpha work order: their decode is one long serial chain.

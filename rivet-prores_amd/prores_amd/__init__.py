@@ -477,7 +477,7 @@ class DecodeTuning:
     persistent: int = 0
     chunk: int = 1
     idct_blocks: int = 96
-    entropy_wg: int = 256
+    entropy_wg: int = 0
     graph: int = 0
 
     def _c(self):

@@ -85,7 +85,7 @@ def check_batch(errs, want, bufs):
 
 
 TUNINGS = [P.DecodeTuning(pers, chunk, blocks, wg, graph)
-           for pers, chunk, blocks, wg, graph in itertools.product((0, 1), (1, 3), (8, 40, 96), (64, 256), (0, 1))
+           for pers, chunk, blocks, wg, graph in itertools.product((0, 1), (1, 3), (8, 40, 96), (0, 64, 256, 768), (0, 1))
            if pers or chunk == 1]
 
 

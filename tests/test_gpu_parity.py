@@ -168,3 +168,40 @@ def test_damaged_frames_give_the_same_error():
         for _ in range(rng.integers(1, 8)):
             b[rng.integers(0, len(b))] = rng.integers(0, 256)
         assert_decode_parity(bytes(b))
+
+
+def _component_bytes(scanned):
+    import ctypes as C
+    arr = np.ascontiguousarray(scanned, dtype=np.int32)
+    cap = 64 * arr.size + 64
+    buf = (C.c_uint8 * cap)()
+    n = O.lib().oracle_encode_coefficients(arr.ctypes.data, arr.size // 64, buf, cap)
+    assert 0 < n <= cap
+    return bytes(buf[:n])
+
+
+def test_coefficients_beyond_16_bits():
+    """Coefficients outside +-32767 (a DC of 40000, levels up to 2^29) decode
+    like the oracle: the engine keeps them beside its 16-bit coefficient
+    store. Streams are coded by the oracle's coefficient coder
+    (src/encode.rs:413-437) into one-slice 32x16 4:2:2 frames."""
+    from test_handbuilt import frame, frame_header, picture, slice_
+
+    rng = np.random.default_rng(8)
+    specials = np.array([32767, -32767, 32768, -32768, 40000, -40000, 1 << 20, -(1 << 20), (1 << 29) + 5,
+                         -(1 << 29)], dtype=np.int64)
+    for case in range(12):
+        comps = []
+        for nb in (8, 4, 4):  # two macroblocks: Y' 8 blocks, Cb and Cr 4 each
+            c = rng.integers(-40, 41, size=nb * 64).astype(np.int64)
+            c[rng.random(nb * 64) < 0.5] = 0
+            k = rng.integers(1, 6)
+            c[rng.integers(0, nb * 64, size=k)] = rng.choice(specials, size=k)
+            if case == 0:
+                c[:nb] = 40000  # every DC
+            comps.append(_component_bytes(c))
+        q = int(rng.integers(1, 225))
+        data = frame(frame_header(32, 16, 2, 0, 0, 0, 0), [picture(1, [slice_(q, *comps, 0)])], 0, 0)
+        print_case("wide", case, q)
+        assert_decode_parity(data)
+        assert_decode_parity(data, 16)
