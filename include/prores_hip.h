@@ -312,6 +312,17 @@ prores_status prores_memcpy(void *dst, const void *src, size_t bytes, int32_t ki
 /* Library build string. */
 const char *prores_version(void);
 
+/* The run/level code tables the entropy decode reads (for inspection and
+ * tests; no device work): level[6 * 1024] entries for the next 10 bits of a
+ * stream (a level code and its sign), run[6 * 512] for the next 9 bits, one
+ * block per distinct codebook, in the order the codebooks first appear
+ * among the contexts of src/vlc.rs:30-68. An entry is
+ *   level: 0x8000 | next << 11 | (signed level & 127) << 4 | (code bits + 1)
+ *   run:   0x8000 | next << 10 | run << 4 | code bits
+ * where next is the block of the context the symbol sets, or 0 when the
+ * code is longer than 9 bits. No counterpart in the reference. */
+void prores_entropy_tables(uint16_t *level, uint16_t *run);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,8 +4,10 @@
 // and placement run on the device. Errors come back per slice and are
 // reported as the reference does: the first in picture order.
 #include <stdlib.h>
+#include <string.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "host_internal.h"
 
@@ -716,5 +718,13 @@ void prores_last_batch_timing(double *us5) {
 }
 
 const char *prores_version(void) { return "prores-hip 0.1 gfx950"; }
+
+void prores_entropy_tables(uint16_t *level, uint16_t *run) {
+  static DevTables t;  // built once; the tables are constant
+  static std::once_flag once;
+  std::call_once(once, [] { build_tables(t); });
+  memcpy(level, t.fast_level, sizeof t.fast_level);
+  memcpy(run, t.fast_run, sizeof t.fast_run);
+}
 
 }  // extern "C"

@@ -151,6 +151,7 @@ EXPORTS = [
     "prores_server_config_default", "prores_server_create", "prores_server_destroy",
     "prores_server_submit_decode", "prores_server_submit_encode", "prores_server_poll",
     "prores_server_release", "prores_server_get_stats", "prores_last_encode_counters",
+    "prores_entropy_tables",
 ]
 
 _lib = None

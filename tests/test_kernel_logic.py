@@ -170,3 +170,53 @@ def test_idct_shared_products_equal_the_reference_sums():
         assert same.all(), (x, np.nonzero(~same)[0][:5])
         nz = ref != 0
         assert np.array_equal(s[nz].view(np.uint32), ref[nz].view(np.uint32)), x
+
+
+# The run and level codebooks of each context (src/vlc.rs:30-68), as
+# (lastRiceQ, kRice, kExp).
+RUN_CONTEXTS = [(2, 0, 1)] * 2 + [(1, 0, 1)] * 2 + [(0, 0, 1)] + [(1, 1, 2)] * 4 + [(0, 1, 2)] * 6 + [(0, 2, 3)]
+LEVEL_CONTEXTS = [(2, 0, 2), (1, 0, 1), (2, 0, 1), (0, 0, 1)] + [(0, 1, 2)] * 4 + [(0, 2, 3)]
+
+
+def _distinct(books):
+    out = []
+    for b in books:
+        if b not in out:
+            out.append(b)
+    return out
+
+
+def test_entropy_tables_equal_the_bit_serial_reader():
+    """pd_entropy's LDS tables (prores_entropy_tables): every prefix of the
+    next 9 (run) or 10 (level + sign) bits holds the code the bit-serial
+    reader (Codebook::read, src/vlc.rs:206-230) decodes from it, with its
+    length and the table block of the context the symbol sets, or 0 when the
+    code is longer than 9 bits."""
+    level = (C.c_uint16 * (6 * 1024))()
+    run = (C.c_uint16 * (6 * 512))()
+    import prores_amd as P
+    P.lib().prores_entropy_tables(level, run)
+    runs, levels = _distinct(RUN_CONTEXTS), _distinct(LEVEL_CONTEXTS)
+    assert len(runs) == 6 and len(levels) == 6
+    run_block = lambda sym: runs.index(RUN_CONTEXTS[min(sym, 15)])
+    level_block = lambda sym: levels.index(LEVEL_CONTEXTS[min(sym, 8)])
+    covered = 0
+    for b, (lq, kr, ke) in enumerate(runs):
+        for p in range(512):
+            r = reference_read(format(p, "09b"), lq, kr, ke)
+            want = 0 if r in (None, "range") else 0x8000 | run_block(r[0]) << 10 | r[0] << 4 | r[1]
+            assert run[b * 512 + p] == want, ("run", b, p, r)
+            covered += want != 0
+    for b, (lq, kr, ke) in enumerate(levels):
+        for p in range(1024):
+            bits = format(p, "010b")
+            r = reference_read(bits[:9], lq, kr, ke)
+            if r in (None, "range"):
+                want = 0
+            else:
+                sym, n = r
+                lvl = -(sym + 1) if bits[n] == "1" else sym + 1
+                want = 0x8000 | level_block(sym) << 11 | (lvl & 127) << 4 | (n + 1)
+            assert level[b * 1024 + p] == want, ("level", b, p, r)
+            covered += want != 0
+    assert covered > 8000
