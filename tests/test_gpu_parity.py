@@ -205,3 +205,18 @@ def test_coefficients_beyond_16_bits():
         print_case("wide", case, q)
         assert_decode_parity(data)
         assert_decode_parity(data, 16)
+
+
+def test_many_long_alpha_streams():
+    """Random alpha coded at 16 bits, 1-macroblock slices: 4,608 of the
+    5,120 slices carry long alpha streams (over 256 bytes, decoded first in
+    the alpha work order) beside short ones; samples and alpha equal the
+    oracle's (src/decode.rs:365-389, 570-595)."""
+    rng = np.random.default_rng(17)
+    f = O.test_frame(2048, 640, 3, 12, 20, 5)
+    f.alpha = rng.integers(0, 4096, size=2048 * 640).astype(np.uint16)
+    f.alpha[: 2048 * 64] = np.repeat(np.arange(0, 2048 * 64 // 16) % 4096, 16).astype(np.uint16)
+    print_case("long alpha streams", 2048, 640)
+    pkt = assert_encode_parity(f, 5, log2=0)
+    assert_decode_parity(pkt)
+    assert_decode_parity(pkt, 10)
