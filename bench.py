@@ -344,11 +344,17 @@ def leg_serving(P, synth, seconds, rate, check):
                     time.sleep(0)
                     with lock:
                         k = id_to_req.get(r.id)
-                if k in sample and r.error is None:
-                    sampled[k] = r.samples().copy() if r.kind == "decode" else r.packet()
-                    if r.kind == "decode" and r.dev_alpha:
-                        sampled[k] = (sampled[k], r.alpha().copy())
-                results[k] = (r.kind, r.latency_us, r.batch_frames, r.error)
+                err = r.error
+                if k in sample and err is None:
+                    try:
+                        sampled[k] = r.samples().copy() if r.kind == "decode" else r.packet()
+                        if r.kind == "decode" and r.dev_alpha:
+                            sampled[k] = (sampled[k], r.alpha().copy())
+                    except P.DeviceError as e:  # counted as a failed request; the leg goes on
+                        log(f"bench: serving: reading result {r.id} ({r.kind}): {e}")
+                        sampled.pop(k, None)
+                        err = e
+                results[k] = (r.kind, r.latency_us, r.batch_frames, err)
                 srv.release(r.id)
             if stop.is_set() and len(results) >= n_req:
                 break

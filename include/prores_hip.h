@@ -308,6 +308,8 @@ void prores_server_get_stats(prores_server *s, prores_server_stats *out);
 void *prores_device_alloc(size_t bytes);
 void prores_device_free(void *p);
 prores_status prores_memcpy(void *dst, const void *src, size_t bytes, int32_t kind);
+/* The HIP error of this thread's last failed prores_memcpy. */
+const char *prores_memcpy_error(void);
 
 /* Library build string. */
 const char *prores_version(void);

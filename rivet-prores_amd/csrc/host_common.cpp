@@ -375,11 +375,19 @@ void prores_device_free(void *p) {
   if (p) (void)hipFree(p);
 }
 
+static thread_local char last_memcpy_error[160];
+
 prores_status prores_memcpy(void *dst, const void *src, size_t bytes, int32_t kind) {
   const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
                           : kind == 2 ? hipMemcpyDeviceToHost
                                       : hipMemcpyDeviceToDevice;
-  return hipMemcpy(dst, src, bytes, k) == hipSuccess ? PRORES_OK : PRORES_ERR_DEVICE;
+  const hipError_t e = hipMemcpy(dst, src, bytes, k);
+  if (e == hipSuccess) return PRORES_OK;
+  snprintf(last_memcpy_error, sizeof last_memcpy_error, "hipMemcpy of %zu bytes: %s (%d)", bytes,
+           hipGetErrorString(e), (int)e);
+  return PRORES_ERR_DEVICE;
 }
+
+const char *prores_memcpy_error(void) { return last_memcpy_error; }
 
 }  // extern "C"

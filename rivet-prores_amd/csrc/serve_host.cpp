@@ -14,6 +14,7 @@
 #include <condition_variable>
 #include <deque>
 #include <map>
+#include <set>
 #include <thread>
 
 #include "host_internal.h"
@@ -330,11 +331,13 @@ void prores_server_destroy(prores_server *s) {
   s->cv_done.notify_all();
   for (auto &w : s->workers)
     if (w.joinable()) w.join();
-  for (auto &q : s->pend)
-    for (Request *r : q) delete r;
-  for (auto &kv : s->live) delete kv.second;
-  for (Request *r : s->done)
-    if (!s->live.count(r->id)) delete r;
+  // Every request once: the queued ones, the admitted ones (live, polled or
+  // not) and the ones completed at submit without admission (done only).
+  std::set<Request *> all;
+  for (auto &q : s->pend) all.insert(q.begin(), q.end());
+  for (auto &kv : s->live) all.insert(kv.second);
+  all.insert(s->done.begin(), s->done.end());
+  for (Request *r : all) delete r;
   prores_decoder_destroy(s->decoder);
   delete s;
 }

@@ -151,7 +151,7 @@ EXPORTS = [
     "prores_server_config_default", "prores_server_create", "prores_server_destroy",
     "prores_server_submit_decode", "prores_server_submit_encode", "prores_server_poll",
     "prores_server_release", "prores_server_get_stats", "prores_last_encode_counters",
-    "prores_entropy_tables",
+    "prores_entropy_tables", "prores_memcpy_error",
 ]
 
 _lib = None
@@ -205,6 +205,7 @@ def lib():
         L.prores_last_encode_timing.argtypes = [P(C.c_double)]
         L.prores_last_encode_counters.argtypes = [P(C.c_uint64)]
         L.prores_version.restype = C.c_char_p
+        L.prores_memcpy_error.restype = C.c_char_p
         L.prores_device_alloc.argtypes = [C.c_size_t]
         L.prores_device_alloc.restype = C.c_void_p
         L.prores_device_free.argtypes = [C.c_void_p]
@@ -750,13 +751,13 @@ class DeviceBuffer:
         a = np.ascontiguousarray(arr)
         assert offset + a.nbytes <= self.nbytes
         if lib().prores_memcpy(self.ptr + offset, a.ctypes.data, a.nbytes, 1):
-            raise DeviceError("host-to-device copy failed")
+            raise DeviceError("host-to-device copy failed: " + lib().prores_memcpy_error().decode())
 
     def download(self, nbytes=None, offset=0, dtype=np.uint8):
         n = self.nbytes - offset if nbytes is None else nbytes
         out = np.empty(n // np.dtype(dtype).itemsize, dtype=dtype)
         if n and lib().prores_memcpy(out.ctypes.data, self.ptr + offset, n, 2):
-            raise DeviceError("device-to-host copy failed")
+            raise DeviceError("device-to-host copy failed: " + lib().prores_memcpy_error().decode())
         return out
 
     def free(self):
@@ -770,11 +771,11 @@ class DeviceBuffer:
 
 def copy_device(dst_ptr, src_ptr, nbytes):
     if nbytes and lib().prores_memcpy(dst_ptr, src_ptr, nbytes, 3):
-        raise DeviceError("device-to-device copy failed")
+        raise DeviceError("device-to-device copy failed: " + lib().prores_memcpy_error().decode())
 
 
 def download_ptr(src_ptr, nbytes):
     out = np.empty(nbytes, dtype=np.uint8)
     if nbytes and lib().prores_memcpy(out.ctypes.data, src_ptr, nbytes, 2):
-        raise DeviceError("device-to-host copy failed")
+        raise DeviceError("device-to-host copy failed: " + lib().prores_memcpy_error().decode())
     return out

@@ -158,3 +158,21 @@ def test_requests_that_cannot_run_fail_at_once(requests_):
             assert got[refused].error.msg == e.msg
         for rid in got:
             srv.release(rid)
+
+
+def test_close_with_results_not_polled(requests_):
+    """Closing the server while results wait to be polled, some polled and
+    not released, and one refused at submit frees every request once."""
+    for rep in range(3):
+        print("case: close with pending results", rep, flush=True)
+        srv = P.Server()
+        ids = [srv.submit_decode(x) if kind == "decode" else srv.submit_encode(*x) for kind, x, want in requests_]
+        ids.append(srv.submit_decode(b"\x00\x00\x00\x08icpf"))  # refused at once
+        deadline = time.time() + 60
+        while srv.stats()["completed"] < len(ids) and time.time() < deadline:
+            time.sleep(0.01)
+        assert srv.stats()["completed"] == len(ids)
+        if rep == 1:
+            got = srv.poll(2, 0)  # polled, never released
+            assert len(got) == 2
+        srv.close()
