@@ -130,6 +130,19 @@ prores_status prores_decode(const prores_decoder *d, const uint8_t *data, size_t
 prores_status prores_decode_into(const prores_decoder *d, const uint8_t *data, size_t len,
                                  prores_frame *out, prores_error *err);
 
+/* Many frames from host memory (Decoder::decode on each of n packets): out[i]
+ * and errs[i] as prores_decode gives them; a frame in error leaves out[i]
+ * empty. The frames go through the GPU in chunks on two streams, so the
+ * uploads and downloads of one chunk overlap the kernels of the next; the
+ * library's frames live in page-locked memory, so the downloads run at the
+ * link's speed. Returns PRORES_OK, or the device error that stopped the
+ * call (then frames not decoded carry it). No counterpart in the reference
+ * beyond Decoder::decode. */
+prores_status prores_decode_host_batch(const prores_decoder *d, size_t n, const uint8_t *const *data,
+                                       const size_t *len, prores_frame *out, prores_error *errs);
+
+/* Frees what prores_decode / prores_decode_host_batch allocated (their
+ * planes sit in a pool of page-locked buffers kept for reuse). */
 void prores_frame_free(prores_frame *f);
 
 /* FrameHeader::parse (src/header.rs:82-170). */

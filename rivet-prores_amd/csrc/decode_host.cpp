@@ -7,7 +7,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <thread>
 
 #include "host_internal.h"
 
@@ -439,6 +441,102 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   return Err{};
 }
 
+// Host memory of the frames the library allocates: page-locked
+// (hipHostMalloc), so device-to-host copies run at the link's speed with no
+// staging, and kept for reuse after prores_frame_free (a fresh allocation of
+// a frame's size costs page faults, or pinning, every time). Up to KEEP
+// bytes wait for reuse; beyond that buffers go back to the system. Never
+// destroyed: frames may be freed while the process exits.
+class PinnedPool {
+  std::mutex mu;
+  std::multimap<size_t, void *> idle;  // capacity -> buffer
+  std::map<void *, size_t> cap;        // every pinned buffer handed out or idle
+  size_t idle_bytes = 0;
+
+ public:
+  static constexpr size_t KEEP = 2ull << 30;
+  // A buffer of at least n bytes, or nullptr when none can be pinned.
+  void *get(size_t n) {
+    const size_t c = (n + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+    {
+      std::lock_guard<std::mutex> g(mu);
+      auto it = idle.lower_bound(c);
+      if (it != idle.end() && it->first <= c + c / 4) {
+        void *p = it->second;
+        idle_bytes -= it->first;
+        idle.erase(it);
+        return p;
+      }
+    }
+    void *p = nullptr;
+    if (hipHostMalloc(&p, c, hipHostMallocDefault) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    cap[p] = c;
+    return p;
+  }
+  // Takes back a buffer of this pool (true), or leaves p alone (false).
+  bool put(void *p) {
+    if (!p) return true;
+    std::vector<void *> release;
+    {
+      std::lock_guard<std::mutex> g(mu);
+      auto it = cap.find(p);
+      if (it == cap.end()) return false;
+      idle.emplace(it->second, p);
+      idle_bytes += it->second;
+      while (idle_bytes > KEEP) {
+        auto big = std::prev(idle.end());
+        idle_bytes -= big->first;
+        cap.erase(big->second);
+        release.push_back(big->second);
+        idle.erase(big);
+      }
+    }
+    for (void *q : release) (void)hipHostFree(q);
+    return true;
+  }
+};
+PinnedPool &pinned() {
+  static PinnedPool *p = new PinnedPool();
+  return *p;
+}
+
+// A decoded frame's fields from its header, and its planes allocated from
+// the pinned pool (malloc when none can be pinned).
+Err frame_outputs(const prores_decoder *d, const Header &hd, prores_frame *out, bool allocate, size_t &total,
+                  size_t &alpha_n) {
+  const prores_frame_header &h = hd.h;
+  const uint32_t bd = d->bit_depth ? d->bit_depth : (h.chroma == 2 ? 10 : 12);
+  prores_plane planes[3];
+  total = prores_frame_layout(h.width, h.height, h.chroma, planes);
+  alpha_n = h.alpha ? (size_t)h.width * h.height : 0;
+  if (allocate) {
+    memset(out, 0, sizeof *out);
+    out->data = (uint16_t *)pinned().get(total * 2 + 2);
+    if (!out->data) out->data = (uint16_t *)malloc(total * 2 + 2);
+    if (alpha_n) {
+      out->alpha = (uint16_t *)pinned().get(alpha_n * 2 + 2);
+      if (!out->alpha) out->alpha = (uint16_t *)malloc(alpha_n * 2 + 2);
+    }
+    if (!out->data || (alpha_n && !out->alpha)) {
+      prores_frame_free(out);
+      return Err{PRORES_ERR_DEVICE, "host allocation failed"};
+    }
+  } else if (!out->data || out->data_len < total || (alpha_n && (!out->alpha || out->alpha_len < alpha_n))) {
+    return config_error("the output buffers are smaller than the frame");
+  }
+  out->width = h.width;
+  out->height = h.height;
+  out->bit_depth = bd;
+  out->chroma = h.chroma;
+  out->interlace = h.interlace;
+  out->metadata = h.metadata;
+  out->data_len = total;
+  out->alpha_len = alpha_n;
+  for (int i = 0; i < 3; i++) out->planes[i] = planes[i];
+  return Err{};
+}
+
 }  // namespace
 }  // namespace prores_hip
 
@@ -481,8 +579,8 @@ prores_status prores_frame_header_parse(const uint8_t *data, size_t len, prores_
 
 void prores_frame_free(prores_frame *f) {
   if (!f) return;
-  free(f->data);
-  free(f->alpha);
+  if (!pinned().put(f->data)) free(f->data);
+  if (!pinned().put(f->alpha)) free(f->alpha);
   f->data = nullptr;
   f->alpha = nullptr;
 }
@@ -492,47 +590,22 @@ static prores_status decode_common(const prores_decoder *d, const uint8_t *data,
   Header hd;
   Err e = parse_frame_header(data, len, hd);
   if (e.ok()) e = check_plausible_size(hd);
+  size_t total = 0, alpha_n = 0;
+  if (e.ok()) e = frame_outputs(d, hd, out, allocate, total, alpha_n);
   if (!e.ok()) {
     fill_error(err, e);
     return (prores_status)e.kind;
   }
-  const prores_frame_header &h = hd.h;
-  const uint32_t bd = d->bit_depth ? d->bit_depth : (h.chroma == 2 ? 10 : 12);
-  prores_plane planes[3];
-  const size_t total = prores_frame_layout(h.width, h.height, h.chroma, planes);
-  const size_t alpha_n = h.alpha ? (size_t)h.width * h.height : 0;
-  if (allocate) {
-    memset(out, 0, sizeof *out);
-    out->data = (uint16_t *)malloc(total * 2 + 2);
-    out->alpha = alpha_n ? (uint16_t *)malloc(alpha_n * 2 + 2) : nullptr;
-    if (!out->data || (alpha_n && !out->alpha)) {
-      prores_frame_free(out);
-      fill_error(err, Err{PRORES_ERR_DEVICE, "host allocation failed"});
-      return PRORES_ERR_DEVICE;
-    }
-  } else if (!out->data || out->data_len < total || (alpha_n && (!out->alpha || out->alpha_len < alpha_n))) {
-    fill_error(err, config_error("the output buffers are smaller than the frame"));
-    return PRORES_ERR_CONFIG;
-  }
-  out->width = h.width;
-  out->height = h.height;
-  out->bit_depth = bd;
-  out->chroma = h.chroma;
-  out->interlace = h.interlace;
-  out->metadata = h.metadata;
-  out->data_len = total;
-  out->alpha_len = alpha_n;
-  for (int i = 0; i < 3; i++) out->planes[i] = planes[i];
-
   Device *dev = nullptr;
   Lane *lane0 = nullptr;
   if (!(e = device(dev)).ok() || !(e = lane(*dev, 0, lane0)).ok()) {
+    if (allocate) prores_frame_free(out);
     fill_error(err, e);
     return (prores_status)e.kind;
   }
   std::lock_guard<std::mutex> g(lane0->mu);
   Workspace &ws = workspace(*lane0);
-  const size_t fsz = h.frame_size;
+  const size_t fsz = hd.h.frame_size;
   const size_t plane_bytes = (total * 2 + 255) & ~(size_t)255;
   if ((e = ws.packet.reserve(fsz + 128)).ok() && (e = ws.planes.reserve(plane_bytes + alpha_n * 2 + 64)).ok() &&
       (e = check(hipMemcpyAsync(ws.packet.p, data, fsz, hipMemcpyHostToDevice, lane0->stream), "H2D packet")).ok()) {
@@ -546,8 +619,12 @@ static prores_status decode_common(const prores_decoder *d, const uint8_t *data,
     e = run_batch(*dev, *lane0, b, t, results);
     if (e.ok()) e = results[0];
     if (e.ok()) {
-      e = check(hipMemcpy(out->data, dd, total * 2, hipMemcpyDeviceToHost), "D2H planes");
-      if (e.ok() && alpha_n) e = check(hipMemcpy(out->alpha, da, alpha_n * 2, hipMemcpyDeviceToHost), "D2H alpha");
+      // Into pinned memory (prores_decode) this runs at the link's speed.
+      e = check(hipMemcpyAsync(out->data, dd, total * 2, hipMemcpyDeviceToHost, lane0->stream), "D2H planes");
+      if (e.ok() && alpha_n)
+        e = check(hipMemcpyAsync(out->alpha, da, alpha_n * 2, hipMemcpyDeviceToHost, lane0->stream), "D2H alpha");
+      Err s2 = check(hipStreamSynchronize(lane0->stream), "hipStreamSynchronize");
+      if (e.ok()) e = s2;
     }
   }
   if (!e.ok() && allocate) prores_frame_free(out);
@@ -609,6 +686,131 @@ static prores_status batch_common(const prores_decoder *d, size_t n, const prore
   std::lock_guard<std::mutex> g(lane0->mu);
   e = decode_batch_on(*dev, *lane0, d->bit_depth, n, jobs, tuning, errs);
   return e.ok() ? PRORES_OK : (prores_status)e.kind;
+}
+
+// Host packets in, host frames out, many at a time: the frames are cut into
+// chunks (up to HOST_CHUNK_BYTES of planes or HOST_CHUNK_FRAMES frames); two
+// lanes, each driven by its own thread, take alternate chunks through
+// upload, one batched launch sequence and download, so one chunk's copies
+// overlap the other's kernels (and uploads overlap downloads: the link
+// carries both directions at once).
+static constexpr size_t HOST_CHUNK_BYTES = 256ull << 20, HOST_CHUNK_FRAMES = 16;
+static constexpr int HOST_LANES[2] = {3, 4};
+
+prores_status prores_decode_host_batch(const prores_decoder *d, size_t n, const uint8_t *const *data,
+                                       const size_t *len, prores_frame *out, prores_error *errs) {
+  struct Item {
+    Header hd;
+    size_t total = 0, alpha_n = 0;
+    Err e;
+  };
+  std::vector<Item> items(n);
+  for (size_t i = 0; i < n; i++) {
+    Item &it = items[i];
+    it.e = parse_frame_header(data[i], len[i], it.hd);
+    if (it.e.ok()) it.e = check_plausible_size(it.hd);
+    if (it.e.ok()) it.e = frame_outputs(d, it.hd, &out[i], true, it.total, it.alpha_n);
+    if (!it.e.ok()) memset(&out[i], 0, sizeof out[i]);
+  }
+  std::vector<std::pair<size_t, size_t>> chunks;
+  for (size_t i = 0; i < n;) {
+    size_t j = i, bytes = 0;
+    while (j < n && j - i < HOST_CHUNK_FRAMES && (j == i || bytes < HOST_CHUNK_BYTES)) {
+      bytes += 2 * (items[j].total + items[j].alpha_n);
+      j++;
+    }
+    chunks.push_back({i, j});
+    i = j;
+  }
+  Device *dev = nullptr;
+  Err de = device(dev);
+  prores_decode_tuning t;
+  prores_decode_tuning_default(&t);
+  if (de.ok()) {
+    std::lock_guard<std::mutex> g(dev->mu);
+    t = dev->tuning;
+  }
+  auto work = [&](int which, Err &fail) {
+    Lane *L = nullptr;
+    if (!(fail = lane(*dev, HOST_LANES[which], L)).ok()) return;
+    std::lock_guard<std::mutex> g(L->mu);
+    Workspace &ws = workspace(*L);
+    for (size_t c = (size_t)which; c < chunks.size(); c += 2) {
+      const size_t a = chunks[c].first, b = chunks[c].second;
+      // Arena offsets: each packet with 64 readable bytes past it, planes
+      // then alpha, 256-byte aligned.
+      std::vector<size_t> poff(b - a), qoff(b - a), aoff(b - a);
+      size_t pbytes = 0, qbytes = 0;
+      for (size_t i = a; i < b; i++) {
+        if (!items[i].e.ok()) continue;
+        poff[i - a] = pbytes;
+        pbytes += (items[i].hd.h.frame_size + 128 + 255) & ~(size_t)255;
+        qoff[i - a] = qbytes;
+        qbytes += (items[i].total * 2 + 255) & ~(size_t)255;
+        aoff[i - a] = qbytes;
+        qbytes += (items[i].alpha_n * 2 + 255) & ~(size_t)255;
+      }
+      if (!pbytes) continue;
+      Err e;
+      if (!(e = ws.packet.reserve(pbytes + 64)).ok() || !(e = ws.planes.reserve(qbytes + 64)).ok()) {
+        fail = e;
+        return;
+      }
+      Batch bt;
+      std::vector<size_t> idx;
+      for (size_t i = a; i < b && e.ok(); i++) {
+        if (!items[i].e.ok()) continue;
+        uint8_t *dp = (uint8_t *)ws.packet.p + poff[i - a];
+        uint16_t *dd = (uint16_t *)((uint8_t *)ws.planes.p + qoff[i - a]);
+        uint16_t *da = items[i].alpha_n ? (uint16_t *)((uint8_t *)ws.planes.p + aoff[i - a]) : nullptr;
+        e = check(hipMemcpyAsync(dp, data[i], items[i].hd.h.frame_size, hipMemcpyHostToDevice, L->stream),
+                  "H2D packet");
+        plan_frame(data[i], len[i], (uint64_t)dp, d->bit_depth, dd, da, bt, &items[i].hd);
+        idx.push_back(i);
+      }
+      std::vector<Err> results;
+      if (e.ok()) e = run_batch(*dev, *L, bt, t, results);
+      if (!e.ok()) {
+        fail = e;
+        return;
+      }
+      for (size_t k = 0; k < idx.size() && e.ok(); k++) {
+        const size_t i = idx[k];
+        items[i].e = results[k];
+        if (!results[k].ok()) continue;
+        e = check(hipMemcpyAsync(out[i].data, (uint8_t *)ws.planes.p + qoff[i - a], items[i].total * 2,
+                                 hipMemcpyDeviceToHost, L->stream),
+                  "D2H planes");
+        if (e.ok() && items[i].alpha_n)
+          e = check(hipMemcpyAsync(out[i].alpha, (uint8_t *)ws.planes.p + aoff[i - a], items[i].alpha_n * 2,
+                                   hipMemcpyDeviceToHost, L->stream),
+                    "D2H alpha");
+      }
+      Err s2 = check(hipStreamSynchronize(L->stream), "hipStreamSynchronize");
+      if (e.ok()) e = s2;
+      if (!e.ok()) {
+        fail = e;
+        return;
+      }
+    }
+  };
+  Err fails[2];
+  if (de.ok() && !chunks.empty()) {
+    std::thread other;
+    if (chunks.size() > 1) other = std::thread(work, 1, std::ref(fails[1]));
+    work(0, fails[0]);
+    if (other.joinable()) other.join();
+  }
+  Err call = !de.ok() ? de : !fails[0].ok() ? fails[0] : fails[1];
+  for (size_t i = 0; i < n; i++) {
+    Err e = call.ok() ? items[i].e : (items[i].e.ok() ? call : items[i].e);
+    if (!e.ok() && (out[i].data || out[i].alpha)) {
+      prores_frame_free(&out[i]);
+      memset(&out[i], 0, sizeof out[i]);
+    }
+    fill_error(&errs[i], e);
+  }
+  return (prores_status)call.kind;
 }
 
 prores_status prores_decode_device_batch(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,

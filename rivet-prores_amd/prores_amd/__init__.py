@@ -12,6 +12,7 @@ import ctypes as C
 import dataclasses
 import enum
 import os
+import weakref
 
 import numpy as np
 
@@ -151,7 +152,7 @@ EXPORTS = [
     "prores_server_config_default", "prores_server_create", "prores_server_destroy",
     "prores_server_submit_decode", "prores_server_submit_encode", "prores_server_poll",
     "prores_server_release", "prores_server_get_stats", "prores_last_encode_counters",
-    "prores_entropy_tables", "prores_memcpy_error",
+    "prores_entropy_tables", "prores_memcpy_error", "prores_decode_host_batch",
 ]
 
 _lib = None
@@ -170,6 +171,8 @@ def lib():
         L.prores_decode.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, P(_Frame), P(_Err)]
         L.prores_decode_into.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, P(_Frame), P(_Err)]
         L.prores_frame_free.argtypes = [P(_Frame)]
+        L.prores_decode_host_batch.argtypes = [C.c_void_p, C.c_size_t, P(C.c_char_p), P(C.c_size_t), P(_Frame),
+                                               P(_Err)]
         L.prores_frame_header_parse.argtypes = [C.c_char_p, C.c_size_t, P(_Header), P(_Err)]
         L.prores_frame_layout.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, P(_Plane)]
         L.prores_frame_layout.restype = C.c_size_t
@@ -383,6 +386,39 @@ class FrameHeader:
 # Decoder (src/decode.rs:19-104).
 
 
+class _FrameMemory:
+    """Owns the planes of a frame the library allocated: freed with
+    prores_frame_free once no array over them remains."""
+
+    def __init__(self, f):
+        self.f = f
+        weakref.finalize(self, _free_frame, f)
+
+
+def _free_frame(f):
+    if _lib is not None:
+        _lib.prores_frame_free(C.byref(f))
+
+
+class _PlaneView:
+    """An array interface over library memory that keeps its owner alive."""
+
+    def __init__(self, owner, ptr, n):
+        self.owner = owner
+        self.__array_interface__ = {"data": (ptr, False), "shape": (n,), "typestr": "<u2", "version": 3}
+
+
+def _owned_frame(f):
+    """A Frame over a library-allocated prores_frame, without copying."""
+    owner = _FrameMemory(f)
+    data = np.asarray(_PlaneView(owner, f.data, f.data_len))
+    alpha = np.asarray(_PlaneView(owner, f.alpha, f.alpha_len)) if f.alpha else None
+    m = f.metadata
+    return Frame(f.width, f.height, f.chroma, f.bit_depth, data, alpha, f.interlace,
+                 Metadata(m.aspect_ratio, m.frame_rate_code, m.color_primaries, m.transfer_characteristic,
+                          m.matrix_coefficients))
+
+
 class Decoder:
     def __init__(self, bit_depth=0, threads=0):
         err = _Err()
@@ -416,17 +452,29 @@ class Decoder:
         err = _Err()
         lib().prores_decode(self._h, data, len(data), C.byref(f), C.byref(err))
         _raise(err)
-        try:
-            arr = np.ctypeslib.as_array(C.cast(f.data, C.POINTER(C.c_uint16)), shape=(f.data_len,)).copy()
-            alpha = None
-            if f.alpha:
-                alpha = np.ctypeslib.as_array(C.cast(f.alpha, C.POINTER(C.c_uint16)), shape=(f.alpha_len,)).copy()
-        finally:
-            lib().prores_frame_free(C.byref(f))
-        m = f.metadata
-        return Frame(f.width, f.height, f.chroma, f.bit_depth, arr, alpha, f.interlace,
-                     Metadata(m.aspect_ratio, m.frame_rate_code, m.color_primaries, m.transfer_characteristic,
-                              m.matrix_coefficients))
+        return _owned_frame(f)
+
+    def decode_batch(self, packets):
+        """Decodes many frames from host memory (prores_decode_host_batch):
+        one Frame or one Error per packet, in order."""
+        packets = [bytes(p) for p in packets]
+        n = len(packets)
+        ptrs = (C.c_char_p * n)(*packets)
+        lens = (C.c_size_t * n)(*[len(p) for p in packets])
+        frames = (_Frame * n)()
+        errs = (_Err * n)()
+        rc = lib().prores_decode_host_batch(self._h, n, ptrs, lens, frames, errs)
+        out = []
+        for i in range(n):
+            if errs[i].kind == 0:
+                f = _Frame()
+                C.pointer(f)[0] = frames[i]
+                out.append(_owned_frame(f))
+            else:
+                out.append(_KINDS.get(errs[i].kind, Error)(errs[i].msg.decode()))
+        if rc == 4 and all(isinstance(o, Error) for o in out):
+            raise out[0]
+        return out
 
     def decode_device_batch(self, jobs, tuning=None):
         """Decodes device-resident packets: ``jobs`` is a DecodeJobs or a list

@@ -220,3 +220,36 @@ def test_many_long_alpha_streams():
     pkt = assert_encode_parity(f, 5, log2=0)
     assert_decode_parity(pkt)
     assert_decode_parity(pkt, 10)
+
+
+def test_host_batch_equals_the_oracle():
+    """prores_decode_host_batch: many host packets (profiles, sizes, fields,
+    alpha, damaged ones) decoded in overlapped chunks give each packet's
+    oracle Result, in order; frames own their planes."""
+    cases = []
+    specs = [(1920, 1080, 2, 10, 3, 3, 0, 0), (333, 75, 3, 12, 5, 1, 1, 2), (720, 486, 2, 10, 0, 2, 1, 0),
+             (512, 288, 3, 12, 4, 0, 0, 1), (3840, 2160, 3, 12, 5, 3, 0, 2)]
+    for i, (w, h, ch, bd, prof, log2, il, atype) in enumerate(specs):
+        f = O.test_frame(w, h, ch, bd, 10 + i, 50 + i)
+        f.interlace = il
+        if atype:
+            f.alpha = ((np.arange(w * h) * (7 + i)) % (1 << bd)).astype(np.uint16)
+        cases.append(O.encode(f, O.config(prof, log2_slice_mbs=log2, alpha=atype or 2)))
+    bad = bytearray(cases[0])
+    bad[len(bad) // 2:len(bad) // 2 + 40] = b"\x5a" * 40
+    packets = cases + [bytes(bad), b"\x00\x00\x00\x10icpf" + b"\x00" * 8] + cases[:2]
+    print_case("host batch", len(packets))
+    got = P.Decoder().decode_batch(packets)
+    assert len(got) == len(packets)
+    for k, (pkt, g) in enumerate(zip(packets, got)):
+        want = O.decode_result(pkt)
+        if want[0] != "Ok":
+            assert isinstance(g, P.Error) and (g.kind, g.msg) == tuple(want), (k, g, want)
+            continue
+        w = want[1]
+        assert not isinstance(g, Exception), (k, g)
+        np.testing.assert_array_equal(g.data, w.data, err_msg=f"packet {k}")
+        if w.alpha is None:
+            assert g.alpha is None
+        else:
+            np.testing.assert_array_equal(g.alpha, w.alpha, err_msg=f"packet {k} alpha")
