@@ -321,6 +321,18 @@ void prores_server_get_stats(prores_server *s, prores_server_stats *out);
 void *prores_device_alloc(size_t bytes);
 void prores_device_free(void *p);
 prores_status prores_memcpy(void *dst, const void *src, size_t bytes, int32_t kind);
+/* The whole frames of a MOV file (or a prefix of one), found by their
+ * 'icpf' identifier and frame_size without parsing the container, as the
+ * reference's sample test and bench do (tests/sample.rs:26-41,
+ * examples/prores_bench.rs:104-122): returns how many there are and stores
+ * the first `max` byte offsets and sizes. */
+size_t prores_find_frames(const uint8_t *data, size_t len, uint64_t *offsets, uint64_t *sizes, size_t max);
+
+/* FNV-1a 64 of `len` bytes from state h (start: 0xcbf29ce484222325), the
+ * hash the reference's bench prints for packets and pictures
+ * (examples/prores_bench.rs:96-101). */
+uint64_t prores_fnv1a(uint64_t h, const void *data, size_t len);
+
 /* The HIP error of this thread's last failed prores_memcpy. */
 const char *prores_memcpy_error(void);
 

@@ -375,6 +375,35 @@ void prores_device_free(void *p) {
   if (p) (void)hipFree(p);
 }
 
+size_t prores_find_frames(const uint8_t *data, size_t len, uint64_t *offsets, uint64_t *sizes, size_t max) {
+  // tests/sample.rs:26-41: a frame is its BE u32 frame_size, 'icpf', at
+  // least 28 bytes and wholly inside the data; the scan goes on after it.
+  size_t count = 0, i = 4;
+  while (i + 4 <= len) {
+    if (data[i] == 'i' && data[i + 1] == 'c' && data[i + 2] == 'p' && data[i + 3] == 'f') {
+      const size_t size = ((size_t)data[i - 4] << 24) | ((size_t)data[i - 3] << 16) | ((size_t)data[i - 2] << 8) |
+                          data[i - 1];
+      if (size >= 28 && size <= len - (i - 4)) {
+        if (count < max) {
+          offsets[count] = i - 4;
+          sizes[count] = size;
+        }
+        count++;
+        i += size - 4;
+        continue;
+      }
+    }
+    i++;
+  }
+  return count;
+}
+
+uint64_t prores_fnv1a(uint64_t h, const void *data, size_t len) {
+  const uint8_t *p = (const uint8_t *)data;
+  for (size_t i = 0; i < len; i++) h = (h ^ p[i]) * 0x100000001b3ull;
+  return h;
+}
+
 static thread_local char last_memcpy_error[160];
 
 prores_status prores_memcpy(void *dst, const void *src, size_t bytes, int32_t kind) {

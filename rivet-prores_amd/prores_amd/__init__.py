@@ -152,7 +152,8 @@ EXPORTS = [
     "prores_server_config_default", "prores_server_create", "prores_server_destroy",
     "prores_server_submit_decode", "prores_server_submit_encode", "prores_server_poll",
     "prores_server_release", "prores_server_get_stats", "prores_last_encode_counters",
-    "prores_entropy_tables", "prores_memcpy_error", "prores_decode_host_batch",
+    "prores_entropy_tables", "prores_memcpy_error", "prores_decode_host_batch", "prores_find_frames",
+    "prores_fnv1a",
 ]
 
 _lib = None
@@ -208,6 +209,10 @@ def lib():
         L.prores_last_encode_timing.argtypes = [P(C.c_double)]
         L.prores_last_encode_counters.argtypes = [P(C.c_uint64)]
         L.prores_version.restype = C.c_char_p
+        L.prores_find_frames.argtypes = [C.c_char_p, C.c_size_t, P(C.c_uint64), P(C.c_uint64), C.c_size_t]
+        L.prores_find_frames.restype = C.c_size_t
+        L.prores_fnv1a.argtypes = [C.c_uint64, C.c_void_p, C.c_size_t]
+        L.prores_fnv1a.restype = C.c_uint64
         L.prores_memcpy_error.restype = C.c_char_p
         L.prores_device_alloc.argtypes = [C.c_size_t]
         L.prores_device_alloc.restype = C.c_void_p
@@ -780,6 +785,41 @@ class Server:
 
 def version():
     return lib().prores_version().decode()
+
+
+def find_frames(data):
+    """(offset, size) of every whole frame in a MOV file's bytes, found by
+    'icpf' and frame_size (prores_find_frames; tests/sample.rs:26-41)."""
+    data = bytes(data)
+    n = lib().prores_find_frames(data, len(data), None, None, 0)
+    offs, sizes = (C.c_uint64 * max(n, 1))(), (C.c_uint64 * max(n, 1))()
+    lib().prores_find_frames(data, len(data), offs, sizes, n)
+    return [(int(offs[i]), int(sizes[i])) for i in range(n)]
+
+
+def fnv1a64(h, b):
+    """FNV-1a 64 over bytes (or an array's bytes), from state h
+    (prores_fnv1a; examples/prores_bench.rs:96-101)."""
+    a = np.ascontiguousarray(np.frombuffer(b, dtype=np.uint8) if isinstance(b, (bytes, bytearray)) else b)
+    return int(lib().prores_fnv1a(h, a.ctypes.data, a.nbytes))
+
+
+def decode_file(path_or_bytes, decoder=None):
+    """Decodes every frame of a MOV file on the GPU, many a call
+    (prores_decode_host_batch): returns (frame count, FNV-1a of every
+    picture's samples as little-endian u16), as the reference's bench reports
+    for a file (examples/prores_bench.rs:104-122)."""
+    data = path_or_bytes if isinstance(path_or_bytes, (bytes, bytearray)) else open(path_or_bytes, "rb").read()
+    dec = decoder or Decoder()
+    frames = find_frames(data)
+    h = 0xCBF29CE484222325
+    for k in range(0, len(frames), 16):
+        part = [data[o:o + n] for o, n in frames[k:k + 16]]
+        for f in dec.decode_batch(part):
+            if isinstance(f, Exception):
+                raise f
+            h = fnv1a64(h, f.data.astype("<u2", copy=False))
+    return len(frames), h
 
 
 # ---------------------------------------------------------------------------

@@ -156,3 +156,36 @@ def test_server_config_is_checked_before_the_device():
         P.Server(hbm_budget=1000)
     with pytest.raises(P.ConfigError):
         P.Server(max_batch_frames=0)
+
+
+def test_find_frames_in_a_mov_like_file():
+    """prores_find_frames (tests/sample.rs:26-41): packets embedded in
+    container bytes are found at their offsets, whole; 'icpf' with a
+    frame_size under 28 or past the data is passed over; a truncated last
+    frame is left out."""
+    rng = np.random.default_rng(3)
+    pk = [O.encode(O.test_frame(w, h, 2, 10, 8, s), O.config(p, log2_slice_mbs=1))
+          for w, h, s, p in ((48, 32, 1, 3), (64, 16, 2, 0), (32, 48, 3, 4 - 2))]
+    buf, want = bytearray(), []
+    buf += rng.integers(0, 256, 37, dtype=np.uint8).tobytes()
+    buf += b"\x00\x00\x00\x10icpf" + b"\x01" * 8          # frame_size 16 < 28
+    buf += b"\x7f\xff\xff\xffgarbage icpf"                  # 'icpf' without a plausible size
+    for p in pk:
+        buf += rng.integers(0, 256, int(rng.integers(0, 50)), dtype=np.uint8).tobytes()
+        want.append((len(buf), len(p)))
+        buf += p
+    buf += b"\x00\x10\x00\x00icpf" + b"\x00" * 40           # runs past the end
+    got = P.find_frames(bytes(buf))
+    assert got == want
+    for (o, n), p in zip(got, pk):
+        assert bytes(buf[o:o + n]) == p
+    assert P.find_frames(bytes(buf)[:want[-1][0] + want[-1][1] - 1]) == want[:-1]
+    assert P.find_frames(b"") == [] and P.find_frames(b"icpf") == []
+
+
+def test_fnv1a_known_values():
+    # FNV-1a 64 published test vectors (the empty string, "a", "foobar").
+    base = 0xCBF29CE484222325
+    assert P.fnv1a64(base, b"") == 0xCBF29CE484222325
+    assert P.fnv1a64(base, b"a") == 0xAF63DC4C8601EC8C
+    assert P.fnv1a64(base, b"foobar") == 0x85944171F73967E8

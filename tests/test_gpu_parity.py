@@ -253,3 +253,20 @@ def test_host_batch_equals_the_oracle():
             assert g.alpha is None
         else:
             np.testing.assert_array_equal(g.alpha, w.alpha, err_msg=f"packet {k} alpha")
+
+
+def test_decode_file_hash_equals_the_oracles():
+    """decode_file (examples/prores_bench.rs:104-122): every frame of a
+    MOV-like file decoded on the GPU; the count and the FNV-1a of the
+    pictures equal the oracle's."""
+    rng = np.random.default_rng(9)
+    buf, pk = bytearray(b"\x00\x00\x00\x20ftypqt  " + b"\x00" * 20), []
+    for i, (w, h, ch, bd, prof) in enumerate([(720, 486, 2, 10, 3), (1280, 720, 2, 10, 1), (256, 144, 3, 12, 4)]):
+        p = O.encode(O.test_frame(w, h, ch, bd, 12, 70 + i), O.config(prof))
+        buf += rng.integers(0, 256, 24, dtype=np.uint8).tobytes() + p
+        pk.append(p)
+    h = 0xCBF29CE484222325
+    for p in pk:
+        h = P.fnv1a64(h, O.decode(p).data.astype("<u2"))
+    print_case("decode file", len(pk))
+    assert P.decode_file(bytes(buf)) == (len(pk), h)
