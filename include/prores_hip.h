@@ -172,11 +172,15 @@ int32_t prores_profile_from_fourcc(const char code[4]); /* -1 if none */
 /* ---- Device-resident batch entry points ----
  * No counterpart in the reference (it runs on CPU threads). For callers that
  * keep bitstreams and planes in GPU memory (the batched path, the serving
- * loop, the benchmark): packets and outputs are device pointers; the host
- * copy of each packet is read for its headers only.
+ * loop, the benchmark): packets and outputs are device pointers. The host
+ * copy of a packet, where the caller has one, is read for its headers only;
+ * without one (host_packet NULL) the library gathers the header bytes on the
+ * device (one small launch and a download of 192 bytes a packet).
+ * A job without dev_packet or dev_data, or without dev_alpha for a frame
+ * that has alpha, fails with PRORES_ERR_CONFIG and decodes nothing.
  */
 typedef struct {
-  const uint8_t *host_packet; /* host bytes: at least the frame and picture headers */
+  const uint8_t *host_packet; /* host bytes (at least the frame and picture headers), or NULL */
   size_t len;                 /* bytes of the packet (>= frame_size) */
   const uint8_t *dev_packet;  /* device copy of the packet; 64 readable bytes past len */
   uint16_t *dev_data;         /* device output planes, Frame::layout at the decoder's depth */

@@ -73,7 +73,7 @@ struct GraphKey {
 
 struct Workspace {
   DevBuf frames, pics, rows, slices, status, first_err, slice_err, coefs, wide, packet, planes, hdr, items, segs,
-      alpha_order;
+      alpha_order, gather, records;
   hipGraphExec_t graph = nullptr;
   GraphKey key;
 };
@@ -82,18 +82,42 @@ Workspace &workspace(Lane &L) {
   return *(Workspace *)L.dec;
 }
 
+// Where plan_frame reads a frame's headers: the whole packet in host memory,
+// or the HDR_BLOB record pd_headers gathered from a packet that is only in
+// device memory.
+struct HeaderSource {
+  const uint8_t *host = nullptr;
+  const uint8_t *record = nullptr;
+  const uint8_t *frame() const { return host ? host : record; }
+  // The picture header at byte `pos` of the packet (its first HDR_PIC_BYTES
+  // bytes are readable), or null if the record holds no picture there.
+  const uint8_t *picture(size_t pos) const {
+    if (host) return host + pos;
+    uint64_t at[2];
+    memcpy(at, record + HDR_POS_AT, sizeof at);
+    for (uint32_t i = 0; i < 2; i++)
+      if (at[i] == pos) return record + HDR_PIC_AT + i * HDR_PIC_BYTES;
+    return nullptr;
+  }
+};
+
 // Lays out one frame's pictures. `dev_packet` is the device address of the
 // packet; `dev_data` / `dev_alpha` the device output planes.
-void plan_frame(const uint8_t *host, size_t len, uint64_t dev_packet, uint32_t want_depth,
+void plan_frame(const HeaderSource &src, size_t len, uint64_t dev_packet, uint32_t want_depth,
                 uint16_t *dev_data, uint16_t *dev_alpha, Batch &b, const Header *parsed) {
   FramePlan fp;
   Header hd;
   if (parsed) {
     hd = *parsed;
   } else {
-    fp.early = parse_frame_header(host, len, hd);
+    fp.early = parse_frame_header(src.frame(), len, hd);
     if (fp.early.ok()) fp.early = check_plausible_size(hd);
   }
+  // The device writes through these: a job without them decodes nothing.
+  if (fp.early.ok() && (!dev_packet || !dev_data || (hd.alpha_bits() && !dev_alpha)))
+    fp.early = config_error(!dev_packet ? "a decode job has no device packet"
+                            : !dev_data ? "a decode job has no device planes"
+                                        : "a decode job has no alpha plane for a frame with alpha");
   if (!fp.early.ok()) {
     b.plans.push_back(fp);
     return;
@@ -140,7 +164,9 @@ void plan_frame(const uint8_t *host, size_t len, uint64_t dev_packet, uint32_t w
       break;
     }
     PicHeader ph;
-    Err e = parse_picture_header(host + pos, h.frame_size - pos, ph);
+    const uint8_t *at = src.picture(pos);
+    Err e = at ? parse_picture_header(at, h.frame_size - pos, ph)
+               : Err{PRORES_ERR_DEVICE, "the gathered header bytes hold no picture header here"};
     if (!e.ok()) {
       fp.deferred = e;
       break;
@@ -612,7 +638,7 @@ static prores_status decode_common(const prores_decoder *d, const uint8_t *data,
     Batch b;
     uint16_t *dd = ws.planes.as<uint16_t>();
     uint16_t *da = alpha_n ? (uint16_t *)((uint8_t *)ws.planes.p + plane_bytes) : nullptr;
-    plan_frame(data, len, (uint64_t)ws.packet.p, d->bit_depth, dd, da, b, &hd);
+    plan_frame(HeaderSource{data, nullptr}, len, (uint64_t)ws.packet.p, d->bit_depth, dd, da, b, &hd);
     std::vector<Err> results;
     prores_decode_tuning t;
     prores_decode_tuning_default(&t);
@@ -646,6 +672,49 @@ prores_status prores_decode_into(const prores_decoder *d, const uint8_t *data, s
 
 namespace prores_hip {
 
+// Plans a batch of device jobs. The header bytes of jobs whose packet is
+// only in device memory (a null host_packet) are gathered first: one
+// pd_headers launch over those packets and one download of HDR_BLOB bytes
+// each.
+Err plan_jobs(Device &d, Lane &L, uint32_t bit_depth, size_t n, const prores_dev_decode_job *jobs, Batch &b) {
+  std::vector<uint64_t> gather;  // device packets, then their lengths
+  for (size_t i = 0; i < n; i++)
+    if (!jobs[i].host_packet && jobs[i].dev_packet) gather.push_back((uint64_t)jobs[i].dev_packet);
+  const size_t m = gather.size();
+  std::vector<uint8_t> records;
+  if (m) {
+    for (size_t i = 0; i < n; i++)
+      if (!jobs[i].host_packet && jobs[i].dev_packet) gather.push_back(jobs[i].len);
+    Workspace &ws = workspace(L);
+    Err e = upload(L, ws.gather, gather);
+    if (e.ok()) e = ws.records.reserve(m * HDR_BLOB + 64);
+    if (!e.ok()) return e;
+    void *pk = ws.gather.p, *ln = (uint64_t *)ws.gather.p + m, *rec = ws.records.p;
+    uint32_t count = (uint32_t)m;
+    void *args[] = {&pk, &ln, &rec, &count};
+    if (!(e = launch(L, d.k_headers, (unsigned)m, 1, 64, args)).ok()) return e;
+    records.resize(m * HDR_BLOB);
+    e = check(hipMemcpyAsync(records.data(), ws.records.p, records.size(), hipMemcpyDeviceToHost, L.stream),
+              "D2H header records");
+    if (e.ok()) e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize(headers)");
+    if (!e.ok()) return e;
+  }
+  size_t k = 0;
+  for (size_t i = 0; i < n; i++) {
+    const prores_dev_decode_job &j = jobs[i];
+    HeaderSource src{j.host_packet, nullptr};
+    if (!j.host_packet && j.dev_packet) src.record = &records[HDR_BLOB * k++];
+    if (!src.frame()) {
+      FramePlan fp;
+      fp.early = config_error("a decode job has neither a host nor a device packet");
+      b.plans.push_back(fp);
+      continue;
+    }
+    plan_frame(src, j.len, (uint64_t)j.dev_packet, bit_depth, j.dev_data, j.dev_alpha, b, nullptr);
+  }
+  return Err{};
+}
+
 Err decode_batch_on(Device &d, Lane &L, uint32_t bit_depth, size_t n, const prores_dev_decode_job *jobs,
                     const prores_decode_tuning *tuning, prores_error *errs) {
   prores_decode_tuning t;
@@ -656,11 +725,9 @@ Err decode_batch_on(Device &d, Lane &L, uint32_t bit_depth, size_t n, const pror
     t = d.tuning;
   }
   Batch b;
-  for (size_t i = 0; i < n; i++)
-    plan_frame(jobs[i].host_packet, jobs[i].len, (uint64_t)jobs[i].dev_packet, bit_depth, jobs[i].dev_data,
-               jobs[i].dev_alpha, b, nullptr);
+  Err e = plan_jobs(d, L, bit_depth, n, jobs, b);
   std::vector<Err> results;
-  Err e = run_batch(d, L, b, t, results);
+  if (e.ok()) e = run_batch(d, L, b, t, results);
   if (!e.ok()) {
     for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
     return e;
@@ -765,7 +832,7 @@ prores_status prores_decode_host_batch(const prores_decoder *d, size_t n, const 
         uint16_t *da = items[i].alpha_n ? (uint16_t *)((uint8_t *)ws.planes.p + aoff[i - a]) : nullptr;
         e = check(hipMemcpyAsync(dp, data[i], items[i].hd.h.frame_size, hipMemcpyHostToDevice, L->stream),
                   "H2D packet");
-        plan_frame(data[i], len[i], (uint64_t)dp, d->bit_depth, dd, da, bt, &items[i].hd);
+        plan_frame(HeaderSource{data[i], nullptr}, len[i], (uint64_t)dp, d->bit_depth, dd, da, bt, &items[i].hd);
         idx.push_back(i);
       }
       std::vector<Err> results;
@@ -866,9 +933,11 @@ prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const pr
   }
   std::lock_guard<std::mutex> g(lane0->mu);
   Batch b;
-  for (size_t i = 0; i < n; i++)
-    plan_frame(jobs[i].host_packet, jobs[i].len, (uint64_t)jobs[i].dev_packet, d->bit_depth, jobs[i].dev_data,
-               jobs[i].dev_alpha, b, nullptr);
+  e = plan_jobs(*dev, *lane0, d->bit_depth, n, jobs, b);
+  if (!e.ok()) {
+    fill_error(err, e);
+    return (prores_status)e.kind;
+  }
   if (reps < 1) reps = 1;
   std::vector<prores_decode_tuning> cands;
   for (uint32_t pers = 0; pers < 2; pers++)

@@ -1,6 +1,8 @@
 // gfx950 decode kernels for the ProRes slice engine.
 //
-// Four launches per batch of pictures:
+// Four launches per batch of pictures (and, for packets the caller holds only
+// in device memory, a fifth before them, pd_headers, that gathers the header
+// bytes the host plans the batch from):
 // Packet and slice offsets are absolute device addresses.
 //   pd_slice_table  one workgroup per picture: the u16 big-endian slice-size
 //                   table becomes byte offsets (an exclusive prefix), plus the
@@ -1236,5 +1238,37 @@ extern "C" __global__ void __launch_bounds__(256)
       __syncthreads();
     }
     if (!persistent) break;
+  }
+}
+
+// Gathers the bytes the host reads to plan a frame (the frame header, and
+// the picture headers that Decoder::decode walks with `pos`:
+// src/decode.rs:85-102, src/header.rs:82-170, 262-290) from packets that are
+// only in device memory, into one HDR_BLOB record a packet. One wave a
+// packet; every read is bounded by the packet's length. The host repeats the
+// header checks on the gathered bytes, so nothing is validated here beyond
+// the bounds.
+extern "C" __global__ void __launch_bounds__(64)
+pd_headers(const uint64_t *packets, const uint64_t *lens, uint8_t *records, uint32_t n) {
+  const uint32_t j = blockIdx.x, t = threadIdx.x;
+  if (j >= n) return;
+  const uint8_t *p = (const uint8_t *)packets[j];
+  const uint64_t len = lens[j];
+  uint8_t *out = records + (uint64_t)j * HDR_BLOB;
+  for (uint32_t i = t; i < HDR_FRAME_BYTES; i += 64) out[i] = i < len ? p[i] : (uint8_t)0;
+  // The first picture follows the frame header (8 + frame_header_size);
+  // an interlaced frame's second follows the first (+ picture_size).
+  uint64_t pos[2] = {~0ull, ~0ull};
+  if (len >= 28) {
+    pos[0] = 8u + ((uint32_t)p[8] << 8 | p[9]);
+    const uint64_t q = pos[0];
+    if (q + HDR_PIC_BYTES <= len && ((p[20] >> 2) & 3) != 0)
+      pos[1] = q + ((uint32_t)p[q + 1] << 24 | (uint32_t)p[q + 2] << 16 | (uint32_t)p[q + 3] << 8 | p[q + 4]);
+  }
+  if (t < 2 * HDR_PIC_BYTES) {
+    const uint64_t q = pos[t / HDR_PIC_BYTES], i = t % HDR_PIC_BYTES;
+    out[HDR_PIC_AT + t] = q != ~0ull && q + HDR_PIC_BYTES <= len ? p[q + i] : (uint8_t)0;
+  } else if (t < 2 * HDR_PIC_BYTES + 2) {
+    ((uint64_t *)(out + HDR_POS_AT))[t - 2 * HDR_PIC_BYTES] = pos[t - 2 * HDR_PIC_BYTES];
   }
 }

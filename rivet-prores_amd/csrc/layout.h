@@ -113,6 +113,15 @@ struct DecBatch {
 struct IdctItem {
   uint32_t first, count, blocks, pad;
 };
+// Header bytes pd_headers gathers from a packet held only in device memory,
+// one HDR_BLOB record a packet: the packet's first HDR_FRAME_BYTES bytes (a
+// frame header with both quantisation matrices ends at byte 156), the first
+// HDR_PIC_BYTES bytes of each of the two pictures, and the two pictures'
+// byte offsets (u64; ~0 where the packet is too short to locate one).
+constexpr uint32_t HDR_FRAME_BYTES = 160, HDR_PIC_BYTES = 8, HDR_BLOB = 192;
+constexpr uint32_t HDR_PIC_AT = HDR_FRAME_BYTES, HDR_POS_AT = HDR_FRAME_BYTES + 2 * HDR_PIC_BYTES;
+static_assert(HDR_POS_AT + 16 == HDR_BLOB, "header record layout");
+
 constexpr uint32_t IDCT_MAX_BLOCKS = 96;  // blocks one workgroup holds in LDS
 constexpr uint32_t IDCT_MAX_SLICES = 12;  // slices of one item (a slice is at least 8 blocks)
 // Entropy decode LDS: two 32-coefficient half-lines a lane (16-byte aligned),

@@ -509,8 +509,11 @@ class Decoder:
 
 
 class DecodeJobs:
-    """A prepared list of device decode jobs (the host packets are referenced,
-    not copied), for repeated batches."""
+    """A prepared list of device decode jobs, for repeated batches. Each job
+    is (packet, dev_packet, dev_data, dev_alpha): `packet` is the host copy of
+    the packet (read for its headers only; kept referenced, not copied), or
+    an int, the packet's length, when the packet is only in device memory
+    and its headers are to be read there."""
 
     def __init__(self, jobs):
         self.n = len(jobs)
@@ -518,10 +521,14 @@ class DecodeJobs:
         self.errs = (_Err * self.n)()
         self._keep = []
         for i, (hp, dp, dd, da) in enumerate(jobs):
-            hp = bytes(hp)
-            self._keep.append(hp)
-            self.arr[i].host_packet = C.cast(C.c_char_p(hp), C.c_void_p)
-            self.arr[i].len = len(hp)
+            if isinstance(hp, int):
+                self.arr[i].host_packet = None
+                self.arr[i].len = hp
+            else:
+                hp = bytes(hp)
+                self._keep.append(hp)
+                self.arr[i].host_packet = C.cast(C.c_char_p(hp), C.c_void_p)
+                self.arr[i].len = len(hp)
             self.arr[i].dev_packet, self.arr[i].dev_data, self.arr[i].dev_alpha = dp, dd, da or None
 
 

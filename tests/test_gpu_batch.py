@@ -124,3 +124,60 @@ def test_autotune_then_batch(batch):
     assert P.get_decode_tuning() == best
     check_batch(dec.decode_device_batch(jobs), want, bufs)
     P.set_decode_tuning(P.DecodeTuning())
+
+
+@pytest.mark.parametrize("graph", (0, 1))
+def test_device_only_packets(batch, graph):
+    """Jobs without a host copy of their packet: the header bytes are gathered
+    on the device. Beside them, jobs with a host copy, and packets cut short
+    (inside the frame header, inside a picture header, inside the slices),
+    whose errors must be the oracle's."""
+    packets, want, bufs, _ = batch
+    print("case: device-only packets, graph", graph, flush=True)
+    cut = packets[3]
+    hsize = int.from_bytes(cut[8:10], "big")
+    # The interlaced frame with its second picture's picture_size made huge,
+    # and a progressive one with a picture_header_size of zero.
+    second = bytearray(packets[0])
+    pos2 = 8 + int.from_bytes(second[8:10], "big")
+    pos2 += int.from_bytes(second[pos2 + 1:pos2 + 5], "big")
+    second[pos2 + 1:pos2 + 5] = b"\xff\xff\xff\xf0"
+    zero = bytearray(packets[2])
+    zero[8 + int.from_bytes(zero[8:10], "big")] &= 7
+    short = [cut[:20], cut[:8 + hsize + 5], cut[:len(cut) // 2], bytes(second), bytes(zero)]
+    extra = []
+    for p in short:
+        dp = P.DeviceBuffer(len(p) + 128)
+        dp.upload(np.frombuffer(p, np.uint8))
+        dd = P.DeviceBuffer(16 << 20)  # a 1920x1080 4:2:2 frame's planes take 8.3 MB
+        da = P.DeviceBuffer(4 << 20)
+        extra.append((dp, dd, da))
+    jobs = [(len(p) if i % 2 == 0 else p, dp.ptr, dd.ptr, da.ptr if da else 0)
+            for i, (p, (dp, dd, da, n, h)) in enumerate(zip(packets, bufs))]
+    jobs += [(len(p), dp.ptr, dd.ptr, da.ptr) for p, (dp, dd, da) in zip(short, extra)]
+    for dp, dd, da, n, h in bufs:
+        dd.upload(np.full(n, 0xABCD, np.uint16))
+    dec = P.Decoder()
+    errs = dec.decode_device_batch(P.DecodeJobs(jobs), P.DecodeTuning(1, 1, 64, 0, graph))
+    check_batch(errs[:len(packets)], want, bufs)
+    for p, e in zip(short, errs[len(packets):]):
+        w = O.decode_result(p)
+        assert w[0] != "Ok" and e is not None and (e.kind, e.msg) == (w[0], w[1]), (len(p), e, w)
+
+
+def test_jobs_without_outputs_fail_alone(batch):
+    """A job with no device packet, no planes, or no alpha plane for a frame
+    with alpha fails with a configuration error and decodes nothing; the
+    other jobs of the batch decode."""
+    packets, want, bufs, _ = batch
+    print("case: jobs without outputs", flush=True)
+    dp, dd, da, n, h = bufs[3]  # the frame with 8-bit alpha
+    assert h.alpha
+    good = bufs[2]
+    good[1].upload(np.zeros(good[3], np.uint16))
+    jobs = [(packets[3], dp.ptr, dd.ptr, 0), (len(packets[3]), 0, dd.ptr, da.ptr),
+            (packets[3], dp.ptr, 0, da.ptr), (len(packets[2]), good[0].ptr, good[1].ptr, 0)]
+    errs = P.Decoder().decode_device_batch(jobs)
+    assert [type(e) for e in errs[:3]] == [P.ConfigError] * 3, errs
+    assert errs[3] is None
+    np.testing.assert_array_equal(good[1].download(good[3] * 2, dtype=np.uint16), want[2][1].data)
