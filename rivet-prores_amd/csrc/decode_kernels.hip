@@ -581,8 +581,168 @@ constexpr uint32_t ALPHA_WORDS = (2048 * 33 / 8) / 8 + 6;
 // A value the whole wave holds, as a scalar.
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Alpha streams of at least this many bytes are parsed by alpha_parallel.
+constexpr uint32_t ALPHA_PAR_BYTES = 64;
+// No pair of a slice's decode starts at or after this bit: 2048 values take
+// at most 2048 pairs of at most 33 bits.
+constexpr uint32_t ALPHA_MAX_BITS = 2048 * 33;
+
+// 64 bits of the staged alpha stream from absolute bit q, MSB first; at
+// least the first 33 are stream bits. The staged u64 words are big-endian
+// values in little-endian LDS, so 32-bit chunk k of the stream is u32 k ^ 1.
+__device__ __forceinline__ uint64_t alpha_window(const uint32_t *w32, uint32_t q) {
+  const uint32_t c = q >> 5;
+  return ((uint64_t)w32[c ^ 1] << 32 | w32[(c + 1) ^ 1]) << (q & 31);
+}
+
+// The length of the (difference, run) pair at the head of window x: a
+// difference of 1 + bits (raw) or 2 + mag (magnitude and sign), then a run
+// of 1, 5 or 16 bits (src/vlc.rs:263-299). Needs the first 22 bits.
+__device__ __forceinline__ uint32_t alpha_pair_len(uint64_t x, uint32_t bits, uint32_t mag) {
+  const uint32_t hi = (uint32_t)(x >> 32);
+  const uint32_t dl = (hi >> 31) ? 1 + bits : 2 + mag;
+  const uint32_t r = (hi << dl) >> 27;  // the run code's first 5 bits
+  return dl + ((r & 16) ? 1u : ((r & 15) ? 5u : 16u));
+}
+
+// The pair at the head of window x (every bit of it inside the window).
+__device__ __forceinline__ void alpha_pair(uint64_t x, uint32_t bits, uint32_t mag, int32_t &d, uint32_t &run) {
+  const uint32_t used = (x >> 63) ? 1 + bits : 2 + mag;
+  const int32_t av = (int32_t)((x >> (63 - mag)) & ((1u << mag) - 1)) + 1;
+  const int32_t small = ((x >> (62 - mag)) & 1) ? -av : av;
+  d = (x >> 63) ? (int32_t)((x >> (63 - bits)) & ((1u << bits) - 1)) : small;
+  const uint64_t x2 = x << used;
+  const uint32_t v = (uint32_t)(x2 >> 59) & 15;
+  run = (x2 >> 63) ? 1u : (v ? v + 1 : (uint32_t)((x2 >> 48) & 0x7FF) + 1);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
+  const uint32_t L = threadIdx.x & 63;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, 64);
+    if (L >= o) v += t;
+  }
+  return v;
+}
+
+// The pairs of a long alpha stream that start at stream bits 0..F (each has
+// all of its bits in the stream, so the reference reads them unchecked;
+// src/decode.rs:570-595), with the whole wave. For each 64-bit window of the
+// stream every lane computes the pair length at one bit position, and a
+// scalar walk over those lengths marks the window's pair starts (a few
+// scalar steps a pair instead of the whole decode). Every 64 windows the
+// lanes decode the marked pairs, one window a lane: the run and difference
+// sums, wave prefix sums, then each pair's value at its first output
+// position, flagged in `marks` (alpha_fill spreads the runs). Stops where
+// the reference's loop does: at the first pair that starts at or past
+// `count` values, or at a run past the end (E_ALPHA_RUN). Returns true when
+// the decode ended here (`err` says how); otherwise `pos` is the first pair
+// start after F, where the reference's checked reads take over, with `n`
+// values decoded and `prev` the last value.
+__device__ bool alpha_parallel(const uint32_t *w32, uint32_t base, uint32_t F, uint32_t bits, uint32_t mag,
+                               uint32_t count, uint32_t ident, uint64_t rm, uint16_t *vals, uint32_t *marks,
+                               uint32_t &pos, uint32_t &n, int32_t &prev, uint32_t &err) {
+  const uint32_t L = threadIdx.x & 63, mask = (1u << bits) - 1;
+  uint32_t w = 0, c = 0;  // the walk: the next pair starts at bit c of window w
+  for (;;) {
+    const uint32_t w0 = w;
+    uint64_t gm = 0;  // lane j: the pair starts of window w0 + j
+    uint32_t nw = 0;
+    bool tail = false;
+    while (nw < 64) {
+      const uint32_t pl = alpha_pair_len(alpha_window(w32, base + 64 * w + L), bits, mag);
+      const uint32_t room = F - 64 * w + 1;  // 64 * w <= F here
+      const uint32_t lim = room < 64 ? room : 64u;
+      uint64_t mk = 0;
+      while (c < lim) {
+        mk |= 1ull << c;
+        c += (uint32_t)__builtin_amdgcn_readlane((int)pl, (int)c);
+      }
+      gm = L == nw ? mk : gm;
+      nw++;
+      if (c < 64) {  // the next pair starts past F inside this window
+        tail = true;
+        break;
+      }
+      c -= 64;
+      w++;
+      if (64 * w > F) {
+        tail = true;
+        break;
+      }
+    }
+    // The group's pairs, one window a lane.
+    const uint64_t mine = L < nw ? gm : 0ull;
+    const uint32_t at = base + 64 * (w0 + L);
+    uint32_t rs = 0, ds = 0;
+    for (uint64_t t = mine; t; t &= t - 1) {
+      int32_t d;
+      uint32_t run;
+      alpha_pair(alpha_window(w32, at + (uint32_t)__builtin_ctzll(t)), bits, mag, d, run);
+      rs += run;
+      ds += (uint32_t)d;
+    }
+    const uint32_t ri = wave_incl_sum(rs), di = wave_incl_sum(ds);
+    uint32_t nn = n + ri - rs, acc = (uint32_t)prev + di - ds;
+    bool stop = false, bad = false;
+    for (uint64_t t = mine; t; t &= t - 1) {
+      if (nn >= count) {
+        stop = true;
+        break;
+      }
+      int32_t d;
+      uint32_t run;
+      alpha_pair(alpha_window(w32, at + (uint32_t)__builtin_ctzll(t)), bits, mag, d, run);
+      acc += (uint32_t)d;
+      if (nn + run > count) {
+        bad = true;
+        break;
+      }
+      vals[nn] = rescale(acc & mask, ident, rm);
+      atomicOr(&marks[nn >> 5], 1u << (nn & 31));
+      nn += run;
+    }
+    if (__ballot(bad)) {
+      err = E_ALPHA_RUN;
+      return true;
+    }
+    if (__ballot(stop)) return true;
+    n += uni((uint32_t)__builtin_amdgcn_readlane((int)ri, 63));
+    prev = (int32_t)(((uint32_t)prev + uni((uint32_t)__builtin_amdgcn_readlane((int)di, 63))) & mask);
+    if (n >= count) return true;
+    if (tail) {
+      pos = 64 * w + c;
+      return false;
+    }
+  }
+}
+
+// Spreads each marked value of `vals` over the positions up to the next
+// mark (lane L: positions 32L..32L+31; a mark's value is only read).
+__device__ __forceinline__ void alpha_fill(uint16_t *vals, const uint32_t *marks) {
+  const uint32_t L = threadIdx.x & 63;
+  const uint32_t mw = marks[L];
+  int32_t last = mw ? (int32_t)(32 * L + 31 - __builtin_clz(mw)) : -1;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const int32_t t = __shfl_up(last, o, 64);
+    if (L >= o) last = t > last ? t : last;
+  }
+  int32_t before = __shfl_up(last, 1, 64);
+  if (L == 0) before = -1;
+  uint16_t cur = before >= 0 ? vals[before] : (uint16_t)0;
+  for (uint32_t k = 0; k < 32; k++) {
+    const uint32_t i = 32 * L + k;
+    if ((mw >> k) & 1)
+      cur = vals[i];
+    else
+      vals[i] = cur;
+  }
+}
+
 __device__ __forceinline__ void alpha_slice(
-    uint32_t g, uint64_t *__restrict__ words, uint16_t *__restrict__ vals,
+    uint32_t g, uint64_t *__restrict__ words, uint16_t *__restrict__ vals, uint32_t *__restrict__ marks,
     const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
     const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
     uint32_t *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err) {
@@ -670,7 +830,24 @@ __device__ __forceinline__ void alpha_slice(
   };
   uint32_t n = 0;
   int32_t prev = -1;
-  while (n < count) {
+  const bool par = len >= ALPHA_PAR_BYTES;
+  if (par) {
+    marks[L] = 0;
+    const uint32_t last = lenbits - 33 < ALPHA_MAX_BITS - 1 ? lenbits - 33 : ALPHA_MAX_BITS - 1;
+    uint32_t p = 0;
+    if (alpha_parallel((const uint32_t *)words, 8 * m, last, bits, mag, count, ident, rm, vals, marks, p, n, prev,
+                       err)) {
+      n = count;  // decoded to the end (or failed: err)
+    } else {
+      const uint32_t q = 8 * m + p;
+      wj = q >> 6;
+      off = q & 63;
+      pos = p;
+      A = word(wj);
+      B = word(wj + 1);
+    }
+  }
+  while (n < count && !err) {
     const uint32_t rem = lenbits - pos;
     int32_t d;
     uint32_t run;
@@ -732,7 +909,12 @@ __device__ __forceinline__ void alpha_slice(
     prev = (int32_t)value;
     if (n + run > count) { err = E_ALPHA_RUN; break; }
     const uint16_t out = rescale(value, ident, rm);
-    if (run <= 64) {
+    if (par) {
+      if (L == 0) {
+        vals[n] = out;
+        atomicOr(&marks[n >> 5], 1u << (n & 31));
+      }
+    } else if (run <= 64) {
       if (L < run) vals[n + L] = out;
     } else {
       // A long run: to a 4-value boundary, then 8 bytes per lane.
@@ -753,6 +935,7 @@ __device__ __forceinline__ void alpha_slice(
     }
     return;
   }
+  if (par) alpha_fill(vals, marks);
   __syncthreads();
   // Visible rows, clipped to the plane width.
   const uint32_t x0 = 16u * S.mb_x, width = F.width;
@@ -937,10 +1120,11 @@ extern "C" __global__ void __launch_bounds__(64)
              uint32_t persistent) {
   __shared__ uint64_t words[ALPHA_WORDS];
   __shared__ __attribute__((aligned(16))) uint16_t vals[2048];
+  __shared__ uint32_t marks[64];
   const uint32_t n = B->n_slices;
   if (!persistent) {
     if (blockIdx.x < n)
-      alpha_slice(order[blockIdx.x], words, vals, frames, pics, slices, pic_status, slice_err, pic_first_err);
+      alpha_slice(order[blockIdx.x], words, vals, marks, frames, pics, slices, pic_status, slice_err, pic_first_err);
     return;
   }
   const uint32_t chunk = B->chunk;
@@ -949,7 +1133,7 @@ extern "C" __global__ void __launch_bounds__(64)
     if (first >= n) break;
     const uint32_t last = first + chunk < n ? first + chunk : n;
     for (uint32_t i = first; i < last; i++) {
-      alpha_slice(order[i], words, vals, frames, pics, slices, pic_status, slice_err, pic_first_err);
+      alpha_slice(order[i], words, vals, marks, frames, pics, slices, pic_status, slice_err, pic_first_err);
       __syncthreads();
     }
   }
