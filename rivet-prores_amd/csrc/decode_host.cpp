@@ -64,8 +64,9 @@ struct Batch {
 // arguments are workspace addresses and launch shapes, so it replays while
 // those stay the same.
 struct GraphKey {
-  void *ptrs[14];
+  void *ptrs[16];
   unsigned grids[4];
+  unsigned ewg;
   prores_decode_tuning t;
   bool alpha;
   bool operator==(const GraphKey &o) const { return memcmp(this, &o, sizeof *this) == 0; }
@@ -73,7 +74,7 @@ struct GraphKey {
 
 struct Workspace {
   DevBuf frames, pics, rows, slices, status, first_err, slice_err, coefs, wide, packet, planes, hdr, items, segs,
-      alpha_order, gather, records;
+      alpha_order, gather, records, slice_size, order;
   hipGraphExec_t graph = nullptr;
   GraphKey key;
 };
@@ -309,11 +310,12 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   if (!(e = ws.coefs.reserve(b.n_coefs * 2 + 64)).ok()) return e;
   if (!(e = ws.wide.reserve(b.n_coefs * 4 + 64)).ok()) return e;
   if (!(e = ws.alpha_order.reserve(G * 4 + 64)).ok()) return e;
+  if (!(e = ws.slice_size.reserve(G * 2 + 64)).ok()) return e;
 
   void *B = ws.hdr.p, *pics = ws.pics.p, *rows = ws.rows.p, *slices = ws.slices.p, *status = ws.status.p;
   void *frames = ws.frames.p, *tables = d.tables, *coefs = ws.coefs.p, *wide = ws.wide.p;
   void *serr = ws.slice_err.p, *ferr = ws.first_err.p, *itemp = ws.items.p, *segp = ws.segs.p;
-  void *aord = ws.alpha_order.p;
+  void *aord = ws.alpha_order.p, *ssz = ws.slice_size.p;
   uint32_t pers = t.persistent;
   const unsigned ewg = entropy_wg(d, t, hdr.n_entropy_items), ewaves = ewg / 64;
   unsigned grids[4];
@@ -328,12 +330,26 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     grids[2] = (unsigned)G;
     grids[3] = (unsigned)n_items;
   }
+  // Entropy items longest first (pd_entropy_order) when the sort fits: the
+  // grid's slots, or the queue's items (0: the kernel reads the count from
+  // the batch, so a captured graph replays for any batch of its shape).
+  const uint32_t slots = t.persistent ? hdr.n_entropy_items : grids[1] * ewaves;
+  uint32_t n_slots = t.persistent ? 0u : slots;
+  void *ordp = nullptr;
+  if (slots <= ORDER_MAX) {
+    if (!(e = ws.order.reserve((size_t)ORDER_MAX * 4 + 64)).ok()) return e;
+    ordp = ws.order.p;
+  }
   auto enqueue = [&](bool timed) -> Err {
     Err x;
     if (timed) (void)hipEventRecord(L.ev[0], L.stream);
     {
-      void *args[] = {&B, &frames, &pics, &rows, &slices, &status, &serr, &ferr, &segp, &itemp, &aord};
+      void *args[] = {&B, &frames, &pics, &rows, &slices, &status, &serr, &ferr, &segp, &itemp, &aord, &ssz};
       if (!(x = launch(L, d.k_slice_table, grids[0], 1, 256, args)).ok()) return x;
+    }
+    if (ordp) {
+      void *args[] = {&B, &ssz, &ordp, &n_slots};
+      if (!(x = launch(L, d.k_order, 1, 1, 1024, args)).ok()) return x;
     }
     if (timed) (void)hipEventRecord(L.ev[1], L.stream);
     // Alpha needs only the slice table: it runs on the second stream beside
@@ -355,7 +371,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
       }
     }
     {
-      void *args[] = {&tables, &B, &frames, &pics, &slices, &status, &coefs, &wide, &serr, &ferr, &pers};
+      void *args[] = {&tables, &B, &frames, &pics, &slices, &status, &coefs, &wide, &serr, &ferr, &ordp, &pers};
       x = launch(L, d.k_entropy, grids[1], 1, ewg, args, ewaves * ENTROPY_LDS_PER_WAVE);
     }
     if (x.ok()) {
@@ -377,9 +393,11 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   } else if (t.graph) {
     GraphKey key;
     memset(&key, 0, sizeof key);
-    void *ptrs[14] = {B, pics, rows, slices, status, frames, tables, coefs, wide, serr, ferr, itemp, segp, aord};
+    void *ptrs[16] = {B, pics, rows, slices, status, frames, tables, coefs, wide, serr, ferr, itemp, segp, aord,
+                      ssz, ordp};
     memcpy(key.ptrs, ptrs, sizeof ptrs);
     memcpy(key.grids, grids, sizeof grids);
+    key.ewg = ewg;
     key.t = t;
     key.alpha = b.any_alpha;
     if (!ws.graph || !(key == ws.key)) {

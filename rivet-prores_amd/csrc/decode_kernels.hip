@@ -384,7 +384,7 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
   while (__ballot(act)) {
     // Pairs from the tables. Every lane's half-lines before `cur` are out,
     // so the pairs may run up to the end of the next half-line.
-    bool moved = false;
+    bool moved = false, blocked = false;
     if (act) {
       uint64_t w = r.window();
       const uint32_t rem = r.remaining();
@@ -401,6 +401,7 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
         const uint32_t el = *(const uint16_t *)((const char *)ftab + (lbo | ((hi2 >> 21) & 0x7FE)));
         const uint32_t t = lr + (el & 15), m = n + ((er >> 4) & 63);
         const bool ok = go && (er & el & FAST_VALID) != 0 && used + t <= rem && m < lim;
+        blocked = blocked || (go && !ok);
         out.store_slot(ok ? (m & 63) : CoefOut::SPARE, ((int32_t)(el << 21)) >> 25);
         n = ok ? m + 1 : n;
         used = ok ? used + t : used;
@@ -419,8 +420,10 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
         }
       }
     }
-    // One pair the reference's way for lanes the tables did not move.
-    const bool slow = act && !moved;
+    // One pair the reference's way for lanes the tables stopped on (a code
+    // past the tables, a run past the ring, the stream's last bits), in the
+    // same step: a lane with many long codes does not take a step for each.
+    const bool slow = act && (!moved || blocked);
     if (__ballot(slow)) {
       bool have = false;
       uint32_t at = 0;
@@ -962,6 +965,8 @@ __device__ __forceinline__ void alpha_slice(
 }  // namespace
 
 // ---------------------------------------------------------------------------
+constexpr uint32_t ST_PER = 16;  // slices a thread per pass of pd_slice_table
+
 // One workgroup per picture (a grid-stride loop in persistent form): the
 // u16 big-endian slice-size table becomes byte offsets (an exclusive
 // prefix), with the bounds check of the table walk (src/decode.rs:202-221).
@@ -974,7 +979,7 @@ extern "C" __global__ void __launch_bounds__(256)
                    SliceRec *__restrict__ slices, uint32_t *__restrict__ pic_status,
                    uint4 *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err,
                    const IdctItem *__restrict__ segs, IdctItem *__restrict__ items,
-                   uint32_t *__restrict__ alpha_order) {
+                   uint32_t *__restrict__ alpha_order, uint16_t *__restrict__ slice_size) {
   const uint32_t tid = threadIdx.x;
   if (blockIdx.x == 0 && tid < 4) B->q[tid] = 0;
   const uint32_t np = B->n_pics, G = B->n_slices;
@@ -991,10 +996,18 @@ extern "C" __global__ void __launch_bounds__(256)
       pic_first_err[p] = 0xFFFFFFFFu;
     }
     __syncthreads();
-    for (uint32_t base = 0; base < P.n_slices; base += 256) {
-      const uint32_t k = base + tid;
-      const uint32_t size = k < P.n_slices ? ((uint32_t)tab[2 * k] << 8) | tab[2 * k + 1] : 0u;
-      uint32_t incl = size;
+    // ST_PER consecutive slices a thread: their sizes are loaded together,
+    // and one workgroup scan a pass gives every offset.
+    for (uint32_t pb = 0; pb < P.n_slices; pb += 256 * ST_PER) {
+      const uint32_t k0 = pb + tid * ST_PER;
+      uint32_t sz[ST_PER], sum = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < ST_PER; j++) {
+        const uint32_t k = k0 + j;
+        sz[j] = k < P.n_slices ? ((uint32_t)tab[2 * k] << 8) | tab[2 * k + 1] : 0u;
+        sum += sz[j];
+      }
+      uint32_t incl = sum;
       for (uint32_t d = 1; d < 64; d <<= 1) {
         const uint32_t o = __shfl_up(incl, d, 64);
         if (lane >= d) incl += o;
@@ -1003,49 +1016,91 @@ extern "C" __global__ void __launch_bounds__(256)
       __syncthreads();
       uint32_t before = 0;
       for (uint32_t w = 0; w < wave; w++) before += wsum[w];
-      const uint64_t excl = carry + before + incl - size;
-      if (k < P.n_slices) {
-        const uint32_t row = k / P.row_slices, j = k - row * P.row_slices;
-        const RowSlot rs = rows[P.layout_base + j];
-        SliceRec sr;
-        sr.off = P.pic_off + data0 + excl;
-        sr.coef = P.coef_base + (uint64_t)row * P.row_coefs + rs.coef_prefix;
-        sr.len = size;
-        sr.pic = p;
-        sr.mb_x = rs.mb_x;
-        sr.mb_y = (uint16_t)row;
-        sr.mbs = rs.mbs;
-        slices[P.slice_base + k] = sr;
-        slice_err[P.slice_base + k] = make_uint4(0, 0, 0, 0);
+      const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+      uint64_t excl = carry + before + incl - sum;
+      // Every load of the pass is issued before any is used: row slots and
+      // slice headers are read unconditionally, at a valid address (the
+      // last slice's slot, the picture header) where the slice has none.
+      uint64_t at[ST_PER];
+      RowSlot rs[ST_PER];
+      uint32_t valid = 0;  // bit j: slice k0 + j
+#pragma unroll
+      for (uint32_t j = 0; j < ST_PER; j++) {
+        const uint32_t k = k0 + j, kk = k < P.n_slices ? k : P.n_slices - 1;
+        if (k < P.n_slices) valid |= 1u << j;
+        at[j] = excl;
+        excl += sz[j];
+        rs[j] = rows[P.layout_base + (kk - (kk / P.row_slices) * P.row_slices)];
       }
       // The alpha work order: slices with a long alpha stream at the front,
       // the rest from the back (a slice outside the picture reads nothing).
-      bool heavy = false;
-      if (k < P.n_slices && has_alpha && size >= 8 && data0 + excl + size <= P.picture_size) {
-        const uint8_t *sl = (const uint8_t *)(P.pic_off + data0 + excl);
-        const uint32_t hs = sl[0] >> 3;
-        const uint32_t comp = ((uint32_t)sl[2] << 8 | sl[3]) + ((uint32_t)sl[4] << 8 | sl[5]) +
-                              ((uint32_t)sl[6] << 8 | sl[7]);
-        heavy = hs <= size && comp <= size - hs && size - hs - comp > ALPHA_HEAVY_BYTES;
+      uint32_t heavy = 0;
+      if (has_alpha) {
+#pragma unroll
+        for (uint32_t j0 = 0; j0 < ST_PER; j0 += 4) {
+          uint32_t hb0[4], comp[4];
+          bool inside[4];
+#pragma unroll
+          for (uint32_t j = j0; j < j0 + 4; j++) {
+            inside[j - j0] = ((valid >> j) & 1) && sz[j] >= 8 && data0 + at[j] + sz[j] <= P.picture_size;
+            const uint8_t *sl = (const uint8_t *)(P.pic_off + (inside[j - j0] ? data0 + at[j] : 0));
+            hb0[j - j0] = sl[0];
+            comp[j - j0] = ((uint32_t)sl[2] << 8 | sl[3]) + ((uint32_t)sl[4] << 8 | sl[5]) +
+                           ((uint32_t)sl[6] << 8 | sl[7]);
+          }
+#pragma unroll
+          for (uint32_t j = j0; j < j0 + 4; j++) {
+            const uint32_t hs = hb0[j - j0] >> 3, c = comp[j - j0];
+            if (inside[j - j0] && hs <= sz[j] && c <= sz[j] - hs && sz[j] - hs - c > ALPHA_HEAVY_BYTES)
+              heavy |= 1u << j;
+          }
+        }
       }
-      const uint64_t hmask = __ballot(k < P.n_slices && heavy);
-      const uint64_t lmask = __ballot(k < P.n_slices && !heavy);
-      const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
+      for (uint32_t j = 0; j < ST_PER; j++) {
+        const uint32_t k = k0 + j;
+        if ((valid >> j) & 1) {
+          const uint32_t row = k / P.row_slices;
+          SliceRec sr;
+          sr.off = P.pic_off + data0 + at[j];
+          sr.coef = P.coef_base + (uint64_t)row * P.row_coefs + rs[j].coef_prefix;
+          sr.len = sz[j];
+          sr.pic = p;
+          sr.mb_x = rs[j].mb_x;
+          sr.mb_y = (uint16_t)row;
+          sr.mbs = rs[j].mbs;
+          slices[P.slice_base + k] = sr;
+          slice_err[P.slice_base + k] = make_uint4(0, 0, 0, 0);
+          slice_size[P.slice_base + k] = (uint16_t)sz[j];
+        }
+      }
+      const uint32_t nh = (uint32_t)__popc(heavy), nl = (uint32_t)__popc(valid & ~heavy);
+      uint32_t hi = nh, li = nl;
+      for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t a = __shfl_up(hi, d, 64), b = __shfl_up(li, d, 64);
+        if (lane >= d) {
+          hi += a;
+          li += b;
+        }
+      }
       uint32_t hb = 0, lb = 0;
-      if (lane == 0) {
-        if (hmask) hb = atomicAdd(&B->heavy, (uint32_t)__popcll(hmask));
-        if (lmask) lb = atomicAdd(&B->light, (uint32_t)__popcll(lmask));
+      if (lane == 63) {
+        if (hi) hb = atomicAdd(&B->heavy, hi);
+        if (li) lb = atomicAdd(&B->light, li);
       }
-      hb = __shfl(hb, 0, 64);
-      lb = __shfl(lb, 0, 64);
-      if (k < P.n_slices) {
-        if (heavy)
-          alpha_order[hb + __popcll(hmask & below)] = P.slice_base + k;
-        else
-          alpha_order[G - 1 - (lb + __popcll(lmask & below))] = P.slice_base + k;
+      hb = __shfl(hb, 63, 64) + hi - nh;
+      lb = __shfl(lb, 63, 64) + li - nl;
+#pragma unroll
+      for (uint32_t j = 0; j < ST_PER; j++) {
+        if ((valid >> j) & 1) {
+          if ((heavy >> j) & 1)
+            alpha_order[hb++] = P.slice_base + k0 + j;
+          else
+            alpha_order[G - 1 - lb++] = P.slice_base + k0 + j;
+        }
       }
       __syncthreads();
-      if (tid == 255) carry += before + incl;
+      if (tid == 0) carry += total;
       __syncthreads();
     }
     if (tid == 0) pic_status[p] = (data0 + carry > P.picture_size) ? PIC_TABLE_ERR : PIC_OK;
@@ -1069,7 +1124,7 @@ extern "C" __global__ void __launch_bounds__(ENTROPY_MAX_WG)
                const DecPic *__restrict__ pics, const SliceRec *__restrict__ slices,
                const uint32_t *__restrict__ pic_status, int16_t *__restrict__ coefs,
                int32_t *__restrict__ coef_wide, uint32_t *__restrict__ slice_err,
-               uint32_t *__restrict__ pic_first_err, uint32_t persistent) {
+               uint32_t *__restrict__ pic_first_err, const uint32_t *__restrict__ order, uint32_t persistent) {
   // LINE_WORDS dwords a thread, then a 64-entry list a wave.
   extern __shared__ uint32_t lines[];
   uint4 *list = (uint4 *)(lines + LINE_WORDS * blockDim.x) + 64 * (threadIdx.x >> 6);
@@ -1081,7 +1136,16 @@ extern "C" __global__ void __launch_bounds__(ENTROPY_MAX_WG)
   const uint32_t n_items = B->n_entropy_items, n_slices = B->n_slices;
   const uint32_t lane = threadIdx.x & 63;
   if (!persistent) {
-    const uint32_t item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t b = blockIdx.x, w = threadIdx.x >> 6, ew = blockDim.x >> 6, nb = gridDim.x;
+    uint32_t item = b * ew + w;
+    if (order) {
+      // `order` holds the items longest first. Tier k of them (nb items)
+      // goes one to a workgroup, odd tiers in reverse, so every workgroup
+      // (one to a CU) gets one item of each tier; wave w takes tier
+      // (w mod 4) * ew/4 + w/4 (of the mappings tried, the fastest).
+      const uint32_t tier = (ew & 3) == 0 ? (w & 3) * (ew >> 2) + (w >> 2) : w;
+      item = order[tier * nb + ((tier & 1) ? nb - 1 - b : b)];
+    }
     const uint32_t g = (item / 3) * 64 + lane;
     if (item < n_items)  // the same for the whole wave
       entropy_stream(g < n_slices, g, item % 3, lines, list, &B->wide, ftab, frames, pics, slices, pic_status, coefs,
@@ -1093,12 +1157,90 @@ extern "C" __global__ void __launch_bounds__(ENTROPY_MAX_WG)
     const uint32_t first = wave_pop(&B->q[0], chunk);
     if (first >= n_items) break;
     const uint32_t last = first + chunk < n_items ? first + chunk : n_items;
-    for (uint32_t item = first; item < last; item++) {
+    for (uint32_t k = first; k < last; k++) {
+      const uint32_t item = order ? order[k] : k;  // longest first
       const uint32_t g = (item / 3) * 64 + lane;
       entropy_stream(g < n_slices, g, item % 3, lines, list, &B->wide, ftab, frames, pics, slices, pic_status, coefs,
                      coef_wide, slice_err, pic_first_err);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// One workgroup: the entropy items of the batch in order of cost, longest
+// first, into order[0, n_slots) (slots past the items hold indices past
+// them; n_slots 0: the batch's item count). An item's cost is its largest
+// slice in bytes (the slice table's sizes): a wave's time follows its
+// longest stream. A counting sort on cost / ORDER_BUCKET_BYTES (items of
+// one bucket in any order); the host launches this only for
+// n_slots <= ORDER_MAX.
+constexpr uint32_t ORDER_BUCKETS = 2048, ORDER_BUCKET_BYTES = 32;
+extern "C" __global__ void __launch_bounds__(1024)
+    pd_entropy_order(const DecBatch *__restrict__ B, const uint16_t *__restrict__ slice_size,
+                     uint32_t *__restrict__ order, uint32_t n_slots) {
+  __shared__ uint32_t cnt[ORDER_BUCKETS];
+  __shared__ uint16_t bucket[ORDER_MAX];
+  __shared__ uint32_t wsum[16];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t G = B->n_slices, groups = (G + 63) / 64, n_items = 3 * groups;
+  if (n_slots == 0) n_slots = n_items;
+  for (uint32_t i = tid; i < ORDER_BUCKETS; i += blockDim.x) cnt[i] = 0;
+  __syncthreads();
+  // Costs: eight lanes a group, eight sizes a lane, every load in flight
+  // together (past the batch: the last slice again, which only repeats a
+  // size).
+  constexpr uint32_t PER = 8;
+  const uint32_t chunks = groups * (64 / PER), stride = blockDim.x;
+  for (uint32_t c0 = 0; c0 < chunks; c0 += 4 * stride) {
+    uint32_t m[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) {
+      const uint32_t c = c0 + u * stride + tid;
+      m[u] = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < PER; j++) {
+        const uint32_t g = PER * c + j;
+        const uint32_t v = slice_size[g < G ? g : G - 1];
+        m[u] = v > m[u] ? v : m[u];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) {
+      uint32_t v = m[u];
+#pragma unroll
+      for (uint32_t o = 4; o; o >>= 1) {
+        const uint32_t x = __shfl_xor(v, o, 64);
+        v = x > v ? x : v;
+      }
+      const uint32_t c = c0 + u * stride + tid, it = 3 * (c / 8) + (lane & 7);
+      if ((lane & 7) < 3 && c < chunks) {
+        // Descending cost: bucket 0 holds the longest items.
+        const uint32_t b = ORDER_BUCKETS - 1 - (v / ORDER_BUCKET_BYTES < ORDER_BUCKETS - 1
+                                                    ? v / ORDER_BUCKET_BYTES : ORDER_BUCKETS - 1);
+        bucket[it] = (uint16_t)b;
+        atomicAdd(&cnt[b], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  // Exclusive scan of the counts, two buckets a thread.
+  const uint32_t c0 = cnt[2 * tid], c1 = cnt[2 * tid + 1];
+  uint32_t incl = c0 + c1;
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+  for (uint32_t w = 0; w < wave; w++) before += wsum[w];
+  const uint32_t ex = before + incl - c0 - c1;
+  __syncthreads();
+  cnt[2 * tid] = ex;
+  cnt[2 * tid + 1] = ex + c0;
+  __syncthreads();
+  for (uint32_t i = tid; i < n_items; i += blockDim.x) order[atomicAdd(&cnt[bucket[i]], 1u)] = i;
+  for (uint32_t i = n_items + tid; i < n_slots; i += blockDim.x) order[i] = i;
 }
 
 // ---------------------------------------------------------------------------

@@ -122,6 +122,11 @@ constexpr uint32_t HDR_FRAME_BYTES = 160, HDR_PIC_BYTES = 8, HDR_BLOB = 192;
 constexpr uint32_t HDR_PIC_AT = HDR_FRAME_BYTES, HDR_POS_AT = HDR_FRAME_BYTES + 2 * HDR_PIC_BYTES;
 static_assert(HDR_POS_AT + 16 == HDR_BLOB, "header record layout");
 
+// Entropy work placement: batches of at most ORDER_MAX grid slots have
+// their entropy items ordered by cost (the largest slice of the item's 64,
+// in bytes) so that every workgroup gets a mix of long and short streams.
+constexpr uint32_t ORDER_MAX = 8192;
+
 constexpr uint32_t IDCT_MAX_BLOCKS = 96;  // blocks one workgroup holds in LDS
 constexpr uint32_t IDCT_MAX_SLICES = 12;  // slices of one item (a slice is at least 8 blocks)
 // Entropy decode LDS: two 32-coefficient half-lines a lane (16-byte aligned),
