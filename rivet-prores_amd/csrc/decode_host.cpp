@@ -66,7 +66,7 @@ struct Batch {
 struct GraphKey {
   void *ptrs[16];
   unsigned grids[4];
-  unsigned ewg;
+  unsigned ewg, idct_lds;
   prores_decode_tuning t;
   bool alpha;
   bool operator==(const GraphKey &o) const { return memcmp(this, &o, sizeof *this) == 0; }
@@ -289,6 +289,11 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   if (G > 0xFFFFFFF0ull) return invalid("a batch of more than 2^32 slices");
   std::vector<IdctItem> segs;
   const uint64_t n_items = plan_segments(b, t.idct_blocks, segs);
+  // pd_idct's dynamic LDS: 128 dequantisation factors a slice of its
+  // largest item.
+  uint32_t max_count = 1;
+  for (const IdctItem &it : segs) max_count = it.count > max_count ? it.count : max_count;
+  const unsigned idct_lds = max_count * 128 * (unsigned)sizeof(float);
   DecBatch hdr;
   memset(&hdr, 0, sizeof hdr);
   hdr.n_pics = (uint32_t)np;
@@ -323,7 +328,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     grids[0] = resident_grid(d, d.k_slice_table, 256);
     grids[1] = resident_grid(d, d.k_entropy, ewg, ewaves * ENTROPY_LDS_PER_WAVE);
     grids[2] = resident_grid(d, d.k_alpha, 64);
-    grids[3] = resident_grid(d, d.k_idct, 256);
+    grids[3] = resident_grid(d, d.k_idct, 256, idct_lds);
   } else {
     grids[0] = (unsigned)np;
     grids[1] = (unsigned)((hdr.n_entropy_items + ewaves - 1) / ewaves);
@@ -377,7 +382,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     if (x.ok()) {
       if (timed) (void)hipEventRecord(L.ev[2], L.stream);
       void *args[] = {&tables, &B, &frames, &pics, &slices, &status, &serr, &coefs, &wide, &itemp, &pers};
-      x = launch(L, d.k_idct, grids[3], 1, 256, args);
+      x = launch(L, d.k_idct, grids[3], 1, 256, args, idct_lds);
       if (timed) (void)hipEventRecord(L.ev[3], L.stream);
     }
     // Whatever happened, the main stream waits for the alpha work before
@@ -398,6 +403,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     memcpy(key.ptrs, ptrs, sizeof ptrs);
     memcpy(key.grids, grids, sizeof grids);
     key.ewg = ewg;
+    key.idct_lds = idct_lds;
     key.t = t;
     key.alpha = b.any_alpha;
     if (!ws.graph || !(key == ws.key)) {

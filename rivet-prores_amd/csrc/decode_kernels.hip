@@ -1447,7 +1447,10 @@ extern "C" __global__ void __launch_bounds__(256)
             const int32_t *__restrict__ coef_wide, const IdctItem *__restrict__ items,
             uint32_t persistent) {
   __shared__ __attribute__((aligned(16))) int16_t sc[IDCT_MAX_BLOCKS * 64];
-  __shared__ __attribute__((aligned(16))) float s_scale[IDCT_MAX_SLICES * 128];
+  // 128 factors for each slice an item can hold: the launch sizes this
+  // (dynamic LDS) for the batch's largest item, so that one-slice items
+  // leave room for more workgroups a CU.
+  extern __shared__ __attribute__((aligned(16))) float s_scale[];
   __shared__ __attribute__((aligned(16))) float tmp[32 * IDCT_TMP];
   uint16_t *const outb = (uint16_t *)tmp;
   __shared__ __attribute__((aligned(16))) uint4 s_blk[IDCT_MAX_BLOCKS];
