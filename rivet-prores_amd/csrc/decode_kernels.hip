@@ -1373,26 +1373,28 @@ __device__ __forceinline__ void idct8(const float a[8], float o[8]) {
 
 template <bool WIDE>
 __device__ __forceinline__ void idct_pass(uint32_t t, uint32_t nblocks, uint32_t r, uint32_t lb,
-                                          const uint4 *s_blk, const int16_t *sc, const float *s_scale,
+                                          const uint2 *s_blk, const int16_t *sc, const float *s_scale,
                                           float *tmp, uint16_t *outb, const uint32_t s2[8],
                                           const int32_t *__restrict__ wide, const uint64_t *s_plane,
                                           const uint32_t *s_rstride, float osc, float obias, float omax) {
-  uint4 D = make_uint4(0, 0, 0, 0);
+  uint2 D = make_uint2(0, 0);
   if (t < nblocks) D = s_blk[t];
-  const uint32_t rows = D.w & 15;
+  // The packed descriptor (pd_idct): coefficient offset, rows, cols, plane,
+  // log2 of the component's blocks, slice within the item.
+  const uint32_t bx0 = D.x & 0x1FFF, rows = (D.x >> 13) & 15, cols = (D.x >> 17) & 15, c = (D.x >> 21) & 3;
+  const uint32_t lg = (D.x >> 23) & 7, fbase = 128 * (D.x >> 26) + (c ? 64u : 0u);
   const bool active = rows != 0;
   float o[8];
   if (active) {
     // Row pass, lane r: row r of the block, F = coefficient x (W q)/8.
-    const uint32_t lg = (D.w >> 12) & 15;
-    const float4 *sp = (const float4 *)(s_scale + D.y + 8 * r);
+    const float4 *sp = (const float4 *)(s_scale + fbase + 8 * r);
     const float4 f0 = sp[0], f1 = sp[1];
     const float fs[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
     float a[8];
 #pragma unroll
     for (uint32_t u = 0; u < 8; u++) {
       // Byte offset of coefficient nb * scan + block (s2 holds 2 scan).
-      const uint32_t at = 2 * D.x + (s2[u] << lg);
+      const uint32_t at = 2 * bx0 + (s2[u] << lg);
       int32_t v = *(const int16_t *)((const char *)sc + at);
       if (WIDE && v == -32768) v = wide[at / 2];
       a[u] = (float)v * fs[u];
@@ -1423,8 +1425,7 @@ __device__ __forceinline__ void idct_pass(uint32_t t, uint32_t nblocks, uint32_t
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (active && r < rows) {
     // Row r of the block to its plane (put_block, src/decode.rs:418-438).
-    const uint32_t c = (D.w >> 8) & 3, cols = (D.w >> 4) & 15;
-    uint16_t *o16 = (uint16_t *)(uintptr_t)s_plane[c] + D.z + (size_t)s_rstride[c] * r;
+    uint16_t *o16 = (uint16_t *)(uintptr_t)s_plane[c] + D.y + (size_t)s_rstride[c] * r;
     const uint4 pk = *(const uint4 *)(outb + lb * IDCT_OUT + 8 * r);
     if (cols == 8 && (((uintptr_t)o16) & 15) == 0) {
       *(uint4 *)o16 = pk;
@@ -1453,12 +1454,13 @@ extern "C" __global__ void __launch_bounds__(256)
   extern __shared__ __attribute__((aligned(16))) float s_scale[];
   __shared__ __attribute__((aligned(16))) float tmp[32 * IDCT_TMP];
   uint16_t *const outb = (uint16_t *)tmp;
-  __shared__ __attribute__((aligned(16))) uint4 s_blk[IDCT_MAX_BLOCKS];
-  __shared__ uint32_t s_start[IDCT_MAX_SLICES + 1], s_q[IDCT_MAX_SLICES], s_mbx[IDCT_MAX_SLICES],
-      s_mby[IDCT_MAX_SLICES], s_mbs[IDCT_MAX_SLICES], s_ok[IDCT_MAX_SLICES];
+  // Kept small: 23,040 bytes of LDS or less a workgroup leave room for
+  // seven workgroups a CU.
+  __shared__ __attribute__((aligned(16))) uint2 s_blk[IDCT_MAX_BLOCKS];
+  // Per slice: first block; mb_x | mb_y << 16; mbs | ok << 8 | qScale << 16.
+  __shared__ uint32_t s_start[IDCT_MAX_SLICES + 1], s_pos[IDCT_MAX_SLICES], s_mq[IDCT_MAX_SLICES];
   __shared__ uint64_t s_plane[3];
   __shared__ uint32_t s_rstride[3];
-  __shared__ uint8_t s_w[128], s_scan[64];
   __shared__ uint32_t s_item;
   const uint32_t tid = threadIdx.x, n_items = B->n_idct_items;
   const uint32_t chunk = persistent ? B->chunk : 1u;
@@ -1491,11 +1493,9 @@ extern "C" __global__ void __launch_bounds__(256)
         // offsets that may point anywhere: nothing of it is read. The alpha
         // word is not looked at: alpha is decoded beside this kernel.
         const bool ok = pic_ok && !(ev.x | ev.y | ev.z);
-        s_q[tid] = ok ? qscale(((const uint8_t *)S.off)[1]) : 0u;
-        s_mbx[tid] = S.mb_x;
-        s_mby[tid] = S.mb_y;
-        s_mbs[tid] = S.mbs;
-        s_ok[tid] = ok;
+        const uint32_t q = ok ? qscale(((const uint8_t *)S.off)[1]) : 0u;
+        s_pos[tid] = (uint32_t)S.mb_x | (uint32_t)S.mb_y << 16;
+        s_mq[tid] = S.mbs | (ok ? 1u : 0u) << 8 | q << 16;
         s_start[tid + 1] = (uint32_t)((S.coef - S0.coef) / 64) + S.mbs * (4 + 2 * cpm);
         if (tid == 0) s_start[0] = 0;
       }
@@ -1503,8 +1503,6 @@ extern "C" __global__ void __launch_bounds__(256)
         s_plane[tid - 32] = F.out_plane[tid - 32];
         s_rstride[tid - 32] = P.row_step * F.plane_w[tid - 32];
       }
-      if (tid >= 64 && tid < 192) s_w[tid - 64] = tid < 128 ? F.luma_w[tid - 64] : F.chroma_w[tid - 128];
-      if (tid >= 192) s_scan[tid - 192] = T->scan[F.interlaced ? 1 : 0][tid - 192];
       {
         const uint4 *src = (const uint4 *)(coefs + S0.coef);
         for (uint32_t i = tid; i < I.blocks * 8; i += 256) ((uint4 *)sc)[i] = src[i];
@@ -1513,7 +1511,7 @@ extern "C" __global__ void __launch_bounds__(256)
       // Dequantisation factors, scale[k] = (f32)(W[k] qScale) / 8 for each
       // slice and matrix (src/decode.rs:356).
       for (uint32_t e = tid; e < I.count * 128; e += 256)
-        s_scale[e] = (float)((uint32_t)s_w[e & 127] * s_q[e >> 7]) / 8.0f;
+        s_scale[e] = (float)((uint32_t)((e & 64) ? F.chroma_w[e & 63] : F.luma_w[e & 63]) * (s_mq[e >> 7] >> 16)) / 8.0f;
       // Block descriptors: x = its first coefficient in sc (the scanned
       // index is nb * scan + block), y = its factors in s_scale, z = its
       // top-left sample in the plane, w = rows | cols << 4 | plane << 8 |
@@ -1521,7 +1519,7 @@ extern "C" __global__ void __launch_bounds__(256)
       for (uint32_t t = tid; t < I.blocks; t += 256) {
         uint32_t j = 0;
         while (j + 1 < I.count && s_start[j + 1] <= t) j++;
-        const uint32_t mbs = s_mbs[j], nbY = 4 * mbs, nbC = cpm * mbs, tl = t - s_start[j];
+        const uint32_t mbs = s_mq[j] & 255, nbY = 4 * mbs, nbC = cpm * mbs, tl = t - s_start[j];
         uint32_t c = 0, b = tl, nb = nbY, cbase = 64 * s_start[j];
         if (tl >= nbY + nbC) {
           c = 2; b = tl - nbY - nbC; nb = nbC; cbase += 64 * (nbY + nbC);
@@ -1530,7 +1528,7 @@ extern "C" __global__ void __launch_bounds__(256)
         }
         // block_position (src/decode.rs:395-413).
         const uint32_t per_mb = c == 0 ? 4u : cpm;
-        const uint32_t mbx = s_mbx[j] + b / per_mb, bb = b % per_mb, mby = s_mby[j];
+        const uint32_t mbx = (s_pos[j] & 0xFFFF) + b / per_mb, bb = b % per_mb, mby = s_pos[j] >> 16;
         uint32_t bx, by;
         if (c == 0) {
           bx = 16 * mbx + 8 * (bb & 1); by = 16u * mby + 8 * (bb >> 1);
@@ -1541,19 +1539,19 @@ extern "C" __global__ void __launch_bounds__(256)
         }
         const uint32_t pw = F.plane_w[c];
         uint32_t rows = 0, cols = 0, out = 0;
-        if (s_ok[j] && bx < pw && by < P.height) {
+        if (((s_mq[j] >> 8) & 1) && bx < pw && by < P.height) {
           rows = P.height - by < 8 ? P.height - by : 8u;
           cols = pw - bx < 8 ? pw - bx : 8u;
           out = (P.first_row + P.row_step * by) * pw + bx;
         }
         const uint32_t lg = (uint32_t)__builtin_ctz(nb);  // nb is a power of two
-        s_blk[t] = make_uint4(cbase + b, 128 * j + (c ? 64u : 0u), out, rows | cols << 4 | c << 8 | lg << 12);
+        s_blk[t] = make_uint2((cbase + b) | rows << 13 | cols << 17 | c << 21 | lg << 23 | j << 26, out);
       }
       __syncthreads();
       const uint32_t r = tid & 7, lb = tid >> 3;
       uint32_t s2[8];
 #pragma unroll
-      for (uint32_t u = 0; u < 8; u++) s2[u] = 2u * s_scan[8 * r + u];
+      for (uint32_t u = 0; u < 8; u++) s2[u] = 2u * T->scan[F.interlaced ? 1 : 0][8 * r + u];
       const float osc = F.out_scale, obias = F.out_bias, omax = F.out_max;
       const int32_t *wide = coef_wide + S0.coef;
       for (uint32_t t0 = 0; t0 < I.blocks; t0 += 32) {
