@@ -181,3 +181,29 @@ def test_jobs_without_outputs_fail_alone(batch):
     assert [type(e) for e in errs[:3]] == [P.ConfigError] * 3, errs
     assert errs[3] is None
     np.testing.assert_array_equal(good[1].download(good[3] * 2, dtype=np.uint16), want[2][1].data)
+
+
+@pytest.mark.parametrize("persistent", (0, 1))
+def test_batch_past_the_ordering_limit(persistent):
+    """22 copies of a 1920x1080 Proxy frame with 1-MB slices: 179,520 slices,
+    8,415 entropy items, more than the entropy ordering sorts (the items then
+    go in their natural order). Every copy must decode to the oracle's
+    samples."""
+    print("case: batch past the ordering limit, persistent", persistent, flush=True)
+    f = O.test_frame(1920, 1080, 2, 10, 12, 3)
+    packet = O.encode(f, O.config(0, log2_slice_mbs=0))
+    want = O.decode(packet)
+    n = 22
+    h = P.FrameHeader.parse(packet)
+    planes = P.layout(h.width, h.height, h.chroma)
+    total = planes[2][0] + planes[2][1] * planes[2][2]
+    dp = P.DeviceBuffer(len(packet) + 128)
+    dp.upload(np.frombuffer(packet, np.uint8))
+    outs = [P.DeviceBuffer(total * 2 + 64) for _ in range(n)]
+    for o in outs:
+        o.upload(np.full(total, 0xABCD, np.uint16))
+    jobs = P.DecodeJobs([(packet, dp.ptr, o.ptr, 0) for o in outs])
+    errs = P.Decoder().decode_device_batch(jobs, P.DecodeTuning(persistent, 4 if persistent else 1, 96, 0, 0))
+    assert errs == [None] * n, errs
+    for i, o in enumerate(outs):
+        np.testing.assert_array_equal(o.download(total * 2, dtype=np.uint16), want.data, err_msg=f"copy {i}")
