@@ -314,7 +314,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   if (!(e = ws.first_err.reserve(np * 4 + 64)).ok()) return e;
   if (!(e = ws.coefs.reserve(b.n_coefs * 2 + 64)).ok()) return e;
   if (!(e = ws.wide.reserve(b.n_coefs * 4 + 64)).ok()) return e;
-  if (!(e = ws.alpha_order.reserve(G * 4 + 64)).ok()) return e;
+  if (!(e = ws.alpha_order.reserve(G * sizeof(AlphaRec) + 64)).ok()) return e;
   if (!(e = ws.slice_size.reserve(G * 2 + 64)).ok()) return e;
 
   void *B = ws.hdr.p, *pics = ws.pics.p, *rows = ws.rows.p, *slices = ws.slices.p, *status = ws.status.p;

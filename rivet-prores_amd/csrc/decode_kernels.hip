@@ -745,38 +745,22 @@ __device__ __forceinline__ void alpha_fill(uint16_t *vals, const uint32_t *marks
 }
 
 __device__ __forceinline__ void alpha_slice(
-    uint32_t g, uint64_t *__restrict__ words, uint16_t *__restrict__ vals, uint32_t *__restrict__ marks,
-    const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics,
-    const SliceRec *__restrict__ slices, const uint32_t *__restrict__ pic_status,
+    const AlphaRec &R, uint64_t *__restrict__ words, uint16_t *__restrict__ vals, uint32_t *__restrict__ marks,
+    const DecFrame *__restrict__ frames, const DecPic *__restrict__ pics, const uint32_t *__restrict__ pic_status,
     uint32_t *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err) {
   const uint32_t L = threadIdx.x;
-  g = uni(g);
-  const SliceRec S = slices[g];
-  if (uni(pic_status[S.pic]) != PIC_OK) return;
-  const DecPic &P = pics[S.pic];
-  const DecFrame &F = frames[P.frame];
+  const uint32_t g = uni(R.g), pic = uni(R.pic);
+  if (uni(pic_status[pic]) != PIC_OK) return;
+  const DecPic &P = pics[pic];
+  const DecFrame &F = frames[uni(R.frame)];
   const uint32_t bits = uni(F.alpha_bits);
-  if (bits == 0) return;
-  const uint8_t *s = (const uint8_t *)S.off;
-  const uint32_t slen = uni(S.len);
-  uint32_t hs, ysz, cbsz, crsz;
-  uint32_t err = uni(slice_header(s, slen, bits, hs, ysz, cbsz, crsz));
-  hs = uni(hs);
-  ysz = uni(ysz);
-  cbsz = uni(cbsz);
-  crsz = uni(crsz);
-  if (err) {
-    if (L == 0) {
-      slice_err[4 * (size_t)g + 3] = err;
-      atomicMin(&pic_first_err[S.pic], g - P.slice_base);
-    }
-    return;
-  }
+  const uint32_t mb_x = uni(R.mb) & 0xFFFF, mb_y = uni(R.mb) >> 16, mbs = uni(R.mbs);
+  uint32_t err = E_NONE;
   // Stage the stream: aligned words from base - m, bytes outside the stream
   // zeroed, and two zero words after it (the reader holds the word after its
   // current one, and never passes the end of the stream).
-  const uint32_t used = hs + ysz + cbsz + crsz, len = slen - used;
-  const uintptr_t a = (uintptr_t)(s + used);
+  const uint32_t len = uni(R.len);
+  const uintptr_t a = (uintptr_t)(((uint64_t)uni((uint32_t)(R.stream >> 32)) << 32) | uni((uint32_t)R.stream));
   const uint32_t m = uni((uint32_t)(a & 7));
   const uint64_t *src = (const uint64_t *)(a - m);
   const uint32_t end_byte = m + len;  // bytes of stream data end here
@@ -798,10 +782,10 @@ __device__ __forceinline__ void alpha_slice(
   // registers (each staged word is read as a wave-uniform value), and the
   // vector lanes only fill runs.
   const uint32_t mag = bits == 8 ? 3u : 6u, mask = (1u << bits) - 1;
-  const uint32_t log2c = uni(4 + (31 - __builtin_clz(S.mbs)));
+  const uint32_t log2c = 4 + (31 - __builtin_clz(mbs));
   const uint32_t cols = 1u << log2c, count = cols * 16;
   const uint32_t rows =
-      uni((uint32_t)S.mb_y + 1 < P.height_in_mb ? 16u : P.height - 16u * (P.height_in_mb - 1));
+      uni(mb_y + 1 < P.height_in_mb ? 16u : P.height - 16u * (P.height_in_mb - 1));
   const uint32_t needed = cols * rows;
   const uint32_t ident = uni(F.rescale_identity);
   const uint64_t rm = ((uint64_t)uni((uint32_t)(F.rescale_m >> 32)) << 32) | uni((uint32_t)F.rescale_m);
@@ -934,17 +918,17 @@ __device__ __forceinline__ void alpha_slice(
   if (err) {
     if (L == 0) {
       slice_err[4 * (size_t)g + 3] = err;
-      atomicMin(&pic_first_err[S.pic], g - P.slice_base);
+      atomicMin(&pic_first_err[pic], g - P.slice_base);
     }
     return;
   }
   if (par) alpha_fill(vals, marks);
   __syncthreads();
   // Visible rows, clipped to the plane width.
-  const uint32_t x0 = 16u * S.mb_x, width = F.width;
+  const uint32_t x0 = 16u * mb_x, width = F.width;
   const uint32_t nvis = cols < width - x0 ? cols : width - x0;
   uint16_t *aout = (uint16_t *)F.out_alpha;
-  const size_t row0 = P.first_row + (size_t)P.row_step * (16u * S.mb_y);
+  const size_t row0 = P.first_row + (size_t)P.row_step * (16u * mb_y);
   const size_t rstride = (size_t)P.row_step * width;
   if ((width & 7) == 0 && (((uintptr_t)aout) & 15) == 0) {
     // x0 and nvis are multiples of 8: 16-byte stores.
@@ -979,7 +963,7 @@ extern "C" __global__ void __launch_bounds__(256)
                    SliceRec *__restrict__ slices, uint32_t *__restrict__ pic_status,
                    uint4 *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err,
                    const IdctItem *__restrict__ segs, IdctItem *__restrict__ items,
-                   uint32_t *__restrict__ alpha_order, uint16_t *__restrict__ slice_size) {
+                   AlphaRec *__restrict__ alpha_order, uint16_t *__restrict__ slice_size) {
   const uint32_t tid = threadIdx.x;
   if (blockIdx.x == 0 && tid < 4) B->q[tid] = 0;
   const uint32_t np = B->n_pics, G = B->n_slices;
@@ -1032,27 +1016,36 @@ extern "C" __global__ void __launch_bounds__(256)
         excl += sz[j];
         rs[j] = rows[P.layout_base + (kk - (kk / P.row_slices) * P.row_slices)];
       }
-      // The alpha work order: slices with a long alpha stream at the front,
-      // the rest from the back (a slice outside the picture reads nothing).
-      uint32_t heavy = 0;
+      // The alpha work order: the slices of frames with alpha whose header
+      // passes slice_header's checks (the colour decode reports the others),
+      // long alpha streams at the front, the rest from the back; each with
+      // where its alpha stream is. A slice outside the picture reads nothing.
+      uint32_t heavy = 0, listed = 0, alen[ST_PER];
+      uint64_t astream[ST_PER];
       if (has_alpha) {
 #pragma unroll
         for (uint32_t j0 = 0; j0 < ST_PER; j0 += 4) {
-          uint32_t hb0[4], comp[4];
+          uint32_t hb0[4], qb[4], comp[4];
           bool inside[4];
 #pragma unroll
           for (uint32_t j = j0; j < j0 + 4; j++) {
             inside[j - j0] = ((valid >> j) & 1) && sz[j] >= 8 && data0 + at[j] + sz[j] <= P.picture_size;
             const uint8_t *sl = (const uint8_t *)(P.pic_off + (inside[j - j0] ? data0 + at[j] : 0));
             hb0[j - j0] = sl[0];
+            qb[j - j0] = sl[1];
             comp[j - j0] = ((uint32_t)sl[2] << 8 | sl[3]) + ((uint32_t)sl[4] << 8 | sl[5]) +
                            ((uint32_t)sl[6] << 8 | sl[7]);
           }
 #pragma unroll
           for (uint32_t j = j0; j < j0 + 4; j++) {
-            const uint32_t hs = hb0[j - j0] >> 3, c = comp[j - j0];
-            if (inside[j - j0] && hs <= sz[j] && c <= sz[j] - hs && sz[j] - hs - c > ALPHA_HEAVY_BYTES)
-              heavy |= 1u << j;
+            // slice_header with alpha: 8 <= hs <= size, 1 <= q <= 224, the
+            // component sizes inside the body.
+            const uint32_t hs = hb0[j - j0] >> 3, q = qb[j - j0], c = comp[j - j0];
+            const bool ok = inside[j - j0] && hs >= 8 && hs <= sz[j] && q >= 1 && q <= 224 && c <= sz[j] - hs;
+            alen[j] = ok ? sz[j] - hs - c : 0u;
+            astream[j] = P.pic_off + data0 + at[j] + hs + c;
+            if (ok) listed |= 1u << j;
+            if (ok && alen[j] > ALPHA_HEAVY_BYTES) heavy |= 1u << j;
           }
         }
       }
@@ -1074,7 +1067,7 @@ extern "C" __global__ void __launch_bounds__(256)
           slice_size[P.slice_base + k] = (uint16_t)sz[j];
         }
       }
-      const uint32_t nh = (uint32_t)__popc(heavy), nl = (uint32_t)__popc(valid & ~heavy);
+      const uint32_t nh = (uint32_t)__popc(heavy), nl = (uint32_t)__popc(listed & ~heavy);
       uint32_t hi = nh, li = nl;
       for (uint32_t d = 1; d < 64; d <<= 1) {
         const uint32_t a = __shfl_up(hi, d, 64), b = __shfl_up(li, d, 64);
@@ -1092,11 +1085,16 @@ extern "C" __global__ void __launch_bounds__(256)
       lb = __shfl(lb, 63, 64) + li - nl;
 #pragma unroll
       for (uint32_t j = 0; j < ST_PER; j++) {
-        if ((valid >> j) & 1) {
-          if ((heavy >> j) & 1)
-            alpha_order[hb++] = P.slice_base + k0 + j;
-          else
-            alpha_order[G - 1 - lb++] = P.slice_base + k0 + j;
+        if ((listed >> j) & 1) {
+          AlphaRec ar;
+          ar.stream = astream[j];
+          ar.g = P.slice_base + k0 + j;
+          ar.len = alen[j];
+          ar.pic = p;
+          ar.frame = P.frame;
+          ar.mb = (uint32_t)rs[j].mb_x | ((k0 + j) / P.row_slices) << 16;
+          ar.mbs = rs[j].mbs;
+          alpha_order[(heavy >> j) & 1 ? hb++ : G - 1 - lb++] = ar;
         }
       }
       __syncthreads();
@@ -1258,15 +1256,17 @@ extern "C" __global__ void __launch_bounds__(64)
     pd_alpha(DecBatch *__restrict__ B, const DecFrame *__restrict__ frames,
              const DecPic *__restrict__ pics, const SliceRec *__restrict__ slices,
              const uint32_t *__restrict__ pic_status, uint32_t *__restrict__ slice_err,
-             uint32_t *__restrict__ pic_first_err, const uint32_t *__restrict__ order,
+             uint32_t *__restrict__ pic_first_err, const AlphaRec *__restrict__ order,
              uint32_t persistent) {
   __shared__ uint64_t words[ALPHA_WORDS];
   __shared__ __attribute__((aligned(16))) uint16_t vals[2048];
   __shared__ uint32_t marks[64];
-  const uint32_t n = B->n_slices;
+  // The order holds `heavy` records at the front and `light` at the back.
+  const uint32_t G = B->n_slices, heavy = B->heavy, n = heavy + B->light;
   if (!persistent) {
-    if (blockIdx.x < n)
-      alpha_slice(order[blockIdx.x], words, vals, marks, frames, pics, slices, pic_status, slice_err, pic_first_err);
+    const uint32_t x = blockIdx.x;
+    if (x < heavy || (x < G && x >= G - B->light))
+      alpha_slice(order[x], words, vals, marks, frames, pics, pic_status, slice_err, pic_first_err);
     return;
   }
   const uint32_t chunk = B->chunk;
@@ -1275,7 +1275,8 @@ extern "C" __global__ void __launch_bounds__(64)
     if (first >= n) break;
     const uint32_t last = first + chunk < n ? first + chunk : n;
     for (uint32_t i = first; i < last; i++) {
-      alpha_slice(order[i], words, vals, marks, frames, pics, slices, pic_status, slice_err, pic_first_err);
+      alpha_slice(order[i < heavy ? i : G - 1 - (i - heavy)], words, vals, marks, frames, pics, pic_status, slice_err,
+                  pic_first_err);
       __syncthreads();
     }
   }

@@ -91,6 +91,21 @@ struct SliceRec {
   uint32_t mbs;
 };
 
+// One slice of the alpha work order (pd_slice_table, read by pd_alpha): its
+// alpha stream and what it belongs to, so that the alpha decode starts from
+// one load. Only slices of frames with alpha whose header passed its checks
+// are listed; the colour decode reports the others' header errors.
+struct AlphaRec {
+  uint64_t stream;   // device address of the alpha stream
+  uint32_t g;        // the slice (its SliceRec and error words)
+  uint32_t len;      // stream bytes
+  uint32_t pic;      // DecPic index
+  uint32_t frame;    // DecFrame index
+  uint32_t mb;       // mb_x | mb_y << 16
+  uint32_t mbs;      // macroblocks
+};
+static_assert(sizeof(AlphaRec) == 32, "AlphaRec layout");
+
 // Counts of a batch and the heads of its work queues, in device memory so
 // that the launch arguments stay the same from one batch to the next (a
 // captured graph replays unchanged). The host uploads it before every
