@@ -72,9 +72,33 @@ struct GraphKey {
   bool operator==(const GraphKey &o) const { return memcmp(this, &o, sizeof *this) == 0; }
 };
 
+// Page-locked host memory, grow-only: a lane's staging for the batch plan
+// going up and the per-picture results coming back, so that each is one
+// asynchronous copy.
+struct HostBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  Err reserve(size_t bytes) {
+    if (bytes <= cap) return Err{};
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = bytes + bytes / 4 + 4096;
+    const hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) return device_error("hipHostMalloc", e);
+    cap = want;
+    return Err{};
+  }
+};
+
 struct Workspace {
-  DevBuf frames, pics, rows, slices, status, first_err, slice_err, coefs, wide, packet, planes, hdr, items, segs,
-      alpha_order, gather, records, slice_size, order;
+  DevBuf slices, slice_err, coefs, wide, packet, planes, items, alpha_order, gather, records, slice_size, order;
+  // The batch plan (frames, pictures, row slots, IDCT segments, header) in
+  // one device buffer, each part at a fixed offset while it fits its
+  // capacity; the per-picture status and first-error words in another.
+  DevBuf plan, picres;
+  HostBuf hplan, hres;
+  size_t cap_frames = 0, cap_pics = 0, cap_rows = 0, cap_segs = 0, cap_np = 0;
   hipGraphExec_t graph = nullptr;
   GraphKey key;
 };
@@ -301,25 +325,45 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   hdr.n_entropy_items = (uint32_t)(3 * ((G + 63) / 64));
   hdr.n_idct_items = (uint32_t)n_items;
   hdr.chunk = t.persistent ? t.chunk : 1;
-  std::vector<DecBatch> hv{hdr};
-  if (!(e = upload(L, ws.frames, b.frames)).ok()) return e;
-  if (!(e = upload(L, ws.pics, b.pics)).ok()) return e;
-  if (!(e = upload(L, ws.rows, b.rows)).ok()) return e;
-  if (!(e = upload(L, ws.segs, segs)).ok()) return e;
-  if (!(e = upload(L, ws.hdr, hv)).ok()) return e;
+  // The plan goes up in one copy from page-locked memory.
+  auto fit = [](size_t &cap, size_t need) {
+    if (need > cap) cap = (need + need / 4 + 4095) & ~(size_t)4095;
+  };
+  fit(ws.cap_frames, b.frames.size() * sizeof(DecFrame));
+  fit(ws.cap_pics, b.pics.size() * sizeof(DecPic));
+  fit(ws.cap_rows, b.rows.size() * sizeof(RowSlot));
+  fit(ws.cap_segs, segs.size() * sizeof(IdctItem));
+  const size_t o_pics = ws.cap_frames, o_rows = o_pics + ws.cap_pics, o_segs = o_rows + ws.cap_rows,
+               o_hdr = o_segs + ws.cap_segs, plan_bytes = o_hdr + sizeof(DecBatch);
+  if (!(e = ws.plan.reserve(plan_bytes)).ok()) return e;
+  if (!(e = ws.hplan.reserve(plan_bytes)).ok()) return e;
+  {
+    uint8_t *h = (uint8_t *)ws.hplan.p;
+    if (!b.frames.empty()) memcpy(h, b.frames.data(), b.frames.size() * sizeof(DecFrame));
+    if (!b.pics.empty()) memcpy(h + o_pics, b.pics.data(), b.pics.size() * sizeof(DecPic));
+    if (!b.rows.empty()) memcpy(h + o_rows, b.rows.data(), b.rows.size() * sizeof(RowSlot));
+    if (!segs.empty()) memcpy(h + o_segs, segs.data(), segs.size() * sizeof(IdctItem));
+    memcpy(h + o_hdr, &hdr, sizeof hdr);
+    if (!(e = check(hipMemcpyAsync(ws.plan.p, ws.hplan.p, plan_bytes, hipMemcpyHostToDevice, L.stream),
+                    "hipMemcpyAsync(plan)")).ok())
+      return e;
+  }
+  fit(ws.cap_np, np * 4);
+  if (!(e = ws.picres.reserve(2 * ws.cap_np)).ok()) return e;
+  if (!(e = ws.hres.reserve(2 * ws.cap_np)).ok()) return e;
   if (!(e = ws.items.reserve(n_items * sizeof(IdctItem) + 64)).ok()) return e;
   if (!(e = ws.slices.reserve(G * sizeof(SliceRec) + 64)).ok()) return e;
   if (!(e = ws.slice_err.reserve(G * 16 + 64)).ok()) return e;
-  if (!(e = ws.status.reserve(np * 4 + 64)).ok()) return e;
-  if (!(e = ws.first_err.reserve(np * 4 + 64)).ok()) return e;
   if (!(e = ws.coefs.reserve(b.n_coefs * 2 + 64)).ok()) return e;
   if (!(e = ws.wide.reserve(b.n_coefs * 4 + 64)).ok()) return e;
   if (!(e = ws.alpha_order.reserve(G * sizeof(AlphaRec) + 64)).ok()) return e;
   if (!(e = ws.slice_size.reserve(G * 2 + 64)).ok()) return e;
 
-  void *B = ws.hdr.p, *pics = ws.pics.p, *rows = ws.rows.p, *slices = ws.slices.p, *status = ws.status.p;
-  void *frames = ws.frames.p, *tables = d.tables, *coefs = ws.coefs.p, *wide = ws.wide.p;
-  void *serr = ws.slice_err.p, *ferr = ws.first_err.p, *itemp = ws.items.p, *segp = ws.segs.p;
+  uint8_t *const pl = (uint8_t *)ws.plan.p;
+  void *B = pl + o_hdr, *pics = pl + o_pics, *rows = pl + o_rows, *slices = ws.slices.p, *status = ws.picres.p;
+  void *frames = pl, *tables = d.tables, *coefs = ws.coefs.p, *wide = ws.wide.p;
+  void *serr = ws.slice_err.p, *ferr = (uint8_t *)ws.picres.p + ws.cap_np, *itemp = ws.items.p,
+       *segp = pl + o_segs;
   void *aord = ws.alpha_order.p, *ssz = ws.slice_size.p;
   uint32_t pers = t.persistent;
   const unsigned ewg = entropy_wg(d, t, hdr.n_entropy_items), ewaves = ewg / 64;
@@ -436,16 +480,12 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     if (!(e = enqueue(true)).ok()) return e;
     (void)hipEventRecord(L.ev[4], L.stream);
   }
-  std::vector<uint32_t> st(np), fe(np);
-  if (np) {
-    if (!(e = check(hipMemcpyAsync(st.data(), ws.status.p, np * 4, hipMemcpyDeviceToHost, L.stream),
-                    "status")).ok())
-      return e;
-    if (!(e = check(hipMemcpyAsync(fe.data(), ws.first_err.p, np * 4, hipMemcpyDeviceToHost, L.stream),
-                    "first_err")).ok())
-      return e;
-  }
+  // Status and first-error words come back in one copy.
+  if (np && !(e = check(hipMemcpyAsync(ws.hres.p, ws.picres.p, 2 * ws.cap_np, hipMemcpyDeviceToHost, L.stream),
+                        "hipMemcpyAsync(results)")).ok())
+    return e;
   if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
+  const uint32_t *st = (const uint32_t *)ws.hres.p, *fe = st + ws.cap_np / 4;
   if (np) {
     // Stages: slice table, colour entropy, alpha (from the end of the slice
     // table to its own end, beside the other two), IDCT, whole batch.
