@@ -64,7 +64,7 @@ struct Batch {
 // arguments are workspace addresses and launch shapes, so it replays while
 // those stay the same.
 struct GraphKey {
-  void *ptrs[16];
+  void *ptrs[17];
   unsigned grids[4];
   unsigned ewg, idct_lds;
   prores_decode_tuning t;
@@ -92,13 +92,13 @@ struct HostBuf {
 };
 
 struct Workspace {
-  DevBuf slices, slice_err, coefs, wide, packet, planes, items, alpha_order, gather, records, slice_size, order;
+  DevBuf slices, slice_err, coefs, wide, packet, planes, items, alpha_order, gather, records, order;
   // The batch plan (frames, pictures, row slots, IDCT segments, header) in
   // one device buffer, each part at a fixed offset while it fits its
   // capacity; the per-picture status and first-error words in another.
   DevBuf plan, picres;
   HostBuf hplan, hres;
-  size_t cap_frames = 0, cap_pics = 0, cap_rows = 0, cap_segs = 0, cap_np = 0;
+  size_t cap_frames = 0, cap_pics = 0, cap_rows = 0, cap_segs = 0, cap_chunks = 0, cap_cost = 0, cap_np = 0;
   hipGraphExec_t graph = nullptr;
   GraphKey key;
 };
@@ -325,6 +325,18 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   hdr.n_entropy_items = (uint32_t)(3 * ((G + 63) / 64));
   hdr.n_idct_items = (uint32_t)n_items;
   hdr.chunk = t.persistent ? t.chunk : 1;
+  // pd_slice_table's chunks: (picture, first slice), ST_CHUNK slices each,
+  // at least one a picture.
+  std::vector<uint32_t> chunks;
+  for (size_t p = 0; p < np; p++) {
+    const uint32_t n = b.pics[p].n_slices;
+    for (uint32_t k = 0; k == 0 || k < n; k += ST_CHUNK) {
+      chunks.push_back((uint32_t)p);
+      chunks.push_back(k);
+    }
+  }
+  if (chunks.size() / 2 > 0xFFFFFFFFull) return invalid("a batch of more than 2^32 slice table chunks");
+  hdr.n_chunks = (uint32_t)(chunks.size() / 2);
   // The plan goes up in one copy from page-locked memory.
   auto fit = [](size_t &cap, size_t need) {
     if (need > cap) cap = (need + need / 4 + 4095) & ~(size_t)4095;
@@ -333,8 +345,13 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   fit(ws.cap_pics, b.pics.size() * sizeof(DecPic));
   fit(ws.cap_rows, b.rows.size() * sizeof(RowSlot));
   fit(ws.cap_segs, segs.size() * sizeof(IdctItem));
+  fit(ws.cap_chunks, chunks.size() * sizeof(uint32_t));
+  // Entropy item costs, one word a group of 64 slices, zero at upload.
+  const size_t cost_bytes = (size_t)((G + 63) / 64) * 4;
+  fit(ws.cap_cost, cost_bytes);
   const size_t o_pics = ws.cap_frames, o_rows = o_pics + ws.cap_pics, o_segs = o_rows + ws.cap_rows,
-               o_hdr = o_segs + ws.cap_segs, plan_bytes = o_hdr + sizeof(DecBatch);
+               o_chunks = o_segs + ws.cap_segs, o_cost = o_chunks + ws.cap_chunks, o_hdr = o_cost + ws.cap_cost,
+               plan_bytes = o_hdr + sizeof(DecBatch);
   if (!(e = ws.plan.reserve(plan_bytes)).ok()) return e;
   if (!(e = ws.hplan.reserve(plan_bytes)).ok()) return e;
   {
@@ -343,6 +360,8 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     if (!b.pics.empty()) memcpy(h + o_pics, b.pics.data(), b.pics.size() * sizeof(DecPic));
     if (!b.rows.empty()) memcpy(h + o_rows, b.rows.data(), b.rows.size() * sizeof(RowSlot));
     if (!segs.empty()) memcpy(h + o_segs, segs.data(), segs.size() * sizeof(IdctItem));
+    if (!chunks.empty()) memcpy(h + o_chunks, chunks.data(), chunks.size() * sizeof(uint32_t));
+    memset(h + o_cost, 0, cost_bytes);
     memcpy(h + o_hdr, &hdr, sizeof hdr);
     if (!(e = check(hipMemcpyAsync(ws.plan.p, ws.hplan.p, plan_bytes, hipMemcpyHostToDevice, L.stream),
                     "hipMemcpyAsync(plan)")).ok())
@@ -357,24 +376,23 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   if (!(e = ws.coefs.reserve(b.n_coefs * 2 + 64)).ok()) return e;
   if (!(e = ws.wide.reserve(b.n_coefs * 4 + 64)).ok()) return e;
   if (!(e = ws.alpha_order.reserve(G * sizeof(AlphaRec) + 64)).ok()) return e;
-  if (!(e = ws.slice_size.reserve(G * 2 + 64)).ok()) return e;
 
   uint8_t *const pl = (uint8_t *)ws.plan.p;
   void *B = pl + o_hdr, *pics = pl + o_pics, *rows = pl + o_rows, *slices = ws.slices.p, *status = ws.picres.p;
   void *frames = pl, *tables = d.tables, *coefs = ws.coefs.p, *wide = ws.wide.p;
   void *serr = ws.slice_err.p, *ferr = (uint8_t *)ws.picres.p + ws.cap_np, *itemp = ws.items.p,
-       *segp = pl + o_segs;
-  void *aord = ws.alpha_order.p, *ssz = ws.slice_size.p;
+       *segp = pl + o_segs, *chp = pl + o_chunks;
+  void *aord = ws.alpha_order.p, *costp = pl + o_cost;
   uint32_t pers = t.persistent;
   const unsigned ewg = entropy_wg(d, t, hdr.n_entropy_items), ewaves = ewg / 64;
   unsigned grids[4];
   if (t.persistent) {
-    grids[0] = resident_grid(d, d.k_slice_table, 256);
+    grids[0] = resident_grid(d, d.k_slice_table, ST_CHUNK);
     grids[1] = resident_grid(d, d.k_entropy, ewg, ewaves * ENTROPY_LDS_PER_WAVE);
     grids[2] = resident_grid(d, d.k_alpha, 64);
     grids[3] = resident_grid(d, d.k_idct, 256, idct_lds);
   } else {
-    grids[0] = (unsigned)np;
+    grids[0] = hdr.n_chunks;
     grids[1] = (unsigned)((hdr.n_entropy_items + ewaves - 1) / ewaves);
     grids[2] = (unsigned)G;
     grids[3] = (unsigned)n_items;
@@ -393,35 +411,37 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     Err x;
     if (timed) (void)hipEventRecord(L.ev[0], L.stream);
     {
-      void *args[] = {&B, &frames, &pics, &rows, &slices, &status, &serr, &ferr, &segp, &itemp, &aord, &ssz};
-      if (!(x = launch(L, d.k_slice_table, grids[0], 1, 256, args)).ok()) return x;
+      void *args[] = {&B, &frames, &pics, &rows, &chp, &slices, &status, &serr, &ferr, &segp, &itemp, &aord, &costp};
+      if (!(x = launch(L, d.k_slice_table, grids[0], 1, ST_CHUNK, args)).ok()) return x;
     }
     if (ordp) {
-      void *args[] = {&B, &ssz, &ordp, &n_slots};
+      void *args[] = {&B, &costp, &ordp, &n_slots};
       if (!(x = launch(L, d.k_order, 1, 1, 1024, args)).ok()) return x;
     }
     if (timed) (void)hipEventRecord(L.ev[1], L.stream);
-    // Alpha needs only the slice table: it runs on the second stream beside
-    // the colour entropy decode and the IDCT, which leave most of the
-    // machine's wave slots free, and joins the main stream at the end.
-    bool forked = false;
+    // Alpha needs only the slice table: the second stream waits for the work
+    // so far and decodes alpha beside the colour entropy decode and the
+    // IDCT, which leave most of the machine's wave slots free. The entropy
+    // decode is launched first and its workgroups (most of a CU's LDS each)
+    // are placed before the alpha workgroups.
+    bool forked = false, joined = false;
     if (b.any_alpha) {
       if (!(x = check(hipEventRecord(L.fork, L.stream), "hipEventRecord")).ok()) return x;
       if (!(x = check(hipStreamWaitEvent(L.aux, L.fork, 0), "hipStreamWaitEvent")).ok()) return x;
       forked = true;
-      void *args[] = {&B, &frames, &pics, &slices, &status, &serr, &ferr, &aord, &pers};
-      x = launch_on(L.aux, d.k_alpha, grids[2], 1, 64, args);
-      if (timed) (void)hipEventRecord(L.ev[5], L.aux);
-      Err j = check(hipEventRecord(L.join, L.aux), "hipEventRecord");
-      if (!j.ok()) return j;  // the stream joins nothing: fail before more work
-      if (!x.ok()) {
-        (void)hipStreamWaitEvent(L.stream, L.join, 0);
-        return x;
-      }
     }
     {
       void *args[] = {&tables, &B, &frames, &pics, &slices, &status, &coefs, &wide, &serr, &ferr, &ordp, &pers};
       x = launch(L, d.k_entropy, grids[1], 1, ewg, args, ewaves * ENTROPY_LDS_PER_WAVE);
+    }
+    if (x.ok() && forked) {
+      void *args[] = {&B, &frames, &pics, &slices, &status, &serr, &ferr, &aord, &pers};
+      const Err a = launch_on(L.aux, d.k_alpha, grids[2], 1, 64, args);
+      if (timed) (void)hipEventRecord(L.ev[5], L.aux);
+      Err j = check(hipEventRecord(L.join, L.aux), "hipEventRecord");
+      if (!j.ok()) return j;  // the stream joins nothing: fail before more work
+      joined = true;
+      x = a;
     }
     if (x.ok()) {
       if (timed) (void)hipEventRecord(L.ev[2], L.stream);
@@ -431,7 +451,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     }
     // Whatever happened, the main stream waits for the alpha work before
     // anything else on it can reuse the workspace.
-    if (forked) {
+    if (joined) {
       Err j = check(hipStreamWaitEvent(L.stream, L.join, 0), "hipStreamWaitEvent");
       if (x.ok()) x = j;
     }
@@ -442,8 +462,8 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   } else if (t.graph) {
     GraphKey key;
     memset(&key, 0, sizeof key);
-    void *ptrs[16] = {B, pics, rows, slices, status, frames, tables, coefs, wide, serr, ferr, itemp, segp, aord,
-                      ssz, ordp};
+    void *ptrs[17] = {B, pics, rows, slices, status, frames, tables, coefs, wide, serr, ferr, itemp, segp, aord,
+                      costp, ordp, chp};
     memcpy(key.ptrs, ptrs, sizeof ptrs);
     memcpy(key.grids, grids, sizeof grids);
     key.ewg = ewg;

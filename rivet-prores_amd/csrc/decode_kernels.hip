@@ -4,9 +4,10 @@
 // in device memory, a fifth before them, pd_headers, that gathers the header
 // bytes the host plans the batch from):
 // Packet and slice offsets are absolute device addresses.
-//   pd_slice_table  one workgroup per picture: the u16 big-endian slice-size
-//                   table becomes byte offsets (an exclusive prefix), plus the
-//                   bounds check of the table walk (src/decode.rs:202-221).
+//   pd_slice_table  one workgroup per 256 slices of a picture: the u16
+//                   big-endian slice-size table becomes byte offsets (an
+//                   exclusive prefix), plus the bounds check of the table walk
+//                   (src/decode.rs:202-221).
 //   pd_entropy      one lane per colour component of a slice: the RDD 36
 //                   adaptive Rice/exp-Golomb DC and run/level decode
 //                   (src/decode.rs:484-540, src/vlc.rs) into a coefficient
@@ -953,161 +954,141 @@ __device__ __forceinline__ void alpha_slice(
 }  // namespace
 
 // ---------------------------------------------------------------------------
-constexpr uint32_t ST_PER = 16;  // slices a thread per pass of pd_slice_table
-
-// One workgroup per picture (a grid-stride loop in persistent form): the
-// u16 big-endian slice-size table becomes byte offsets (an exclusive
-// prefix), with the bounds check of the table walk (src/decode.rs:202-221).
-// It also clears the batch's queue heads, its slice error words and its
-// first-error slots, so the launch sequence needs no memset, and writes the
-// picture's IDCT items.
-extern "C" __global__ void __launch_bounds__(256)
+// One workgroup per chunk of ST_CHUNK consecutive slices of a picture (a
+// grid-stride loop over the chunks in persistent form), one thread a slice:
+// the u16 big-endian slice-size table becomes byte offsets (an exclusive
+// prefix: the sizes before the chunk summed, then a scan of the chunk's), with
+// the bounds check of the table walk (src/decode.rs:202-221). It also clears
+// the batch's queue heads, its slice error words and its first-error slots, so
+// the launch sequence needs no memset, and writes the picture's IDCT items.
+// Spread over many workgroups, the scattered slice-header reads of a batch
+// are in flight on every CU at once.
+extern "C" __global__ void __launch_bounds__(ST_CHUNK)
     pd_slice_table(DecBatch *__restrict__ B, const DecFrame *__restrict__ frames,
                    const DecPic *__restrict__ pics, const RowSlot *__restrict__ rows,
-                   SliceRec *__restrict__ slices, uint32_t *__restrict__ pic_status,
-                   uint4 *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err,
-                   const IdctItem *__restrict__ segs, IdctItem *__restrict__ items,
-                   AlphaRec *__restrict__ alpha_order, uint16_t *__restrict__ slice_size) {
+                   const uint2 *__restrict__ chunks, SliceRec *__restrict__ slices,
+                   uint32_t *__restrict__ pic_status, uint4 *__restrict__ slice_err,
+                   uint32_t *__restrict__ pic_first_err, const IdctItem *__restrict__ segs,
+                   IdctItem *__restrict__ items, AlphaRec *__restrict__ alpha_order,
+                   uint32_t *__restrict__ group_cost) {
+  static_assert(ST_CHUNK == 256, "four waves a chunk");
   const uint32_t tid = threadIdx.x;
   if (blockIdx.x == 0 && tid < 4) B->q[tid] = 0;
-  const uint32_t np = B->n_pics, G = B->n_slices;
-  __shared__ uint32_t wsum[4];
-  __shared__ uint64_t carry;
+  const uint32_t nc = B->n_chunks, G = B->n_slices;
+  __shared__ uint32_t wsum[4], wpre[4];
   const uint32_t lane = tid & 63, wave = tid >> 6;
-  for (uint32_t p = blockIdx.x; p < np; p += gridDim.x) {
+  for (uint32_t x = blockIdx.x; x < nc; x += gridDim.x) {
+    // The chunk: its picture and its first slice (a multiple of ST_CHUNK).
+    const uint2 C = chunks[x];
+    const uint32_t p = C.x, k0 = C.y, k = k0 + tid;
     const DecPic P = pics[p];
     const bool has_alpha = frames[P.frame].alpha_bits != 0;
     const uint8_t *tab = (const uint8_t *)(P.pic_off + P.header_size);
     const uint64_t data0 = P.header_size + 2ull * P.n_slices;  // relative to the picture
-    if (tid == 0) {
-      carry = 0;
-      pic_first_err[p] = 0xFFFFFFFFu;
+    const bool valid = k < P.n_slices;
+    // Table entries before the chunk: thread t sums entries t, t + 256, ...
+    uint32_t pre = 0;
+#pragma unroll 4
+    for (uint32_t i = tid; i < k0; i += ST_CHUNK) pre += (uint32_t)tab[2 * i] << 8 | tab[2 * i + 1];
+    const uint32_t sz = valid ? ((uint32_t)tab[2 * k] << 8 | tab[2 * k + 1]) : 0u;
+    RowSlot rs = {0, 0, 0};
+    if (valid) rs = rows[P.layout_base + (k - (k / P.row_slices) * P.row_slices)];
+    uint32_t incl = sz, psum = pre;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      const uint32_t a = __shfl_up(incl, d, 64), b = __shfl_xor(psum, d, 64);
+      if (lane >= d) incl += a;
+      psum += b;
     }
+    if (lane == 63) wsum[wave] = incl;
+    if (lane == 0) wpre[wave] = psum;
     __syncthreads();
-    // ST_PER consecutive slices a thread: their sizes are loaded together,
-    // and one workgroup scan a pass gives every offset.
-    for (uint32_t pb = 0; pb < P.n_slices; pb += 256 * ST_PER) {
-      const uint32_t k0 = pb + tid * ST_PER;
-      uint32_t sz[ST_PER], sum = 0;
-#pragma unroll
-      for (uint32_t j = 0; j < ST_PER; j++) {
-        const uint32_t k = k0 + j;
-        sz[j] = k < P.n_slices ? ((uint32_t)tab[2 * k] << 8) | tab[2 * k + 1] : 0u;
-        sum += sz[j];
-      }
-      uint32_t incl = sum;
-      for (uint32_t d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += o;
-      }
-      if (lane == 63) wsum[wave] = incl;
-      __syncthreads();
-      uint32_t before = 0;
-      for (uint32_t w = 0; w < wave; w++) before += wsum[w];
-      const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-      uint64_t excl = carry + before + incl - sum;
-      // Every load of the pass is issued before any is used: row slots and
-      // slice headers are read unconditionally, at a valid address (the
-      // last slice's slot, the picture header) where the slice has none.
-      uint64_t at[ST_PER];
-      RowSlot rs[ST_PER];
-      uint32_t valid = 0;  // bit j: slice k0 + j
-#pragma unroll
-      for (uint32_t j = 0; j < ST_PER; j++) {
-        const uint32_t k = k0 + j, kk = k < P.n_slices ? k : P.n_slices - 1;
-        if (k < P.n_slices) valid |= 1u << j;
-        at[j] = excl;
-        excl += sz[j];
-        rs[j] = rows[P.layout_base + (kk - (kk / P.row_slices) * P.row_slices)];
-      }
-      // The alpha work order: the slices of frames with alpha whose header
-      // passes slice_header's checks (the colour decode reports the others),
-      // long alpha streams at the front, the rest from the back; each with
-      // where its alpha stream is. A slice outside the picture reads nothing.
-      uint32_t heavy = 0, listed = 0, alen[ST_PER];
-      uint64_t astream[ST_PER];
-      if (has_alpha) {
-#pragma unroll
-        for (uint32_t j0 = 0; j0 < ST_PER; j0 += 4) {
-          uint32_t hb0[4], qb[4], comp[4];
-          bool inside[4];
-#pragma unroll
-          for (uint32_t j = j0; j < j0 + 4; j++) {
-            inside[j - j0] = ((valid >> j) & 1) && sz[j] >= 8 && data0 + at[j] + sz[j] <= P.picture_size;
-            const uint8_t *sl = (const uint8_t *)(P.pic_off + (inside[j - j0] ? data0 + at[j] : 0));
-            hb0[j - j0] = sl[0];
-            qb[j - j0] = sl[1];
-            comp[j - j0] = ((uint32_t)sl[2] << 8 | sl[3]) + ((uint32_t)sl[4] << 8 | sl[5]) +
-                           ((uint32_t)sl[6] << 8 | sl[7]);
-          }
-#pragma unroll
-          for (uint32_t j = j0; j < j0 + 4; j++) {
-            // slice_header with alpha: 8 <= hs <= size, 1 <= q <= 224, the
-            // component sizes inside the body.
-            const uint32_t hs = hb0[j - j0] >> 3, q = qb[j - j0], c = comp[j - j0];
-            const bool ok = inside[j - j0] && hs >= 8 && hs <= sz[j] && q >= 1 && q <= 224 && c <= sz[j] - hs;
-            alen[j] = ok ? sz[j] - hs - c : 0u;
-            astream[j] = P.pic_off + data0 + at[j] + hs + c;
-            if (ok) listed |= 1u << j;
-            if (ok && alen[j] > ALPHA_HEAVY_BYTES) heavy |= 1u << j;
-          }
-        }
-      }
-#pragma unroll
-      for (uint32_t j = 0; j < ST_PER; j++) {
-        const uint32_t k = k0 + j;
-        if ((valid >> j) & 1) {
-          const uint32_t row = k / P.row_slices;
-          SliceRec sr;
-          sr.off = P.pic_off + data0 + at[j];
-          sr.coef = P.coef_base + (uint64_t)row * P.row_coefs + rs[j].coef_prefix;
-          sr.len = sz[j];
-          sr.pic = p;
-          sr.mb_x = rs[j].mb_x;
-          sr.mb_y = (uint16_t)row;
-          sr.mbs = rs[j].mbs;
-          slices[P.slice_base + k] = sr;
-          slice_err[P.slice_base + k] = make_uint4(0, 0, 0, 0);
-          slice_size[P.slice_base + k] = (uint16_t)sz[j];
-        }
-      }
-      const uint32_t nh = (uint32_t)__popc(heavy), nl = (uint32_t)__popc(listed & ~heavy);
-      uint32_t hi = nh, li = nl;
-      for (uint32_t d = 1; d < 64; d <<= 1) {
-        const uint32_t a = __shfl_up(hi, d, 64), b = __shfl_up(li, d, 64);
-        if (lane >= d) {
-          hi += a;
-          li += b;
-        }
-      }
-      uint32_t hb = 0, lb = 0;
-      if (lane == 63) {
-        if (hi) hb = atomicAdd(&B->heavy, hi);
-        if (li) lb = atomicAdd(&B->light, li);
-      }
-      hb = __shfl(hb, 63, 64) + hi - nh;
-      lb = __shfl(lb, 63, 64) + li - nl;
-#pragma unroll
-      for (uint32_t j = 0; j < ST_PER; j++) {
-        if ((listed >> j) & 1) {
-          AlphaRec ar;
-          ar.stream = astream[j];
-          ar.g = P.slice_base + k0 + j;
-          ar.len = alen[j];
-          ar.pic = p;
-          ar.frame = P.frame;
-          ar.mb = (uint32_t)rs[j].mb_x | ((k0 + j) / P.row_slices) << 16;
-          ar.mbs = rs[j].mbs;
-          alpha_order[(heavy >> j) & 1 ? hb++ : G - 1 - lb++] = ar;
-        }
-      }
-      __syncthreads();
-      if (tid == 0) carry += total;
-      __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; w++) before += wsum[w];
+    const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    const uint64_t carry = (uint64_t)wpre[0] + wpre[1] + wpre[2] + wpre[3];
+    const uint64_t at = carry + before + incl - sz;
+    if (tid == 0) {
+      if (k0 == 0) pic_first_err[p] = 0xFFFFFFFFu;
+      // The chunk that holds the table's last entry checks its end.
+      if (k0 + ST_CHUNK >= P.n_slices)
+        pic_status[p] = (data0 + carry + total > P.picture_size) ? PIC_TABLE_ERR : PIC_OK;
     }
-    if (tid == 0) pic_status[p] = (data0 + carry > P.picture_size) ? PIC_TABLE_ERR : PIC_OK;
-    // The picture's IDCT items: every row cut into the same segments.
-    for (uint32_t i = tid; i < P.height_in_mb * P.n_segs; i += 256) {
+    // The alpha work order: the slices of frames with alpha whose header
+    // passes slice_header's checks (the colour decode reports the others),
+    // long alpha streams at the front, the rest from the back; each with
+    // where its alpha stream is. A slice outside the picture reads nothing.
+    bool listed = false, heavy = false;
+    uint32_t alen = 0;
+    uint64_t astream = 0;
+    if (has_alpha && valid && sz >= 8 && data0 + at + sz <= P.picture_size) {
+      const uint8_t *sl = (const uint8_t *)(P.pic_off + data0 + at);
+      const uint32_t hs = sl[0] >> 3, q = sl[1];
+      const uint32_t c = ((uint32_t)sl[2] << 8 | sl[3]) + ((uint32_t)sl[4] << 8 | sl[5]) +
+                         ((uint32_t)sl[6] << 8 | sl[7]);
+      // slice_header with alpha: 8 <= hs <= size, 1 <= q <= 224, the
+      // component sizes inside the body.
+      listed = hs >= 8 && hs <= sz && q >= 1 && q <= 224 && c <= sz - hs;
+      alen = listed ? sz - hs - c : 0u;
+      heavy = listed && alen > ALPHA_HEAVY_BYTES;
+      astream = P.pic_off + data0 + at + hs + c;
+    }
+    if (valid) {
+      const uint32_t row = k / P.row_slices;
+      SliceRec sr;
+      sr.off = P.pic_off + data0 + at;
+      sr.coef = P.coef_base + (uint64_t)row * P.row_coefs + rs.coef_prefix;
+      sr.len = sz;
+      sr.pic = p;
+      sr.mb_x = rs.mb_x;
+      sr.mb_y = (uint16_t)row;
+      sr.mbs = rs.mbs;
+      slices[P.slice_base + k] = sr;
+      slice_err[P.slice_base + k] = make_uint4(0, 0, 0, 0);
+    }
+    {
+      // The entropy items' cost (pd_entropy_order): the largest slice of
+      // each group of 64 consecutive slices of the batch, zero at upload.
+      // A wave's slices fall in at most two groups: one atomic each.
+      const uint32_t g0 = (P.slice_base + k0 + 64 * wave) / 64, gk = (P.slice_base + k) / 64;
+      uint32_t m0 = valid && gk == g0 ? sz : 0u, m1 = valid && gk != g0 ? sz : 0u;
+      const uint64_t in0 = __ballot(valid && gk == g0), in1 = __ballot(valid && gk != g0);
+      for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t a = __shfl_xor(m0, d, 64), b = __shfl_xor(m1, d, 64);
+        m0 = a > m0 ? a : m0;
+        m1 = b > m1 ? b : m1;
+      }
+      if (lane == 0) {
+        if (in0) atomicMax(&group_cost[g0], m0);
+        if (in1) atomicMax(&group_cost[g0 + 1], m1);
+      }
+    }
+    if (has_alpha) {
+      // Places in the work order: one atomic a wave for each end.
+      const uint64_t hm = __ballot(heavy), lm = __ballot(listed && !heavy);
+      uint32_t hb = 0, lb = 0;
+      if (lane == 0) {
+        if (hm) hb = atomicAdd(&B->heavy, (uint32_t)__popcll(hm));
+        if (lm) lb = atomicAdd(&B->light, (uint32_t)__popcll(lm));
+      }
+      hb = __shfl(hb, 0, 64);
+      lb = __shfl(lb, 0, 64);
+      const uint64_t below = (1ull << lane) - 1;
+      if (listed) {
+        AlphaRec ar;
+        ar.stream = astream;
+        ar.g = P.slice_base + k;
+        ar.len = alen;
+        ar.pic = p;
+        ar.frame = P.frame;
+        ar.mb = (uint32_t)rs.mb_x | (k / P.row_slices) << 16;
+        ar.mbs = rs.mbs;
+        alpha_order[heavy ? hb + (uint32_t)__popcll(hm & below) : G - 1 - lb - (uint32_t)__popcll(lm & below)] = ar;
+      }
+    }
+    // The picture's IDCT items (every row cut into the same segments),
+    // shared among the picture's chunks.
+    const uint32_t nch = P.n_slices ? (P.n_slices + ST_CHUNK - 1) / ST_CHUNK : 1u;
+    for (uint32_t i = k0 + tid; i < P.height_in_mb * P.n_segs; i += nch * ST_CHUNK) {
       const uint32_t row = i / P.n_segs;
       IdctItem it = segs[P.seg_base + (i - row * P.n_segs)];
       it.first += P.slice_base + row * P.row_slices;
@@ -1172,13 +1153,13 @@ extern "C" __global__ void __launch_bounds__(ENTROPY_MAX_WG)
 // One workgroup: the entropy items of the batch in order of cost, longest
 // first, into order[0, n_slots) (slots past the items hold indices past
 // them; n_slots 0: the batch's item count). An item's cost is its largest
-// slice in bytes (the slice table's sizes): a wave's time follows its
+// slice in bytes (pd_slice_table's group costs): a wave's time follows its
 // longest stream. A counting sort on cost / ORDER_BUCKET_BYTES (items of
 // one bucket in any order); the host launches this only for
 // n_slots <= ORDER_MAX.
 constexpr uint32_t ORDER_BUCKETS = 2048, ORDER_BUCKET_BYTES = 32;
 extern "C" __global__ void __launch_bounds__(1024)
-    pd_entropy_order(const DecBatch *__restrict__ B, const uint16_t *__restrict__ slice_size,
+    pd_entropy_order(const DecBatch *__restrict__ B, const uint32_t *__restrict__ group_cost,
                      uint32_t *__restrict__ order, uint32_t n_slots) {
   __shared__ uint32_t cnt[ORDER_BUCKETS];
   __shared__ uint16_t bucket[ORDER_MAX];
@@ -1188,41 +1169,15 @@ extern "C" __global__ void __launch_bounds__(1024)
   if (n_slots == 0) n_slots = n_items;
   for (uint32_t i = tid; i < ORDER_BUCKETS; i += blockDim.x) cnt[i] = 0;
   __syncthreads();
-  // Costs: eight lanes a group, eight sizes a lane, every load in flight
-  // together (past the batch: the last slice again, which only repeats a
-  // size).
-  constexpr uint32_t PER = 8;
-  const uint32_t chunks = groups * (64 / PER), stride = blockDim.x;
-  for (uint32_t c0 = 0; c0 < chunks; c0 += 4 * stride) {
-    uint32_t m[4];
-#pragma unroll
-    for (uint32_t u = 0; u < 4; u++) {
-      const uint32_t c = c0 + u * stride + tid;
-      m[u] = 0;
-#pragma unroll
-      for (uint32_t j = 0; j < PER; j++) {
-        const uint32_t g = PER * c + j;
-        const uint32_t v = slice_size[g < G ? g : G - 1];
-        m[u] = v > m[u] ? v : m[u];
-      }
-    }
-#pragma unroll
-    for (uint32_t u = 0; u < 4; u++) {
-      uint32_t v = m[u];
-#pragma unroll
-      for (uint32_t o = 4; o; o >>= 1) {
-        const uint32_t x = __shfl_xor(v, o, 64);
-        v = x > v ? x : v;
-      }
-      const uint32_t c = c0 + u * stride + tid, it = 3 * (c / 8) + (lane & 7);
-      if ((lane & 7) < 3 && c < chunks) {
-        // Descending cost: bucket 0 holds the longest items.
-        const uint32_t b = ORDER_BUCKETS - 1 - (v / ORDER_BUCKET_BYTES < ORDER_BUCKETS - 1
-                                                    ? v / ORDER_BUCKET_BYTES : ORDER_BUCKETS - 1);
-        bucket[it] = (uint16_t)b;
-        atomicAdd(&cnt[b], 1u);
-      }
-    }
+  // Costs, one a group of 64 slices (its largest slice, from
+  // pd_slice_table); the group's three items share it.
+  for (uint32_t grp = tid; grp < groups; grp += blockDim.x) {
+    const uint32_t v = group_cost[grp];
+    // Descending cost: bucket 0 holds the longest items.
+    const uint32_t b = ORDER_BUCKETS - 1 - (v / ORDER_BUCKET_BYTES < ORDER_BUCKETS - 1
+                                                ? v / ORDER_BUCKET_BYTES : ORDER_BUCKETS - 1);
+    bucket[3 * grp] = bucket[3 * grp + 1] = bucket[3 * grp + 2] = (uint16_t)b;
+    atomicAdd(&cnt[b], 3u);
   }
   __syncthreads();
   // Exclusive scan of the counts, two buckets a thread.

@@ -118,7 +118,12 @@ struct DecBatch {
   uint32_t heavy, light;     // alpha work order fill counts (zero at upload)
   uint32_t wide;             // nonzero once the entropy decode stores a value outside +-32767
   uint32_t q[4];             // queue heads: entropy, alpha, idct
+  uint32_t n_chunks;         // pd_slice_table's chunks of ST_CHUNK slices
+  uint32_t pad[3];
 };
+// Slices a pd_slice_table workgroup (one thread each); a picture's chunks
+// start at multiples of it, and a picture without slices has one chunk.
+constexpr uint32_t ST_CHUNK = 256;
 
 // A run of consecutive slices of one picture transformed by one workgroup
 // (their coefficients are contiguous in the arena). As a row segment,
