@@ -418,7 +418,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
       void *args[] = {&B, &costp, &ordp, &n_slots};
       if (!(x = launch(L, d.k_order, 1, 1, 1024, args)).ok()) return x;
     }
-    if (timed) (void)hipEventRecord(L.ev[1], L.stream);
+    if (timed && !(x = check(hipEventRecord(L.ev[1], L.stream), "hipEventRecord")).ok()) return x;
     // Alpha needs only the slice table: the second stream waits for the work
     // so far and decodes alpha beside the colour entropy decode and the
     // IDCT, which leave most of the machine's wave slots free. The entropy
@@ -426,8 +426,11 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     // are placed before the alpha workgroups.
     bool forked = false, joined = false;
     if (b.any_alpha) {
-      if (!(x = check(hipEventRecord(L.fork, L.stream), "hipEventRecord")).ok()) return x;
-      if (!(x = check(hipStreamWaitEvent(L.aux, L.fork, 0), "hipStreamWaitEvent")).ok()) return x;
+      // A timed batch forks at its stage event: one marker less between
+      // the order and entropy launches.
+      hipEvent_t fork = timed ? L.ev[1] : L.fork;
+      if (!timed && !(x = check(hipEventRecord(fork, L.stream), "hipEventRecord")).ok()) return x;
+      if (!(x = check(hipStreamWaitEvent(L.aux, fork, 0), "hipStreamWaitEvent")).ok()) return x;
       forked = true;
     }
     {

@@ -495,6 +495,8 @@ class Decoder:
             rc = lib().prores_decode_device_batch_tuned(self._h, jobs.n, jobs.arr, C.byref(t), jobs.errs)
         if rc == 4 or (rc and tuning is not None):
             _raise(jobs.errs[0])
+        if not jobs._kinds.any():
+            return [None] * jobs.n
         return [None if e.kind == 0 else _KINDS.get(e.kind, Error)(e.msg.decode()) for e in jobs.errs]
 
     def autotune(self, jobs, reps=3):
@@ -519,6 +521,8 @@ class DecodeJobs:
         self.n = len(jobs)
         self.arr = (_DevDecodeJob * self.n)()
         self.errs = (_Err * self.n)()
+        # The result kinds, read in one step after each batch.
+        self._kinds = np.frombuffer(self.errs, dtype=np.int32).reshape(self.n, C.sizeof(_Err) // 4)[:, 0]
         self._keep = []
         for i, (hp, dp, dd, da) in enumerate(jobs):
             if isinstance(hp, int):
