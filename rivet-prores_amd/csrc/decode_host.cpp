@@ -812,6 +812,10 @@ Err decode_batch_on(Device &d, Lane &L, uint32_t bit_depth, size_t n, const pror
     t = d.tuning;
   }
   Batch b;
+  b.frames.reserve(n);
+  b.pics.reserve(2 * n);
+  b.plans.reserve(n);
+  b.rows.reserve(2 * n * 64);
   Err e = plan_jobs(d, L, bit_depth, n, jobs, b);
   std::vector<Err> results;
   if (e.ok()) e = run_batch(d, L, b, t, results);
