@@ -391,30 +391,40 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
     // so the pairs may run up to the end of the next half-line.
     bool moved = false, blocked = false;
     if (act) {
-      uint64_t w = r.window();
+      // The chain runs on speculative copies (window, contexts, position,
+      // bits used), advanced by every pair whatever it finds: an entry past
+      // the tables is 0 (book 0, no bits), so a lane that has stopped keeps
+      // reading inside the tables. A pair commits only while its lane has
+      // not stopped, off the chain.
+      uint64_t ws = r.window();
       const uint32_t rem = r.remaining();
       const uint32_t lim0 = 32 * (out.cur + 2), lim = lim0 < total ? lim0 : total;
-      uint32_t used = 0;
+      uint32_t used = 0, us = 0, rs = rbo, ls = lbo, ns = n;
       bool go = true;
+#pragma unroll
       for (uint32_t k = 0; k < FAST_PAIRS; k++) {
-        if (!__ballot(go)) break;
-        const uint32_t hi = (uint32_t)(w >> 32), lo = (uint32_t)w;
+        if (k == FAST_PAIRS / 2 && !__ballot(go)) break;
+        const uint32_t hi = (uint32_t)(ws >> 32), lo = (uint32_t)ws;
         const uint32_t er =
-            *(const uint16_t *)((const char *)ftab + FAST_RUN_BASE + (rbo | ((hi >> 22) & 0x3FE)));
+            *(const uint16_t *)((const char *)ftab + FAST_RUN_BASE + (rs | ((hi >> 22) & 0x3FE)));
         const uint32_t lr = er & 15;
-        const uint32_t hi2 = __builtin_amdgcn_alignbit(hi, lo, 32 - lr);  // w << lr, top word
-        const uint32_t el = *(const uint16_t *)((const char *)ftab + (lbo | ((hi2 >> 21) & 0x7FE)));
-        const uint32_t t = lr + (el & 15), m = n + ((er >> 4) & 63);
-        const bool ok = go && (er & el & FAST_VALID) != 0 && used + t <= rem && m < lim;
+        const uint32_t hi2 = __builtin_amdgcn_alignbit(hi, lo, 32 - lr);  // ws << lr, top word
+        const uint32_t el = *(const uint16_t *)((const char *)ftab + (ls | ((hi2 >> 21) & 0x7FE)));
+        const uint32_t t = lr + (el & 15), m = ns + ((er >> 4) & 63);
+        ws <<= t;
+        rs = er & 0x1C00;
+        ls = el & 0x3800;
+        us += t;
+        ns = m + 1;
+        const bool ok = go && (er & el & FAST_VALID) != 0 && us <= rem && m < lim;
         blocked = blocked || (go && !ok);
         out.store_slot(ok ? (m & 63) : CoefOut::SPARE, ((int32_t)(el << 21)) >> 25);
-        n = ok ? m + 1 : n;
-        used = ok ? used + t : used;
-        w = ok ? w << t : w;
-        rbo = ok ? (er & 0x1C00) : rbo;
-        lbo = ok ? (el & 0x3800) : lbo;
+        n = ok ? ns : n;
+        used = ok ? us : used;
+        rbo = ok ? rs : rbo;
+        lbo = ok ? ls : lbo;
         moved = moved || ok;
-        go = ok && used <= FAST_START;
+        go = ok && us <= FAST_START;
       }
       if (moved) {
         r.consume(used);
@@ -755,7 +765,11 @@ __device__ __forceinline__ void alpha_slice(
     uint32_t *__restrict__ slice_err, uint32_t *__restrict__ pic_first_err) {
   const uint32_t L = threadIdx.x;
   const uint32_t g = uni(R.g), pic = uni(R.pic);
-  if (uni(pic_status[pic]) != PIC_OK) return;
+  // The picture's status is loaded with the stream and looked at once the
+  // stream is staged: one round trip to memory instead of two. The record
+  // names only a slice whose bytes lie inside its picture, so the staging
+  // reads stay inside the packet whatever the status.
+  const uint32_t status = pic_status[pic];
   const DecPic &P = pics[pic];
   const DecFrame &F = frames[uni(R.frame)];
   const uint32_t bits = uni(F.alpha_bits);
@@ -771,17 +785,26 @@ __device__ __forceinline__ void alpha_slice(
   const uint32_t end_byte = m + len;  // bytes of stream data end here
   const uint32_t have = (end_byte + 7) / 8;
   const uint32_t stage = have + 2 < ALPHA_WORDS ? have + 2 : ALPHA_WORDS;
-  for (uint32_t j = L; j < stage; j += 64) {
-    uint64_t w = 0;
-    if (j < have) {
-      w = __builtin_bswap64(src[j]);
+  // Four words a lane in flight at a time.
+  for (uint32_t j0 = 0; j0 < stage; j0 += 256) {
+    uint64_t raw[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) {
+      const uint32_t j = j0 + 64 * u + L;
+      raw[u] = j < have && j < stage ? src[j] : 0ull;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) {
+      const uint32_t j = j0 + 64 * u + L;
+      uint64_t w = __builtin_bswap64(raw[u]);
       const uint32_t lo = 8 * j, hi = lo + 8;
       if (lo < m) w &= ~0ull >> (8 * (m - lo));
-      if (hi > end_byte) w &= ~0ull << (8 * (hi - end_byte));
+      if (hi > end_byte) w &= hi - end_byte >= 8 ? 0ull : ~0ull << (8 * (hi - end_byte));
+      if (j < stage) words[j] = w;
     }
-    words[j] = w;
   }
   __syncthreads();
+  if (uni(status) != PIC_OK) return;
 
   // The parse is the same in every lane: its state lives in scalar
   // registers (each staged word is read as a wave-uniform value), and the
