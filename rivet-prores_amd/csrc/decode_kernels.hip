@@ -172,7 +172,6 @@ __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t
 // arena. The *_all calls are made by every lane of the wave together.
 struct CoefOut {
   static constexpr uint32_t NONE = 0xFFFFFFFFu;
-  static constexpr uint32_t SPARE = 64;  // a slot after the ring that is never written out
   int16_t *base;
   int32_t *wide;
   uint32_t *lds;   // the workgroup's lines (LINE_WORDS dwords a lane)
@@ -181,6 +180,7 @@ struct CoefOut {
   uint32_t cur;    // the half-line being filled
   uint32_t pend;   // the pending half-line, or NONE
   uint32_t *any_wide;  // the batch's flag: set when a value goes to the side arena
+  uint32_t spare;      // u16 index in lds of a slot never written out: this lane's list entry
   __device__ __forceinline__ void clear() {
     uint4 *l = (uint4 *)(lds + line);
 #pragma unroll
@@ -201,9 +201,10 @@ struct CoefOut {
     }
     ((uint16_t *)(lds + line))[n & 63] = (uint16_t)(int16_t)h;
   }
-  // A value of magnitude below 32768 at ring slot `slot` (SPARE: nowhere).
-  __device__ __forceinline__ void store_slot(uint32_t slot, int32_t v) {
-    ((uint16_t *)(lds + line))[slot] = (uint16_t)(int16_t)v;
+  // A value of magnitude below 32768 at ring slot `slot`, or nowhere (the
+  // spare slot) when `keep` is false.
+  __device__ __forceinline__ void store_slot(bool keep, uint32_t slot, int32_t v) {
+    ((uint16_t *)lds)[keep ? 2 * line + slot : spare] = (uint16_t)(int16_t)v;
   }
   // Every lane's pending half-line to the arena; its half of the ring cleared.
   __device__ __forceinline__ void flush_pending_all() {
@@ -418,7 +419,7 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
         ns = m + 1;
         const bool ok = go && (er & el & FAST_VALID) != 0 && us <= rem && m < lim;
         blocked = blocked || (go && !ok);
-        out.store_slot(ok ? (m & 63) : CoefOut::SPARE, ((int32_t)(el << 21)) >> 25);
+        out.store_slot(ok, m & 63, ((int32_t)(el << 21)) >> 25);
         n = ok ? ns : n;
         used = ok ? us : used;
         rbo = ok ? rs : rbo;
@@ -560,7 +561,8 @@ __device__ __forceinline__ void entropy_stream(
   uint32_t err = E_NONE, nb = 0, slice_base = 0;
   bool act = false;
   Reader r;
-  CoefOut out{coefs, coef_wide, lds, list, LINE_WORDS * threadIdx.x, 0, CoefOut::NONE, any_wide};
+  CoefOut out{coefs, coef_wide, lds, list, LINE_WORDS * threadIdx.x, 0, CoefOut::NONE, any_wide,
+              2 * (uint32_t)((uint32_t *)(list + (threadIdx.x & 63)) - lds)};
   if (valid) {
     S = slices[g];
     if (pic_status[S.pic] == PIC_OK) {

@@ -150,10 +150,12 @@ constexpr uint32_t ORDER_MAX = 8192;
 constexpr uint32_t IDCT_MAX_BLOCKS = 96;  // blocks one workgroup holds in LDS
 constexpr uint32_t IDCT_MAX_SLICES = 12;  // slices of one item (a slice is at least 8 blocks)
 // Entropy decode LDS: two 32-coefficient half-lines a lane (16-byte aligned),
-// and a 64-entry list of 16-byte entries a wave.
-constexpr uint32_t LINE_WORDS = 36;
+// and a 64-entry list of 16-byte entries a wave. Kept to 126 KB for a
+// 12-wave workgroup with its tables, so two alpha workgroups fit beside it
+// on a CU.
+constexpr uint32_t LINE_WORDS = 32;
 constexpr uint32_t ENTROPY_LDS_PER_WAVE = 64 * LINE_WORDS * 4 + 64 * 16;
-constexpr uint32_t ENTROPY_MAX_WG = 768;  // 12 waves: 120 KB of lines and lists plus 12 KB of tables
+constexpr uint32_t ENTROPY_MAX_WG = 768;  // 12 waves: 108 KB of lines and lists plus 18 KB of tables
 
 // Slices whose alpha stream is longer than this many bytes go first in the
 // alpha work order: their decode is one long serial chain.
