@@ -60,6 +60,10 @@ __device__ __forceinline__ int32_t symbol_to_signed(uint32_t s) {
 // bits past the end of the stream read as zero, as the reference's 64-bit
 // peek does (src/bits.rs:30-40).
 typedef const __attribute__((address_space(1))) uint64_t gword_t;
+// Global memory reached through an address held as an integer (a record,
+// an LDS list entry, a readlane): a generic (flat) access would also count
+// on the LDS counter, and every LDS wait after it would wait for it too.
+#define GLOBAL __attribute__((address_space(1)))
 
 struct Reader {
   gword_t *src;
@@ -74,7 +78,7 @@ struct Reader {
     r0 = src[c];
     r1 = src[c + 1];
   }
-  __device__ __forceinline__ void init(const uint8_t *base, uint32_t len) {
+  __device__ __forceinline__ void init(const GLOBAL uint8_t *base, uint32_t len) {
     const uintptr_t a = (uintptr_t)base;
     const uint32_t m = (uint32_t)(a & 7);
     src = (gword_t *)(a - m);
@@ -172,8 +176,8 @@ __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t
 // arena. The *_all calls are made by every lane of the wave together.
 struct CoefOut {
   static constexpr uint32_t NONE = 0xFFFFFFFFu;
-  int16_t *base;
-  int32_t *wide;
+  GLOBAL int16_t *base;
+  GLOBAL int32_t *wide;
   uint32_t *lds;   // the workgroup's lines (LINE_WORDS dwords a lane)
   uint4 *list;     // the wave's 64 list entries
   uint32_t line;   // this lane's line: dword index into lds
@@ -187,7 +191,7 @@ struct CoefOut {
     for (uint32_t i = 0; i < 8; i++) l[i] = make_uint4(0, 0, 0, 0);
   }
   __device__ __forceinline__ void zero_half(uint32_t h) {
-    uint4 *o = (uint4 *)(base + 32 * h);
+    GLOBAL uint4 *o = (GLOBAL uint4 *)(base + 32 * h);
 #pragma unroll
     for (uint32_t i = 0; i < 4; i++) o[i] = make_uint4(0, 0, 0, 0);
   }
@@ -226,7 +230,7 @@ struct CoefOut {
         uint4 *src = (uint4 *)(lds + ent.x) + (L & 3);
         const uint4 v = *src;
         *src = make_uint4(0, 0, 0, 0);
-        ((uint4 *)(uintptr_t)(((uint64_t)ent.w << 32) | ent.z))[L & 3] = v;
+        ((GLOBAL uint4 *)(uintptr_t)(((uint64_t)ent.w << 32) | ent.z))[L & 3] = v;
       }
     }
   }
@@ -261,9 +265,9 @@ struct CoefOut {
       m &= m - 1;
       const uint32_t first = lane_u32(cur, j) + 1;
       const uint32_t count = lane_u32(nhalves, j) - first;
-      int16_t *const b = lane_base(j);
+      GLOBAL int16_t *const b = lane_base(j);
       for (uint32_t k = L >> 2; k < count; k += 16)
-        ((uint4 *)(b + 32 * (first + k)))[L & 3] = make_uint4(0, 0, 0, 0);
+        ((GLOBAL uint4 *)(b + 32 * (first + k)))[L & 3] = make_uint4(0, 0, 0, 0);
     }
   }
   // Lane j's value of a 32-bit lane variable, and of `base` (j uniform).
@@ -272,10 +276,10 @@ struct CoefOut {
   __device__ __forceinline__ static uint32_t lane_u32(uint32_t v, uint32_t j) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
   }
-  __device__ __forceinline__ int16_t *lane_base(uint32_t j) const {
+  __device__ __forceinline__ GLOBAL int16_t *lane_base(uint32_t j) const {
     const uint64_t lo = lane_u32((uint32_t)(uintptr_t)base, j);
     const uint64_t hi = lane_u32((uint32_t)((uintptr_t)base >> 32), j);
-    return (int16_t *)(uintptr_t)((hi << 32) | lo);
+    return (GLOBAL int16_t *)(uintptr_t)((hi << 32) | lo);
   }
 };
 
@@ -512,7 +516,7 @@ __device__ uint32_t decode_component(Reader &r, uint32_t nb, CoefOut &out, bool 
 
 // The slice header (src/decode.rs:301-338): header size, component sizes
 // and their checks. Returns the error word, E_NONE when the header is sound.
-__device__ __forceinline__ uint32_t slice_header(const uint8_t *s, uint32_t slen, uint32_t alpha_bits,
+__device__ __forceinline__ uint32_t slice_header(const GLOBAL uint8_t *s, uint32_t slen, uint32_t alpha_bits,
                                                  uint32_t &hs, uint32_t &ysz, uint32_t &cbsz,
                                                  uint32_t &crsz) {
   const uint32_t min_header = alpha_bits ? 8u : 6u;
@@ -561,7 +565,8 @@ __device__ __forceinline__ void entropy_stream(
   uint32_t err = E_NONE, nb = 0, slice_base = 0;
   bool act = false;
   Reader r;
-  CoefOut out{coefs, coef_wide, lds, list, LINE_WORDS * threadIdx.x, 0, CoefOut::NONE, any_wide,
+  CoefOut out{(GLOBAL int16_t *)coefs, (GLOBAL int32_t *)coef_wide, lds, list, LINE_WORDS * threadIdx.x, 0,
+              CoefOut::NONE, any_wide,
               2 * (uint32_t)((uint32_t *)(list + (threadIdx.x & 63)) - lds)};
   if (valid) {
     S = slices[g];
@@ -569,7 +574,7 @@ __device__ __forceinline__ void entropy_stream(
       const DecPic &P = pics[S.pic];
       const DecFrame &F = frames[P.frame];
       slice_base = P.slice_base;
-      const uint8_t *s = (const uint8_t *)S.off;
+      const GLOBAL uint8_t *s = (const GLOBAL uint8_t *)(uintptr_t)S.off;
       uint32_t hs, ysz, cbsz, crsz;
       err = slice_header(s, S.len, F.alpha_bits, hs, ysz, cbsz, crsz);
       if (!err) {
@@ -580,8 +585,8 @@ __device__ __forceinline__ void entropy_stream(
         const uint64_t cofs =
             S.coef + (comp == 0 ? 0ull : (comp == 1 ? 256ull * S.mbs : (256ull + 64ull * cpm) * S.mbs));
         r.init(s + off, len);
-        out.base = coefs + cofs;
-        out.wide = coef_wide + cofs;
+        out.base = (GLOBAL int16_t *)(coefs + cofs);
+        out.wide = (GLOBAL int32_t *)(coef_wide + cofs);
         out.clear();
         act = true;
       }
@@ -783,7 +788,7 @@ __device__ __forceinline__ void alpha_slice(
   const uint32_t len = uni(R.len);
   const uintptr_t a = (uintptr_t)(((uint64_t)uni((uint32_t)(R.stream >> 32)) << 32) | uni((uint32_t)R.stream));
   const uint32_t m = uni((uint32_t)(a & 7));
-  const uint64_t *src = (const uint64_t *)(a - m);
+  const GLOBAL uint64_t *src = (const GLOBAL uint64_t *)(a - m);
   const uint32_t end_byte = m + len;  // bytes of stream data end here
   const uint32_t have = (end_byte + 7) / 8;
   const uint32_t stage = have + 2 < ALPHA_WORDS ? have + 2 : ALPHA_WORDS;
@@ -957,7 +962,7 @@ __device__ __forceinline__ void alpha_slice(
   // Visible rows, clipped to the plane width.
   const uint32_t x0 = 16u * mb_x, width = F.width;
   const uint32_t nvis = cols < width - x0 ? cols : width - x0;
-  uint16_t *aout = (uint16_t *)F.out_alpha;
+  GLOBAL uint16_t *aout = (GLOBAL uint16_t *)(uintptr_t)F.out_alpha;
   const size_t row0 = P.first_row + (size_t)P.row_step * (16u * mb_y);
   const size_t rstride = (size_t)P.row_step * width;
   if ((width & 7) == 0 && (((uintptr_t)aout) & 15) == 0) {
@@ -965,7 +970,7 @@ __device__ __forceinline__ void alpha_slice(
     const uint32_t per_row = nvis / 8;
     for (uint32_t c = L; c < rows * per_row; c += 64) {
       const uint32_t r = c / per_row, k = c - r * per_row;
-      *(uint4 *)(aout + (row0 * width + r * rstride) + x0 + 8 * k) =
+      *(GLOBAL uint4 *)(aout + (row0 * width + r * rstride) + x0 + 8 * k) =
           *(const uint4 *)(vals + (r << log2c) + 8 * k);
     }
   } else {
@@ -1008,7 +1013,7 @@ extern "C" __global__ void __launch_bounds__(ST_CHUNK)
     const uint32_t p = C.x, k0 = C.y, k = k0 + tid;
     const DecPic P = pics[p];
     const bool has_alpha = frames[P.frame].alpha_bits != 0;
-    const uint8_t *tab = (const uint8_t *)(P.pic_off + P.header_size);
+    const GLOBAL uint8_t *tab = (const GLOBAL uint8_t *)(uintptr_t)(P.pic_off + P.header_size);
     const uint64_t data0 = P.header_size + 2ull * P.n_slices;  // relative to the picture
     const bool valid = k < P.n_slices;
     // Table entries before the chunk: thread t sums entries t, t + 256, ...
@@ -1046,7 +1051,7 @@ extern "C" __global__ void __launch_bounds__(ST_CHUNK)
     uint32_t alen = 0;
     uint64_t astream = 0;
     if (has_alpha && valid && sz >= 8 && data0 + at + sz <= P.picture_size) {
-      const uint8_t *sl = (const uint8_t *)(P.pic_off + data0 + at);
+      const GLOBAL uint8_t *sl = (const GLOBAL uint8_t *)(uintptr_t)(P.pic_off + data0 + at);
       const uint32_t hs = sl[0] >> 3, q = sl[1];
       const uint32_t c = ((uint32_t)sl[2] << 8 | sl[3]) + ((uint32_t)sl[4] << 8 | sl[5]) +
                          ((uint32_t)sl[6] << 8 | sl[7]);
@@ -1410,10 +1415,10 @@ __device__ __forceinline__ void idct_pass(uint32_t t, uint32_t nblocks, uint32_t
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (active && r < rows) {
     // Row r of the block to its plane (put_block, src/decode.rs:418-438).
-    uint16_t *o16 = (uint16_t *)(uintptr_t)s_plane[c] + D.y + (size_t)s_rstride[c] * r;
+    GLOBAL uint16_t *o16 = (GLOBAL uint16_t *)(uintptr_t)s_plane[c] + D.y + (size_t)s_rstride[c] * r;
     const uint4 pk = *(const uint4 *)(outb + lb * IDCT_OUT + 8 * r);
     if (cols == 8 && (((uintptr_t)o16) & 15) == 0) {
-      *(uint4 *)o16 = pk;
+      *(GLOBAL uint4 *)o16 = pk;
     } else {
       const uint32_t wv[4] = {pk.x, pk.y, pk.z, pk.w};
       for (uint32_t x = 0; x < cols; x++) o16[x] = (uint16_t)(wv[x >> 1] >> (16 * (x & 1)));
@@ -1478,7 +1483,7 @@ extern "C" __global__ void __launch_bounds__(256)
         // offsets that may point anywhere: nothing of it is read. The alpha
         // word is not looked at: alpha is decoded beside this kernel.
         const bool ok = pic_ok && !(ev.x | ev.y | ev.z);
-        const uint32_t q = ok ? qscale(((const uint8_t *)S.off)[1]) : 0u;
+        const uint32_t q = ok ? qscale(((const GLOBAL uint8_t *)(uintptr_t)S.off)[1]) : 0u;
         s_pos[tid] = (uint32_t)S.mb_x | (uint32_t)S.mb_y << 16;
         s_mq[tid] = S.mbs | (ok ? 1u : 0u) << 8 | q << 16;
         s_start[tid + 1] = (uint32_t)((S.coef - S0.coef) / 64) + S.mbs * (4 + 2 * cpm);
