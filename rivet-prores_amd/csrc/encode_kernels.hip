@@ -30,6 +30,11 @@
 
 using namespace prores_hip;
 
+// Global memory reached through an address held as an integer (frame
+// planes, the output arena): a generic (flat) access would also count on the
+// LDS counter, and every LDS wait after it would wait for it too.
+#define GLOBAL __attribute__((address_space(1)))
+
 namespace {
 
 __device__ __forceinline__ uint32_t clz64(uint64_t v) { return v ? __builtin_clzll(v) : 64; }
@@ -279,7 +284,8 @@ extern "C" __global__ void __launch_bounds__(256)
       }
       const uint32_t pw = F.plane_w[c];
       const uint32_t prow = by + r < P.height - 1 ? by + r : P.height - 1;
-      const uint16_t *src = (const uint16_t *)F.in_plane[c] + (size_t)(P.first_row + P.row_step * prow) * pw;
+      const GLOBAL uint16_t *src =
+          (const GLOBAL uint16_t *)(uintptr_t)F.in_plane[c] + (size_t)(P.first_row + P.row_step * prow) * pw;
       float p[8];
 #pragma unroll
       for (uint32_t x = 0; x < 8; x++) {
@@ -336,7 +342,7 @@ extern "C" __global__ void __launch_bounds__(64)
   const uint32_t bits = F.alpha_bits, mag = bits == 8 ? 3u : 6u, maxd = bits == 8 ? 8u : 64u;
   const uint32_t lg = 4 + ilog2(S.mbs), cols = 1u << lg, count = 16 * cols;
   const uint32_t x0 = 16u * S.mb_x, width = F.width;
-  const uint16_t *alpha = (const uint16_t *)F.in_alpha;
+  const GLOBAL uint16_t *alpha = (const GLOBAL uint16_t *)(uintptr_t)F.in_alpha;
   const uint32_t ident = F.rescale_identity, max_in = F.max_in;
   const uint64_t m = F.rescale_m;
   for (uint32_t i = L; i < count; i += 64) {
@@ -751,10 +757,10 @@ __device__ __forceinline__ void ring_put(uint32_t *ring, uint32_t o, uint64_t v,
 }
 
 // Stores bytes [lo, hi) of big-endian ring word `w` at `A + 4 w`.
-__device__ __forceinline__ void store_word(uint8_t *A, uint32_t w, uint32_t val, uint32_t lo,
+__device__ __forceinline__ void store_word(GLOBAL uint8_t *A, uint32_t w, uint32_t val, uint32_t lo,
                                            uint32_t hi) {
   if (lo == 0 && hi == 4) {
-    *(uint32_t *)(A + 4 * (size_t)w) = __builtin_bswap32(val);
+    *(GLOBAL uint32_t *)(A + 4 * (size_t)w) = __builtin_bswap32(val);
   } else {
     for (uint32_t b = lo; b < hi; b++) A[4 * (size_t)w + b] = (uint8_t)(val >> (24 - 8 * b));
   }
@@ -777,11 +783,11 @@ extern "C" __global__ void __launch_bounds__(64)
   const EncSlice S = slices[g];
   const EncPic &P = pics[S.pic];
   const EncFrame &F = frames[P.frame];
-  uint8_t *out = (uint8_t *)F.out;
+  GLOBAL uint8_t *out = (GLOBAL uint8_t *)(uintptr_t)F.out;
   const uint32_t k = g - P.slice_base;
   const uint32_t u = used[g], qi = qidx[g], hb = P.header_bytes;
   if (L == 0) {
-    uint8_t *te = out + P.out_off + 8 + 2ull * k;
+    GLOBAL uint8_t *te = out + P.out_off + 8 + 2ull * k;
     te[0] = (uint8_t)(u >> 8);
     te[1] = (uint8_t)u;
   }
@@ -790,7 +796,7 @@ extern "C" __global__ void __launch_bounds__(64)
     const uint32_t n = P.n_slices;
     const uint64_t end = (S.pic + 1 < F.first_pic + F.n_pics) ? pics[S.pic + 1].out_off : frame_size[P.frame];
     const uint32_t psz = (uint32_t)(end - P.out_off);
-    uint8_t *ph = out + P.out_off;
+    GLOBAL uint8_t *ph = out + P.out_off;
     ph[0] = 8 << 3;
     ph[1] = (uint8_t)(psz >> 24);
     ph[2] = (uint8_t)(psz >> 16);
@@ -813,7 +819,7 @@ extern "C" __global__ void __launch_bounds__(64)
     if (L == 0)
       for (uint32_t p = F.first_pic; p < F.first_pic + F.n_pics; p++) {
         if (pics[p].n_slices) continue;
-        uint8_t *ph = out + pics[p].out_off;
+        GLOBAL uint8_t *ph = out + pics[p].out_off;
         ph[0] = 8 << 3;
         ph[1] = 0; ph[2] = 0; ph[3] = 0; ph[4] = 8;
         ph[5] = 0; ph[6] = 0;
@@ -823,9 +829,9 @@ extern "C" __global__ void __launch_bounds__(64)
   for (uint32_t i = L; i < RING; i += 64) ring[i] = 0;
   __syncthreads();
 
-  uint8_t *const sa = out + slice_out[g] + hb;  // first byte of the colour data
+  GLOBAL uint8_t *const sa = out + slice_out[g] + hb;  // first byte of the colour data
   const uint32_t lead = (uint32_t)((uintptr_t)sa & 3);
-  uint8_t *const A = sa - lead;
+  GLOBAL uint8_t *const A = sa - lead;
   uint32_t pos = 8 * lead, flushed = 0;
   const uint32_t cpm = F.chroma == 2 ? 2u : 4u;
   const float *qt = qtabs + (size_t)P.qtab * 225 * 128;
