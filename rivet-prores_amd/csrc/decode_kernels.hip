@@ -787,27 +787,40 @@ __device__ __forceinline__ void alpha_slice(
   // current one, and never passes the end of the stream).
   const uint32_t len = uni(R.len);
   const uintptr_t a = (uintptr_t)(((uint64_t)uni((uint32_t)(R.stream >> 32)) << 32) | uni((uint32_t)R.stream));
-  const uint32_t m = uni((uint32_t)(a & 7));
+  // A short stream comes from the record (from its first byte: m = 0).
+  const bool from_rec = len <= ALPHA_HEAD_BYTES;
+  const uint32_t m = from_rec ? 0u : uni((uint32_t)(a & 7));
   const GLOBAL uint64_t *src = (const GLOBAL uint64_t *)(a - m);
   const uint32_t end_byte = m + len;  // bytes of stream data end here
   const uint32_t have = (end_byte + 7) / 8;
   const uint32_t stage = have + 2 < ALPHA_WORDS ? have + 2 : ALPHA_WORDS;
-  // Four words a lane in flight at a time.
-  for (uint32_t j0 = 0; j0 < stage; j0 += 256) {
-    uint64_t raw[4];
-#pragma unroll
-    for (uint32_t u = 0; u < 4; u++) {
-      const uint32_t j = j0 + 64 * u + L;
-      raw[u] = j < have && j < stage ? src[j] : 0ull;
+  if (from_rec) {
+    if (L < stage) {
+      const uint32_t h0 = uni(R.head[0]), h1 = uni(R.head[1]), h2 = uni(R.head[2]), h3 = uni(R.head[3]);
+      uint64_t w = L == 0 ? ((uint64_t)h1 << 32 | h0) : L == 1 ? ((uint64_t)h3 << 32 | h2) : 0ull;
+      w = L < have ? __builtin_bswap64(w) : 0ull;
+      const uint32_t hi = 8 * L + 8;
+      if (L < have && hi > end_byte) w &= ~0ull << (8 * (hi - end_byte));
+      words[L] = w;
     }
+  } else {
+    // Four words a lane in flight at a time.
+    for (uint32_t j0 = 0; j0 < stage; j0 += 256) {
+      uint64_t raw[4];
 #pragma unroll
-    for (uint32_t u = 0; u < 4; u++) {
-      const uint32_t j = j0 + 64 * u + L;
-      uint64_t w = __builtin_bswap64(raw[u]);
-      const uint32_t lo = 8 * j, hi = lo + 8;
-      if (lo < m) w &= ~0ull >> (8 * (m - lo));
-      if (hi > end_byte) w &= hi - end_byte >= 8 ? 0ull : ~0ull << (8 * (hi - end_byte));
-      if (j < stage) words[j] = w;
+      for (uint32_t u = 0; u < 4; u++) {
+        const uint32_t j = j0 + 64 * u + L;
+        raw[u] = j < have && j < stage ? src[j] : 0ull;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 4; u++) {
+        const uint32_t j = j0 + 64 * u + L;
+        uint64_t w = __builtin_bswap64(raw[u]);
+        const uint32_t lo = 8 * j, hi = lo + 8;
+        if (lo < m) w &= ~0ull >> (8 * (m - lo));
+        if (hi > end_byte) w &= hi - end_byte >= 8 ? 0ull : ~0ull << (8 * (hi - end_byte));
+        if (j < stage) words[j] = w;
+      }
     }
   }
   __syncthreads();
@@ -1048,7 +1061,7 @@ extern "C" __global__ void __launch_bounds__(ST_CHUNK)
     // long alpha streams at the front, the rest from the back; each with
     // where its alpha stream is. A slice outside the picture reads nothing.
     bool listed = false, heavy = false;
-    uint32_t alen = 0;
+    uint32_t alen = 0, ahead[4] = {0, 0, 0, 0};
     uint64_t astream = 0;
     if (has_alpha && valid && sz >= 8 && data0 + at + sz <= P.picture_size) {
       const GLOBAL uint8_t *sl = (const GLOBAL uint8_t *)(uintptr_t)(P.pic_off + data0 + at);
@@ -1061,6 +1074,14 @@ extern "C" __global__ void __launch_bounds__(ST_CHUNK)
       alen = listed ? sz - hs - c : 0u;
       heavy = listed && alen > ALPHA_HEAVY_BYTES;
       astream = P.pic_off + data0 + at + hs + c;
+      if (listed && alen <= ALPHA_HEAD_BYTES) {
+        // Inside the packet and its 64 readable bytes past the end.
+        const GLOBAL uint8_t *h = (const GLOBAL uint8_t *)(uintptr_t)astream;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++)
+          ahead[i] = (uint32_t)h[4 * i] | (uint32_t)h[4 * i + 1] << 8 | (uint32_t)h[4 * i + 2] << 16 |
+                     (uint32_t)h[4 * i + 3] << 24;
+      }
     }
     if (valid) {
       const uint32_t row = k / P.row_slices;
@@ -1112,6 +1133,8 @@ extern "C" __global__ void __launch_bounds__(ST_CHUNK)
         ar.frame = P.frame;
         ar.mb = (uint32_t)rs.mb_x | (k / P.row_slices) << 16;
         ar.mbs = rs.mbs;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) ar.head[i] = ahead[i];
         alpha_order[heavy ? hb + (uint32_t)__popcll(hm & below) : G - 1 - lb - (uint32_t)__popcll(lm & below)] = ar;
       }
     }

@@ -103,8 +103,12 @@ struct AlphaRec {
   uint32_t frame;    // DecFrame index
   uint32_t mb;       // mb_x | mb_y << 16
   uint32_t mbs;      // macroblocks
+  uint32_t head[4];  // the stream's first ALPHA_HEAD_BYTES bytes, in memory order
 };
-static_assert(sizeof(AlphaRec) == 32, "AlphaRec layout");
+static_assert(sizeof(AlphaRec) == 48, "AlphaRec layout");
+// A stream this short is decoded from its record alone (most alpha slices:
+// a constant slice is one difference and one run, 5 bytes).
+constexpr uint32_t ALPHA_HEAD_BYTES = 16;
 
 // Counts of a batch and the heads of its work queues, in device memory so
 // that the launch arguments stay the same from one batch to the next (a
