@@ -209,7 +209,17 @@ void prores_server::worker(int which) {
         for (size_t i = 0; i < ok.size(); i++)
           jobs[i] = prores_dev_decode_job{ok[i]->packet.data(), ok[i]->packet.size(), (const uint8_t *)ok[i]->dev_in,
                                           (uint16_t *)ok[i]->dev_out, (uint16_t *)ok[i]->dev_out_alpha};
-        (void)decode_batch_on(*dev, L, cfg.decode_bit_depth, jobs.size(), jobs.data(), nullptr, errs.data());
+        // The device's batch tuning, launched directly: a graph capture on
+        // this worker's streams is invalidated (and the batch fails) when
+        // another thread of the process copies through the legacy stream
+        // meanwhile, as clients reading their results do.
+        prores_decode_tuning t;
+        {
+          std::lock_guard<std::mutex> g(dev->mu);
+          t = dev->tuning;
+        }
+        t.graph = 0;
+        (void)decode_batch_on(*dev, L, cfg.decode_bit_depth, jobs.size(), jobs.data(), &t, errs.data());
       } else {
         std::vector<prores_dev_encode_job> jobs(ok.size());
         for (size_t i = 0; i < ok.size(); i++) {
