@@ -209,7 +209,10 @@ prores_status prores_decode_device_batch(const prores_decoder *d, size_t n,
  *                batch has at least 12 streams' waves a CU).
  *   graph        1: the launch sequence is captured once as a hipGraph and
  *                replayed while the workspace and launch shapes stay the
- *                same (with persistent = 1 the shapes never change). */
+ *                same (with persistent = 1 the shapes never change). A
+ *                capture fails, and its batch with it, if another thread
+ *                copies through the legacy (null) stream meanwhile; the
+ *                serving loop therefore never captures. */
 typedef struct {
   uint32_t persistent, chunk, idct_blocks, entropy_wg, graph;
 } prores_decode_tuning;
