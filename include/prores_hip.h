@@ -209,10 +209,13 @@ prores_status prores_decode_device_batch(const prores_decoder *d, size_t n,
  *                batch has at least 12 streams' waves a CU).
  *   graph        1: the launch sequence is captured once as a hipGraph and
  *                replayed while the workspace and launch shapes stay the
- *                same (with persistent = 1 the shapes never change). A
- *                capture fails, and its batch with it, if another thread
- *                copies through the legacy (null) stream meanwhile; the
- *                serving loop therefore never captures. */
+ *                same (with persistent = 1 the shapes never change). The
+ *                library's own copies never use the legacy (null) stream;
+ *                a capture that fails all the same (another thread's work
+ *                invalidated it) is not retried for that shape, and its
+ *                batches run as stream launches with the same results.
+ *                prores_decode_host_batch and the serving loop always use
+ *                stream launches. */
 typedef struct {
   uint32_t persistent, chunk, idct_blocks, entropy_wg, graph;
 } prores_decode_tuning;
@@ -248,6 +251,12 @@ typedef struct {
 /* Encodes n frames with one launch sequence (Encoder::encode semantics per
  * frame, src/encode.rs:223-287); errs[i] receives frame i's result. */
 prores_status prores_encode_device_batch(size_t n, prores_dev_encode_job *jobs, prores_error *errs);
+
+/* Graph capture counters of the decode batches on the current device, all
+ * lanes: [0] captures made, [1] captures that failed (their batches ran as
+ * stream launches), [2] batches launched as a captured graph. No
+ * counterpart in the reference. */
+void prores_graph_counters(uint64_t *out3);
 
 /* Microseconds of the last decode batch's kernels by stage, measured with
  * HIP events on the library's stream: [0] slice table, [1] colour entropy

@@ -101,6 +101,9 @@ struct Workspace {
   size_t cap_frames = 0, cap_pics = 0, cap_rows = 0, cap_segs = 0, cap_chunks = 0, cap_cost = 0, cap_np = 0;
   hipGraphExec_t graph = nullptr;
   GraphKey key;
+  bool graph_failed = false;    // the capture for `key` failed
+  // Captures made, captures that failed, graph launches (this lane).
+  uint64_t graph_captures = 0, graph_failures = 0, graph_replays = 0;
 };
 Workspace &workspace(Lane &L) {
   if (!L.dec) L.dec = new Workspace();
@@ -460,9 +463,11 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     }
     return x;
   };
-  if (np == 0) {
-    L.dec_us[0] = L.dec_us[1] = L.dec_us[2] = L.dec_us[3] = L.dec_us[4] = 0;
-  } else if (t.graph) {
+  // A capture that fails (another thread's work invalidated it, or the
+  // runtime refused a node) leaves the stream out of capture mode with
+  // nothing launched: the batch then runs as stream launches.
+  bool replay = t.graph != 0;
+  if (np && replay) {
     GraphKey key;
     memset(&key, 0, sizeof key);
     void *ptrs[17] = {B, pics, rows, slices, status, frames, tables, coefs, wide, serr, ferr, itemp, segp, aord,
@@ -473,31 +478,39 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     key.idct_lds = idct_lds;
     key.t = t;
     key.alpha = b.any_alpha;
-    if (!ws.graph || !(key == ws.key)) {
+    // Captured once per key; a key whose capture failed runs as stream
+    // launches without trying again.
+    if (!(key == ws.key) || (!ws.graph && !ws.graph_failed)) {
       if (ws.graph) {
         (void)hipGraphExecDestroy(ws.graph);
         ws.graph = nullptr;
       }
       hipGraph_t graph = nullptr;
-      if (!(e = check(hipStreamBeginCapture(L.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture")).ok())
-        return e;
-      Err ce = enqueue(false);
-      Err ee = check(hipStreamEndCapture(L.stream, &graph), "hipStreamEndCapture");
-      if (!ce.ok()) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return ce;
+      bool captured = false;
+      if (hipStreamBeginCapture(L.stream, hipStreamCaptureModeRelaxed) == hipSuccess) {
+        const Err ce = enqueue(false);
+        const hipError_t ee = hipStreamEndCapture(L.stream, &graph);
+        captured = ce.ok() && ee == hipSuccess && graph;
       }
-      if (!ee.ok()) return ee;
-      e = check(hipGraphInstantiate(&ws.graph, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-      (void)hipGraphDestroy(graph);
-      if (!e.ok()) {
+      if (captured && hipGraphInstantiate(&ws.graph, graph, nullptr, nullptr, 0) != hipSuccess) {
         ws.graph = nullptr;
-        return e;
+        captured = false;
       }
+      if (graph) (void)hipGraphDestroy(graph);
+      if (!captured) (void)hipGetLastError();  // the failed capture's error is not this batch's
       ws.key = key;
+      ws.graph_failed = !captured;
+      ws.graph_captures += captured;
+      ws.graph_failures += !captured;
     }
+    replay = ws.graph != nullptr;
+  }
+  if (np == 0) {
+    L.dec_us[0] = L.dec_us[1] = L.dec_us[2] = L.dec_us[3] = L.dec_us[4] = 0;
+  } else if (replay) {
     (void)hipEventRecord(L.ev[0], L.stream);
     if (!(e = check(hipGraphLaunch(ws.graph, L.stream), "hipGraphLaunch")).ok()) return e;
+    ws.graph_replays++;
     (void)hipEventRecord(L.ev[4], L.stream);
   } else {
     if (!(e = enqueue(true)).ok()) return e;
@@ -513,7 +526,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
     // Stages: slice table, colour entropy, alpha (from the end of the slice
     // table to its own end, beside the other two), IDCT, whole batch.
     float ms[4] = {0, 0, 0, 0}, all = 0;
-    if (!t.graph) {
+    if (!replay) {
       (void)hipEventElapsedTime(&ms[0], L.ev[0], L.ev[1]);
       (void)hipEventElapsedTime(&ms[1], L.ev[1], L.ev[2]);
       if (b.any_alpha) (void)hipEventElapsedTime(&ms[2], L.ev[1], L.ev[5]);
@@ -538,8 +551,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
       } else if (fe[p] != 0xFFFFFFFFu) {
         uint32_t words[4];
         const uint64_t g = b.pics[p].slice_base + (uint64_t)fe[p];
-        if (!(e = check(hipMemcpy(words, (uint8_t *)ws.slice_err.p + 16 * g, 16, hipMemcpyDeviceToHost),
-                        "slice_err")).ok())
+        if (!(e = copy_on(L, words, (uint8_t *)ws.slice_err.p + 16 * g, 16, hipMemcpyDeviceToHost)).ok())
           return e;
         for (int c = 0; c < 4; c++)
           if (words[c]) {
@@ -888,6 +900,9 @@ prores_status prores_decode_host_batch(const prores_decoder *d, size_t n, const 
     std::lock_guard<std::mutex> g(dev->mu);
     t = dev->tuning;
   }
+  // Stream launches: the two lanes' threads copy while the other launches,
+  // and a replayed graph saves nothing next to the link's transfer time.
+  t.graph = 0;
   auto work = [&](int which, Err &fail) {
     Lane *L = nullptr;
     if (!(fail = lane(*dev, HOST_LANES[which], L)).ok()) return;
@@ -1079,7 +1094,27 @@ void prores_last_batch_timing(double *us5) {
   for (int i = 0; i < 5; i++) us5[i] = lane0->dec_us[i];
 }
 
-const char *prores_version(void) { return "prores-hip 0.1 gfx950"; }
+void prores_graph_counters(uint64_t *out3) {
+  out3[0] = out3[1] = out3[2] = 0;
+  Device *dev = nullptr;
+  if (!device(dev).ok()) return;
+  for (int i = 0; i < MAX_LANES; i++) {
+    Lane *L = nullptr;
+    {
+      std::lock_guard<std::mutex> g(dev->mu);
+      L = dev->lanes[i];
+    }
+    if (!L) continue;
+    std::lock_guard<std::mutex> g(L->mu);
+    if (!L->dec) continue;
+    const Workspace &ws = *(const Workspace *)L->dec;
+    out3[0] += ws.graph_captures;
+    out3[1] += ws.graph_failures;
+    out3[2] += ws.graph_replays;
+  }
+}
+
+const char *prores_version(void) { return "prores-hip 0.2 gfx950"; }
 
 void prores_entropy_tables(uint16_t *level, uint16_t *run) {
   static DevTables t;  // built once; the tables are constant

@@ -1018,7 +1018,8 @@ extern "C" __global__ void __launch_bounds__(ST_CHUNK)
   const uint32_t tid = threadIdx.x;
   if (blockIdx.x == 0 && tid < 4) B->q[tid] = 0;
   const uint32_t nc = B->n_chunks, G = B->n_slices;
-  __shared__ uint32_t wsum[4], wpre[4];
+  __shared__ uint32_t wsum[4];
+  __shared__ uint64_t wpre[4];
   const uint32_t lane = tid & 63, wave = tid >> 6;
   for (uint32_t x = blockIdx.x; x < nc; x += gridDim.x) {
     // The chunk: its picture and its first slice (a multiple of ST_CHUNK).
@@ -1030,15 +1031,20 @@ extern "C" __global__ void __launch_bounds__(ST_CHUNK)
     const uint64_t data0 = P.header_size + 2ull * P.n_slices;  // relative to the picture
     const bool valid = k < P.n_slices;
     // Table entries before the chunk: thread t sums entries t, t + 256, ...
-    uint32_t pre = 0;
+    // In 64 bits: a picture of more than 65,537 slices can hold more than
+    // 2^32 bytes of sizes before a chunk, and a wrapped sum could pass the
+    // bounds check below.
+    uint64_t pre = 0;
 #pragma unroll 4
     for (uint32_t i = tid; i < k0; i += ST_CHUNK) pre += (uint32_t)tab[2 * i] << 8 | tab[2 * i + 1];
     const uint32_t sz = valid ? ((uint32_t)tab[2 * k] << 8 | tab[2 * k + 1]) : 0u;
     RowSlot rs = {0, 0, 0};
     if (valid) rs = rows[P.layout_base + (k - (k / P.row_slices) * P.row_slices)];
-    uint32_t incl = sz, psum = pre;
+    uint32_t incl = sz;  // at most 256 sizes of at most 65535: no wrap
+    uint64_t psum = pre;
     for (uint32_t d = 1; d < 64; d <<= 1) {
-      const uint32_t a = __shfl_up(incl, d, 64), b = __shfl_xor(psum, d, 64);
+      const uint32_t a = __shfl_up(incl, d, 64);
+      const uint64_t b = __shfl_xor(psum, d, 64);
       if (lane >= d) incl += a;
       psum += b;
     }
@@ -1048,7 +1054,7 @@ extern "C" __global__ void __launch_bounds__(ST_CHUNK)
     uint32_t before = 0;
     for (uint32_t w = 0; w < wave; w++) before += wsum[w];
     const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    const uint64_t carry = (uint64_t)wpre[0] + wpre[1] + wpre[2] + wpre[3];
+    const uint64_t carry = wpre[0] + wpre[1] + wpre[2] + wpre[3];
     const uint64_t at = carry + before + incl - sz;
     if (tid == 0) {
       if (k0 == 0) pic_first_err[p] = 0xFFFFFFFFu;

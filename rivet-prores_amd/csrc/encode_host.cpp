@@ -335,7 +335,7 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   {
     // Diagnostics: sizes the share-out computed past the window.
     std::vector<EncState> st(NP);
-    if (NP && hipMemcpy(st.data(), ws.states.p, NP * sizeof(EncState), hipMemcpyDeviceToHost) == hipSuccess) {
+    if (NP && copy_on(L, st.data(), ws.states.p, NP * sizeof(EncState), hipMemcpyDeviceToHost).ok()) {
       uint64_t total = 0, most = 0, uni = 0;
       for (const EncState &x : st) {
         total += x.fallback;
@@ -467,7 +467,7 @@ prores_status prores_encode(const prores_config *cfg, const prores_frame *frame,
       *out = (uint8_t *)malloc(sizes[0] ? sizes[0] : 1);
       if (!*out) e = Err{PRORES_ERR_DEVICE, "host allocation failed"};
     }
-    if (e.ok()) e = check(hipMemcpy(*out, (uint8_t *)ws.out.p + offsets[0], sizes[0], hipMemcpyDeviceToHost), "D2H packet");
+    if (e.ok()) e = copy_on(*lane0, *out, (uint8_t *)ws.out.p + offsets[0], sizes[0], hipMemcpyDeviceToHost);
     if (e.ok()) {
       *out_len = sizes[0];
     } else {

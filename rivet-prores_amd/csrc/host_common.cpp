@@ -257,6 +257,56 @@ Err DevBuf::reserve(size_t bytes) {
   return Err{};
 }
 
+// Streams for copies outside a lane: non-blocking, default priority, kept in
+// an idle list per device, so copies from many threads run at once and none
+// goes through the legacy stream (whose implicit synchronisation with other
+// streams can invalidate a graph capture running on a lane).
+namespace {
+struct CopyStreams {
+  std::mutex mu;
+  std::vector<hipStream_t> idle[64];
+};
+CopyStreams &copy_streams() {
+  static CopyStreams *c = new CopyStreams();  // never destroyed: copies may run at exit
+  return *c;
+}
+}  // namespace
+
+Err copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  if (!bytes) return Err{};
+  int id = 0;
+  hipError_t he = hipGetDevice(&id);
+  if (he != hipSuccess) return device_error("hipGetDevice", he);
+  if (id < 0 || id >= 64) return Err{PRORES_ERR_DEVICE, "device index out of range"};
+  CopyStreams &cs = copy_streams();
+  hipStream_t s = nullptr;
+  {
+    std::lock_guard<std::mutex> g(cs.mu);
+    if (!cs.idle[id].empty()) {
+      s = cs.idle[id].back();
+      cs.idle[id].pop_back();
+    }
+  }
+  if (!s && (he = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess)
+    return device_error("hipStreamCreate(copy)", he);
+  he = hipMemcpyAsync(dst, src, bytes, kind, s);
+  const hipError_t se = hipStreamSynchronize(s);
+  {
+    std::lock_guard<std::mutex> g(cs.mu);
+    cs.idle[id].push_back(s);
+  }
+  if (he != hipSuccess) return device_error("hipMemcpyAsync", he);
+  if (se != hipSuccess) return device_error("hipStreamSynchronize(copy)", se);
+  return Err{};
+}
+
+Err copy_on(Lane &L, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  if (!bytes) return Err{};
+  Err e = check(hipMemcpyAsync(dst, src, bytes, kind, L.stream), "hipMemcpyAsync");
+  Err s = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize");
+  return e.ok() ? s : e;
+}
+
 Err launch_on(hipStream_t s, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args,
               unsigned lds) {
   if (gx == 0 || gy == 0) return Err{};
@@ -288,8 +338,7 @@ static Err init_device(Device &d, int id) {
   DevTables host;
   build_tables(host);
   if (!(e = check(hipMalloc(&d.tables, sizeof(DevTables)), "hipMalloc(tables)")).ok()) return e;
-  if (!(e = check(hipMemcpy(d.tables, &host, sizeof host, hipMemcpyHostToDevice), "tables")).ok())
-    return e;
+  if (!(e = copy_sync(d.tables, &host, sizeof host, hipMemcpyHostToDevice)).ok()) return e;
   prores_decode_tuning_default(&d.tuning);
   return Err{};
 }
@@ -411,10 +460,9 @@ prores_status prores_memcpy(void *dst, const void *src, size_t bytes, int32_t ki
   const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
                           : kind == 2 ? hipMemcpyDeviceToHost
                                       : hipMemcpyDeviceToDevice;
-  const hipError_t e = hipMemcpy(dst, src, bytes, k);
-  if (e == hipSuccess) return PRORES_OK;
-  snprintf(last_memcpy_error, sizeof last_memcpy_error, "hipMemcpy of %zu bytes: %s (%d)", bytes,
-           hipGetErrorString(e), (int)e);
+  const Err e = copy_sync(dst, src, bytes, k);
+  if (e.ok()) return PRORES_OK;
+  snprintf(last_memcpy_error, sizeof last_memcpy_error, "copy of %zu bytes: %s", bytes, e.msg.c_str());
   return PRORES_ERR_DEVICE;
 }
 

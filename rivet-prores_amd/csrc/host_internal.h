@@ -126,6 +126,11 @@ Err launch(Lane &L, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void
 Err launch_on(hipStream_t s, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args,
               unsigned lds = 0);
 Err check(hipError_t e, const char *where);
+// Copies that wait for their completion without touching the legacy stream:
+// on a library copy stream of the current device, or on a lane's stream
+// (the caller holds L.mu).
+Err copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind);
+Err copy_on(Lane &L, void *dst, const void *src, size_t bytes, hipMemcpyKind kind);
 
 void build_tables(DevTables &t);
 

@@ -209,10 +209,11 @@ void prores_server::worker(int which) {
         for (size_t i = 0; i < ok.size(); i++)
           jobs[i] = prores_dev_decode_job{ok[i]->packet.data(), ok[i]->packet.size(), (const uint8_t *)ok[i]->dev_in,
                                           (uint16_t *)ok[i]->dev_out, (uint16_t *)ok[i]->dev_out_alpha};
-        // The device's batch tuning, launched directly: a graph capture on
-        // this worker's streams is invalidated (and the batch fails) when
-        // another thread of the process copies through the legacy stream
-        // meanwhile, as clients reading their results do.
+        // The device's batch tuning, launched directly: a capture on this
+        // worker's streams can be invalidated by other threads of the
+        // process (clients reading their results through the legacy
+        // stream), and the batches of a serving loop change shape too often
+        // for a captured graph to be replayed.
         prores_decode_tuning t;
         {
           std::lock_guard<std::mutex> g(dev->mu);
@@ -220,6 +221,10 @@ void prores_server::worker(int which) {
         }
         t.graph = 0;
         (void)decode_batch_on(*dev, L, cfg.decode_bit_depth, jobs.size(), jobs.data(), &t, errs.data());
+        // A batch that failed part-way may still have copies or kernels on
+        // the stream: none may touch the inputs once they go back to the pool.
+        (void)hipStreamSynchronize(L.stream);
+        (void)hipStreamSynchronize(L.aux);
       } else {
         std::vector<prores_dev_encode_job> jobs(ok.size());
         for (size_t i = 0; i < ok.size(); i++) {

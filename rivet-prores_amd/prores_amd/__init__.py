@@ -153,7 +153,7 @@ EXPORTS = [
     "prores_server_submit_decode", "prores_server_submit_encode", "prores_server_poll",
     "prores_server_release", "prores_server_get_stats", "prores_last_encode_counters",
     "prores_entropy_tables", "prores_memcpy_error", "prores_decode_host_batch", "prores_find_frames",
-    "prores_fnv1a",
+    "prores_fnv1a", "prores_graph_counters",
 ]
 
 _lib = None
@@ -208,6 +208,7 @@ def lib():
         L.prores_last_batch_timing.argtypes = [P(C.c_double)]
         L.prores_last_encode_timing.argtypes = [P(C.c_double)]
         L.prores_last_encode_counters.argtypes = [P(C.c_uint64)]
+        L.prores_graph_counters.argtypes = [P(C.c_uint64)]
         L.prores_version.restype = C.c_char_p
         L.prores_find_frames.argtypes = [C.c_char_p, C.c_size_t, P(C.c_uint64), P(C.c_uint64), C.c_size_t]
         L.prores_find_frames.restype = C.c_size_t
@@ -579,6 +580,14 @@ def last_encode_counters():
     mean uniform qi] of the last encode batch."""
     out = (C.c_uint64 * 4)()
     lib().prores_last_encode_counters(out)
+    return list(out)
+
+
+def graph_counters():
+    """[captures made, captures that failed, graph launches] of the decode
+    batches on the current device."""
+    out = (C.c_uint64 * 3)()
+    lib().prores_graph_counters(out)
     return list(out)
 
 

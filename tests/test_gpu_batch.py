@@ -24,6 +24,7 @@ CASES = [
     (1024, 540, 3, 12, 4, 3, 0, 1, True),   # 4444, 8-bit alpha, loaded matrices
     (333, 75, 3, 12, 5, 3, 2, 2, False),    # odd size, bottom field first, 16-bit alpha
     (96, 32, 2, 10, 2, 3, 0, 0, False),     # tiny
+    (4096, 2160, 3, 12, 4, 3, 0, 1, True),  # the batch's 4096x2160 4444 kind: 8-bit alpha, loaded matrices
 ]
 
 

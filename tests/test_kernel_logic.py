@@ -220,3 +220,76 @@ def test_entropy_tables_equal_the_bit_serial_reader():
             assert level[b * 1024 + p] == want, ("level", b, p, r)
             covered += want != 0
     assert covered > 8000
+
+
+# ---------------------------------------------------------------------------
+# pd_slice_table (csrc/decode_kernels.hip): a workgroup per chunk of 256
+# slices. Thread t sums table entries t, t + 256, ... before the chunk; the 64
+# lanes of each wave add their sums (xor shuffles); the four wave sums are the
+# chunk's carry, and the chunk holding the table's last entry checks
+# data0 + carry + its own sizes against picture_size (the reference's walk,
+# src/decode.rs:202-221, errs at the first slice that ends past it; with
+# sizes >= 0 that is "the last end passes picture_size").
+
+
+def chunk_verdict(sizes, data0, picture_size, bits):
+    """The last chunk's bounds check with the sums kept in `bits`-bit words
+    (the kernel's per-thread, per-wave and carry words)."""
+    mask = (1 << bits) - 1
+    n = len(sizes)
+    k0 = ((n - 1) // 256) * 256
+    sizes = np.asarray(sizes, dtype=np.uint64)
+    before = sizes[:k0]
+    pad = (-len(before)) % 256
+    per_thread = np.concatenate([before, np.zeros(pad, np.uint64)]).reshape(-1, 256).sum(axis=0)
+    per_thread = [int(v) & mask for v in per_thread]
+    carry = 0
+    for w in range(4):
+        s = 0
+        for t in range(64 * w, 64 * w + 64):
+            s = (s + per_thread[t]) & mask
+        carry += s  # the four wave sums are added in 64 bits
+    total = int(sizes[k0:].sum())
+    return data0 + carry + total > picture_size  # True: PIC_TABLE_ERR
+
+
+def reference_walk(sizes, data0, picture_size):
+    """src/decode.rs:211-221: True when some slice runs past picture_size."""
+    at = data0
+    for s in sizes:
+        at += int(s)
+        if at > picture_size:
+            return True
+    return False
+
+
+def crafted_wrap_table(n=266_240):
+    """Sizes whose per-wave sums before the last chunk are exactly 2^32 each
+    and whose last chunk is empty: 32-bit sums wrap to a carry of 0."""
+    k0 = ((n - 1) // 256) * 256
+    sizes = np.zeros(n, np.uint64)
+    idx = np.arange(k0)
+    for w in range(4):
+        sel = idx[(idx % 256) // 64 == w]
+        base, rem = divmod(1 << 32, len(sel))
+        assert base + 1 <= 65535
+        sizes[sel] = base
+        sizes[sel[:rem]] += 1
+    return sizes
+
+
+def test_slice_table_carry_does_not_wrap():
+    sizes = crafted_wrap_table()
+    data0 = 8 + 2 * len(sizes)
+    picture_size = data0 + 64
+    assert reference_walk(sizes, data0, picture_size)
+    # 32-bit sums would pass the check (the bug); the kernel's 64-bit sums fail it.
+    assert not chunk_verdict(sizes, data0, picture_size, 32)
+    assert chunk_verdict(sizes, data0, picture_size, 64)
+    # Ordinary tables: the 64-bit verdict equals the reference's walk.
+    rng = np.random.default_rng(5)
+    for n in (1, 255, 256, 257, 4050, 70_000):
+        s = rng.integers(0, 65536, n).astype(np.uint64)
+        total = int(s.sum())
+        for ps in (data0, total + data0 - 1, total + data0, total + data0 + 5):
+            assert chunk_verdict(s, data0, ps, 64) == reference_walk(s, data0, ps)
