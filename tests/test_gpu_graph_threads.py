@@ -30,7 +30,7 @@ def _packets():
 def test_graph_batches_beside_copies_and_serving():
     packets = _packets()
     want = [O.decode(p) for p in packets]
-    bufs, jobs = [], []
+    bufs, jobs, keep = [], [], []
     for p in packets:
         hd = P.FrameHeader.parse(p)
         planes = P.layout(hd.width, hd.height, hd.chroma)
@@ -39,6 +39,7 @@ def test_graph_batches_beside_copies_and_serving():
         dp.upload(np.frombuffer(p, np.uint8))
         dd = P.DeviceBuffer(n * 2 + 64)
         bufs.append((dd, n))
+        keep.append(dp)  # the packets must outlive every batch that reads them
         jobs.append((p, dp.ptr, dd.ptr, 0))
     jobs = P.DecodeJobs(jobs)
     enc_frame = O.test_frame(512, 288, 2, 10, 9, 77)
