@@ -112,21 +112,100 @@ def profiled_traffic(workload, kernel):
     return d.get("traffic_bytes", {}).get(kernel)
 
 
-def cpu_baseline(O, packets, budget_s):
-    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-    best = None
-    runs = 0
-    t_start = time.perf_counter()
-    while runs == 0 or (time.perf_counter() - t_start < budget_s and runs < 20):
+def host_cpu():
+    """(model name, logical CPUs of the machine, CPUs this process may use)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return model, os.cpu_count(), allowed
+
+
+def best_rate(fn, units, budget_s, max_runs):
+    """Best-of-N rate (units per second) of fn(), N runs within budget_s (at
+    least one), as the reference's bench keeps its best run
+    (examples/prores_bench.rs:153-172)."""
+    best, runs, t_start = None, 0, time.perf_counter()
+    while runs == 0 or (time.perf_counter() - t_start < budget_s and runs < max_runs):
         t0 = time.perf_counter()
-        for p in packets:
-            O.decode(p, 0, threads)
+        fn()
         dt = time.perf_counter() - t0
         best = dt if best is None else min(best, dt)
         runs += 1
-    return {"value": round(len(packets) / best, 3), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"the {len(packets)} benchmark packets decoded by the C++ oracle's slice-parallel pool "
-                      f"(one task per macroblock row), best of {runs} runs"}
+    return units / best, runs
+
+
+def cpu_baseline(O, packets, c4, c1, budget_s):
+    """The CPU oracle (the reference's algorithm restated in C++, with its
+    slice-parallel pool: one task per macroblock row for decode, chunks of
+    slices for encode) on this host's cores and on one thread, for the
+    headline decode (C2), the 2160p HQ encode (C4) and the 1080p HQ round
+    trip (C1). Samples are bounded so the whole baseline takes about
+    budget_s seconds."""
+    model, nproc, allowed = host_cpu()
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = min(allowed, int(env)) if env and env.isdigit() and int(env) > 0 else allowed
+    share = budget_s / 6.0
+
+    def dec(ps, t):
+        return lambda: [O.decode(p, 0, t) for p in ps]
+
+    def enc(frames, t):
+        return lambda: [O.encode(of, O.config(3, luma=l, chroma=c, threads=t)) for of, l, c in frames]
+
+    c2_all, runs = best_rate(dec(packets, threads), len(packets), share, 20)
+    one = packets[:2]
+    c2_one, runs1 = best_rate(dec(one, 1), len(one), share, 3)
+    c4_all, runs4 = best_rate(enc(c4[:4], threads), len(c4[:4]), share, 10)
+    c4_one, runs41 = best_rate(enc(c4[:1], 1), 1, share, 2)
+    c1_frame, c1_cfg = c1
+
+    def rt(t):
+        cfg = O.config(3, threads=t)
+        return lambda: O.decode(O.encode(c1_frame, cfg), 0, t)
+
+    c1_all, _ = best_rate(rt(threads), 1, share / 2, 20)
+    c1_one, _ = best_rate(rt(1), 1, share / 2, 5)
+    return {"value": round(c2_all, 3), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"decode of the {len(packets)} headline packets (16 x 3840x2160 4444 XQ + 16-bit alpha) "
+                      f"by the C++ oracle's slice-parallel pool, {threads} threads, best of {runs} runs",
+            "cpu_model": model, "nproc": nproc, "cpus_allowed": allowed,
+            "one_thread": {"value": round(c2_one, 3), "unit": "frames/s", "cores": 1,
+                           "sample": f"decode of {len(one)} headline packets, best of {runs1} runs"},
+            "encode_2160p_hq": {"all_cores_fps": round(c4_all, 3), "one_thread_fps": round(c4_one, 3),
+                                "cores": threads,
+                                "sample": f"encode of {len(c4[:4])} (all cores, best of {runs4}) and 1 (one "
+                                          f"thread, best of {runs41}) of the 32 x 3840x2160 422 HQ frames"},
+            "roundtrip_1080p_hq": {"all_cores_fps": round(c1_all, 3), "one_thread_fps": round(c1_one, 3),
+                                   "cores": threads,
+                                   "sample": "encode then decode of the 1920x1080 422 HQ frame"}}
+
+
+def fnv_packets(P, packets):
+    """FNV-1a 64 over the packets in order (examples/prores_bench.rs:173-176)."""
+    h = 0xCBF29CE484222325
+    for p in packets:
+        h = P.fnv1a64(h, p)
+    return f"{h:016x}"
+
+
+def fnv_pictures(P, pictures):
+    """FNV-1a 64 over each picture's samples then its alpha, little-endian
+    u16 (examples/prores_bench.rs:177-183)."""
+    h = 0xCBF29CE484222325
+    for data, alpha in pictures:
+        h = P.fnv1a64(h, np.ascontiguousarray(data, dtype="<u2"))
+        if alpha is not None:
+            h = P.fnv1a64(h, np.ascontiguousarray(alpha, dtype="<u2"))
+    return f"{h:016x}"
 
 
 def timed(fn, steps):
@@ -226,11 +305,15 @@ def leg_mixed_decode(P, synth, steps, warmup, check):
     return out
 
 
-def leg_batch_encode(P, synth, steps, warmup, check):
+def leg_batch_encode(P, synth, steps, warmup, check, keep_for_cpu):
     """32 x 3840x2160 422 HQ from 10-bit input, 8-MB slices; frames 16-31
     use seeded loaded matrices (entries 4-32); every fourth frame adds
     hard-edged rectangles. "batched" is one encode call for the 32 frames,
-    "plain" one call per frame."""
+    "plain" one call per frame. Every packet of the last timed batch is
+    compared with the oracle's encode of the same frame and Config.
+    `roofline` is the whole launch sequence's: algorithmic bytes (planes read,
+    packets written) over the batch's kernel time; `stage_us` breaks that
+    time down kernel by kernel (HIP events on the library's stream)."""
     rng = np.random.default_rng(11)
     frames, items, bufs = [], [], []
     for i in range(32):
@@ -263,35 +346,97 @@ def leg_batch_encode(P, synth, steps, warmup, check):
     for _ in range(warmup):
         batched()
         plain()
-    b_s = timed(batched, steps)
-    stage = P.last_encode_timing()
-    batched()
+    stages = {k: 0.0 for k in P.ENCODE_STAGES}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batched()
+        for k, v in P.last_encode_stages().items():
+            stages[k] += v / steps
+    b_s = (time.perf_counter() - t0) / steps
     sizes = [r[1] for r in last["res"]]
-    # The output arena is reused by the next encode call: keep the sample now.
-    sample = {i: P.download_ptr(*last["res"][i]).tobytes() for i in (0, 17)}
+    # The output arena is reused by the next encode call: download now.
+    got = [P.download_ptr(*r).tobytes() for r in last["res"]]
     target = P.Profile.Hq.target_frame_bytes(3840, 2160)
     p_s = timed(plain, steps)
+    in_bytes = sum(f.data.nbytes for f, _ in frames)
+    out_bytes = sum(sizes)
+    kernels_us = sum(stages[k] for k in P.ENCODE_STAGES[:-1])
+    dom = max(P.ENCODE_STAGES[:-1], key=lambda k: stages[k])
+    gbs = (in_bytes + out_bytes) / (kernels_us * 1e-6) / 1e9
     out = {"workload": "encode 32 x 3840x2160 422 HQ 10-bit, 8-MB slices, half loaded matrices, "
                        "a quarter with hard-edged rectangles",
            "frames": 32, "batched_fps": round(32 / b_s, 2), "plain_fps": round(32 / p_s, 2),
            "batched_over_plain": round(p_s / b_s, 3), "ms_per_batch": round(b_s * 1e3, 3),
-           "stage_us": {"fdct_alpha": round(stage[0], 1), "rate_control": round(stage[1], 1),
-                        "pack": round(stage[2], 1), "all": round(stage[3], 1)},
+           "stage_us": {k: round(v, 1) for k, v in stages.items()},
+           "rate_control_us": round(stages["probes"] + stages["window"] + stages["shareout"] + stages["offsets"], 1),
+           "kernels_us": round(kernels_us, 1),
+           "roofline": {"bound": "hbm", "kernel": "the whole launch sequence (dominant: pe_" + dom + ")",
+                        "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                        "algorithmic_bytes": int(in_bytes + out_bytes)},
            "target_frame_bytes": target, "max_frame_bytes": max(sizes), "min_frame_bytes": min(sizes),
-           "all_at_or_under_target": all(s <= target for s in sizes)}
+           "all_at_or_under_target": all(s <= target for s in sizes),
+           "packets_fnv1a": fnv_packets(P, got)}
     if check:
         import oracle_lib as O
-        ok = True
-        t0 = time.perf_counter()
-        for i in (0, 17):
-            f, cfg = frames[i]
-            oc = O.config(int(cfg.profile), luma=cfg.luma_matrix, chroma=cfg.chroma_matrix)
+        want, oracle_in = [], []
+        for f, cfg in frames:
             of = O.Frame(f.width, f.height, int(f.chroma), f.bit_depth, f.data, None, 0)
-            ok &= O.encode(of, oc) == sample[i]
-        cpu_s = (time.perf_counter() - t0) / 2
-        out["bit_exact_sample"] = {"frames": [0, 17], "bytes_equal_oracle": bool(ok)}
-        out["cpu_oracle_encode_s_per_frame"] = round(cpu_s, 3)
+            oracle_in.append((of, cfg.luma_matrix, cfg.chroma_matrix))
+            want.append(O.encode(of, O.config(3, luma=cfg.luma_matrix, chroma=cfg.chroma_matrix)))
+        same = [g == w for g, w in zip(got, want)]
+        out["bit_exact"] = {"frames": 32, "bytes_equal_oracle": all(same),
+                            "frames_differing": [i for i, x in enumerate(same) if not x]}
+        out["oracle_packets_fnv1a"] = fnv_packets(P, want)
+        keep_for_cpu.extend(oracle_in)
     return out
+
+
+def leg_roundtrip_1080p(P, synth, steps, warmup, check):
+    """The plain round trip of one 1920x1080 progressive 422 HQ frame from
+    10-bit input, 8-MB slices, default matrix (target 917,583 bytes): encode
+    then decode, device-resident, one call each. Both must reproduce the
+    oracle's bytes and samples."""
+    f = synth.frame(1920, 1080, P.ChromaFormat.Yuv422, 10, seed=1)
+    cfg = P.Config(P.Profile.Hq)
+    dd = P.DeviceBuffer(f.data.nbytes + 64)
+    dd.upload(f.data)
+    dp = P.DeviceBuffer(4 << 20)
+    do = P.DeviceBuffer(f.data.nbytes + 64)
+    dec = P.Decoder()
+    last = {}
+
+    def step():
+        r = P.encode_device_batch([(cfg, f, dd.ptr, 0)])[0]
+        if isinstance(r, Exception):
+            raise r
+        P.copy_device(dp.ptr, r[0], r[1])
+        last["n"] = r[1]
+        jobs = P.DecodeJobs([(r[1], dp.ptr, do.ptr, 0)])
+        if dec.decode_device_batch(jobs)[0] is not None:
+            raise RuntimeError("1080p round trip decode failed")
+
+    for _ in range(warmup):
+        step()
+    rt_s = timed(step, steps)
+    packet = dp.download(last["n"]).tobytes()
+    picture = do.download(f.data.nbytes, dtype=np.uint16)
+    out = {"workload": "round trip (encode, then decode) of one 1920x1080 422 HQ 10-bit frame, 8-MB slices",
+           "roundtrip_fps": round(1 / rt_s, 2), "ms_per_roundtrip": round(rt_s * 1e3, 3),
+           "packet_bytes": len(packet), "target_frame_bytes": P.Profile.Hq.target_frame_bytes(1920, 1080),
+           "packets_fnv1a": fnv_packets(P, [packet]), "pictures_fnv1a": fnv_pictures(P, [(picture, None)])}
+    c1 = None
+    if check:
+        import oracle_lib as O
+        of = O.Frame(1920, 1080, 2, 10, f.data, None, 0)
+        want = O.encode(of, O.config(3))
+        od = O.decode(want)
+        out["bit_exact"] = {"bytes_equal_oracle": want == packet,
+                            "samples_equal_oracle": bool(np.array_equal(od.data, picture))}
+        out["oracle_packets_fnv1a"] = fnv_packets(P, [want])
+        out["oracle_pictures_fnv1a"] = fnv_pictures(P, [(od.data, None)])
+        c1 = (of, None)
+    return out, c1
 
 
 SERVE_SIZES = [(720, 486, 1), (1280, 720, 0), (1920, 1080, 0), (2048, 1080, 0), (3840, 2160, 0), (4096, 2160, 0)]
@@ -431,14 +576,17 @@ def main():
     ap.add_argument("--frames", type=int, default=16)
     ap.add_argument("--width", type=int, default=3840)
     ap.add_argument("--height", type=int, default=2160)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=18.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="mixed,encode,serve",
-                    help="extra legs after the headline decode: mixed (96 mixed frames, batched vs plain "
-                         "decode), encode (32-frame batched encode), serve (the serving loop), "
-                         "comma-separated, or none")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--legs", default="roundtrip,mixed,encode,serve",
+                    help="extra legs after the headline decode: roundtrip (1080p HQ encode + decode), mixed "
+                         "(96 mixed frames, batched vs plain decode), encode (32-frame batched encode), serve "
+                         "(the serving loop), comma-separated, or none")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=300.0, help="offered requests per second")
+    ap.add_argument("--serve-peak-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-peak-rate", type=float, default=1000.0,
+                    help="a second serving run at this offered load (0: none)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="after the timed steps, one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, for scaling profiler byte counters)")
@@ -485,6 +633,10 @@ def main():
     batched_s = (time.perf_counter() - t0) / args.steps
     stage /= args.steps
     log(f"bench: batched {batched_s * 1e3:.2f} ms/step")
+    # Every frame of the last timed batch, before the plain loop reuses the
+    # output buffers.
+    batched_out = [(outs[i][0].download(samples * 2, dtype=np.uint16),
+                    outs[i][1].download(w * h * 2, dtype=np.uint16)) for i in range(n)]
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plain()
@@ -573,31 +725,60 @@ def main():
     result["host_batch_fps"] = round(n / host_batch_s, 2)
     result["host_batch_gbs"] = round((colour_bytes + alpha_bytes + packet_bytes) / host_batch_s / 1e9, 2)
 
+    result["hashes"] = {"packets_fnv1a": fnv_packets(P, packets), "pictures_fnv1a": fnv_pictures(P, batched_out)}
+
     legs = [x for x in args.legs.split(",") if x and x != "none"]
+    check = not args.no_cpu
+    c1 = None
+    c4_frames = []
+    if "roundtrip" in legs:
+        log("bench: 1080p round trip leg")
+        result["roundtrip_1080p"], c1 = leg_roundtrip_1080p(P, synth, args.steps, args.warmup, check)
     if "mixed" in legs:
         log("bench: mixed decode leg")
-        result["mixed_decode"] = leg_mixed_decode(P, synth, args.steps, args.warmup, not args.no_cpu)
+        result["mixed_decode"] = leg_mixed_decode(P, synth, args.steps, args.warmup, check)
     if "encode" in legs:
         log("bench: batch encode leg")
-        result["batch_encode"] = leg_batch_encode(P, synth, max(1, args.steps // 2), 1, not args.no_cpu)
-
+        result["batch_encode"] = leg_batch_encode(P, synth, max(1, args.steps // 2), 1, check, c4_frames)
     if "serve" in legs:
-        log("bench: serving leg")
-        result["serving"] = leg_serving(P, synth, args.serve_seconds, args.serve_rate, not args.no_cpu)
+        log(f"bench: serving leg, {args.serve_seconds} s at {args.serve_rate}/s")
+        result["serving"] = leg_serving(P, synth, args.serve_seconds, args.serve_rate, check)
+        if args.serve_peak_rate > 0:
+            log(f"bench: serving leg, {args.serve_peak_seconds} s at {args.serve_peak_rate}/s")
+            result["serving_peak"] = leg_serving(P, synth, args.serve_peak_seconds, args.serve_peak_rate, check)
 
-    if not args.no_cpu:
+    if check:
         import oracle_lib as O
-        # The checker: one frame's GPU packet against the oracle's bytes, and
-        # the GPU's decoded planes against the oracle's decode.
+        # The checker: every headline frame's GPU packet against the
+        # oracle's encode, and every frame of the timed batch against the
+        # oracle's decode of its packet.
+        log("bench: checking the headline frames against the oracle")
         oc = O.config(5)
-        of = O.Frame(w, h, 3, 12, frames[0].data, frames[0].alpha)
-        result["bit_exact_encode"] = O.encode(of, oc) == packets[0]
-        od = O.decode(packets[0])
-        gd = outs[0][0].download(samples * 2, dtype=np.uint16)
-        ga = outs[0][1].download(w * h * 2, dtype=np.uint16)
-        result["bit_exact_decode"] = bool(np.array_equal(od.data, gd) and np.array_equal(od.alpha, ga))
+        want_packets, want_pictures, same_enc, same_dec = [], [], [], []
+        for i in range(n):
+            of = O.Frame(w, h, 3, 12, frames[i].data, frames[i].alpha)
+            wp = O.encode(of, oc)
+            want_packets.append(wp)
+            same_enc.append(wp == packets[i])
+            od = O.decode(packets[i])
+            want_pictures.append((od.data, od.alpha))
+            same_dec.append(bool(np.array_equal(od.data, batched_out[i][0]) and
+                                 np.array_equal(od.alpha, batched_out[i][1])))
+        result["bit_exact_encode"] = all(same_enc)
+        result["bit_exact_decode"] = all(same_dec)
+        result["bit_exact_frames"] = {"checked": n, "packets_differing": [i for i, x in enumerate(same_enc) if not x],
+                                      "pictures_differing": [i for i, x in enumerate(same_dec) if not x]}
+        result["hashes"]["oracle_packets_fnv1a"] = fnv_packets(P, want_packets)
+        result["hashes"]["oracle_pictures_fnv1a"] = fnv_pictures(P, want_pictures)
+        del want_pictures
         log("bench: cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(O, packets, args.cpu_seconds)
+        if not c4_frames:
+            f4 = synth.frame(3840, 2160, P.ChromaFormat.Yuv422, 10, seed=9000, rectangles=True)
+            c4_frames = [(O.Frame(3840, 2160, 2, 10, f4.data, None, 0), None, None)]
+        if c1 is None:
+            f1 = synth.frame(1920, 1080, P.ChromaFormat.Yuv422, 10, seed=1)
+            c1 = (O.Frame(1920, 1080, 2, 10, f1.data, None, 0), None)
+        result["cpu_baseline"] = cpu_baseline(O, packets, c4_frames, c1, args.cpu_seconds)
     print(json.dumps(result), flush=True)
 
 

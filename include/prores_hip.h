@@ -269,6 +269,14 @@ void prores_last_batch_timing(double *us5);
  * (host sync for the output sizes included). */
 void prores_last_encode_timing(double *us4);
 
+/* Microseconds of the last encode batch on the per-call lane, kernel by
+ * kernel from HIP events on its stream: [0] pe_fdct, [1] pe_alpha, [2] the
+ * nine probe rounds (pe_size + pe_search), [3] pe_window, [4] pe_shareout,
+ * [5] pe_offsets, [6] pe_pack, [7] the whole batch (the host's wait for the
+ * packet sizes included). The reference's bench_api times the same steps
+ * per block on the CPU (src/bench_api.rs:95-178). */
+void prores_last_encode_stages(double *us8);
+
 /* Counters of the last encode batch on the per-call lane: [0] sizes the
  * share-out computed past its precomputed window, [1] the most for one
  * picture, [2] pictures, [3] mean uniform quantiser index. No counterpart in

@@ -278,16 +278,18 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   void *alen = ws.alpha_len.p, *stab = ws.size_tab.p, *used = ws.used.p, *qidx = ws.qidx.p;
   void *sout = ws.slice_out.p, *fsize = ws.frame_size.p, *tables = d.tables, *deep = ws.deep.p;
   uint32_t g32 = (uint32_t)G, np32 = NP, window = WINDOW;
-  (void)hipEventRecord(L.ev[0], L.stream);
+  hipEvent_t *ev = L.eev;
+  (void)hipEventRecord(ev[0], L.stream);
   {
     void *args[] = {&tables, &frames, &pics, &slices, &coefs};
     if (!(e = launch(L, d.k_fdct, g32, 1, 256, args)).ok()) return e;
   }
+  (void)hipEventRecord(ev[1], L.stream);
   {
     void *args[] = {&frames, &pics, &slices, &g32, &ascr, &alen, &states};
     if (!(e = launch(L, d.k_alpha_enc, g32, 1, 64, args)).ok()) return e;
   }
-  (void)hipEventRecord(L.ev[1], L.stream);
+  (void)hipEventRecord(ev[2], L.stream);
   // qi = 224, then at most 8 bisection steps (224 < 2^8).
   for (int probe = 0; probe < 9; probe++) {
     void *a1[] = {&frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states};
@@ -295,20 +297,23 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
     void *a2[] = {&pics, &states, &np32, &window};
     if (!(e = launch(L, d.k_search, (np32 + 63) / 64, 1, 64, a2)).ok()) return e;
   }
+  (void)hipEventRecord(ev[3], L.stream);
   {
     void *args[] = {&frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &stab};
     if (!(e = launch(L, d.k_window, (g32 + 3) / 4, 1, 256, args)).ok()) return e;
   }
+  (void)hipEventRecord(ev[4], L.stream);
   if (G && !(e = check(hipMemsetAsync(ws.deep.p, 0, G * ENC_DEEP * 4, L.stream), "memset")).ok()) return e;
   {
     void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &states, &stab, &deep, &used, &qidx};
     if (!(e = launch(L, d.k_shareout, np32, 1, 64 * SHARE_WAVES, args)).ok()) return e;
   }
+  (void)hipEventRecord(ev[5], L.stream);
   {
     void *args[] = {&frames, &pics, &used, &sout, &fsize};
     if (!(e = launch(L, d.k_offsets, NF, 1, 256, args)).ok()) return e;
   }
-  (void)hipEventRecord(L.ev[2], L.stream);
+  (void)hipEventRecord(ev[6], L.stream);
   sizes.assign(NF, 0);
   if (NF && !(e = check(hipMemcpyAsync(sizes.data(), ws.frame_size.p, NF * 8, hipMemcpyDeviceToHost, L.stream),
                         "frame sizes")).ok())
@@ -324,13 +329,12 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   if (!(e = ws.out.reserve(total + 256)).ok()) return e;
   for (uint32_t f = 0; f < NF; f++) b.frames[f].out = (uint64_t)ws.out.p + offsets[f];
   if (!(e = upload(L, ws.frames, b.frames)).ok()) return e;
-  (void)hipEventRecord(L.ev[3], L.stream);
+  (void)hipEventRecord(ev[7], L.stream);
   {
     void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &ascr, &used, &qidx, &sout, &fsize, &hdrs};
     if (!(e = launch(L, d.k_pack, g32, 1, 64, args)).ok()) return e;
   }
-  hipEvent_t done = L.ev[4];
-  (void)hipEventRecord(done, L.stream);
+  (void)hipEventRecord(ev[8], L.stream);
   if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
   {
     // Diagnostics: sizes the share-out computed past the window.
@@ -348,15 +352,19 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
       L.enc_counters[3] = NP ? uni / NP : 0;
     }
   }
-  float a = 0, r = 0, p = 0, all = 0;
-  (void)hipEventElapsedTime(&a, L.ev[0], L.ev[1]);
-  (void)hipEventElapsedTime(&r, L.ev[1], L.ev[2]);
-  (void)hipEventElapsedTime(&p, L.ev[3], done);
-  (void)hipEventElapsedTime(&all, L.ev[0], done);
-  L.enc_us[0] = a * 1000.0;
-  L.enc_us[1] = r * 1000.0;
-  L.enc_us[2] = p * 1000.0;
-  L.enc_us[3] = all * 1000.0;
+  // Stages: transform, alpha, probes, window, share-out, offsets, pack, and
+  // the whole batch (the host's wait for the packet sizes included).
+  const int from[8] = {0, 1, 2, 3, 4, 5, 7, 0}, to[8] = {1, 2, 3, 4, 5, 6, 8, 8};
+  for (int i = 0; i < 8; i++) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[from[i]], ev[to[i]]);
+    L.enc_stage_us[i] = ms * 1000.0;
+  }
+  const double *st = L.enc_stage_us;
+  L.enc_us[0] = st[0] + st[1];
+  L.enc_us[1] = st[2] + st[3] + st[4] + st[5];
+  L.enc_us[2] = st[6];
+  L.enc_us[3] = st[7];
   return Err{};
 }
 
@@ -500,6 +508,15 @@ void prores_last_encode_counters(uint64_t *out4) {
   if (!device(dev).ok() || !lane(*dev, 0, lane0).ok()) return;
   std::lock_guard<std::mutex> g(lane0->mu);
   for (int i = 0; i < 4; i++) out4[i] = lane0->enc_counters[i];
+}
+
+void prores_last_encode_stages(double *us8) {
+  Device *dev = nullptr;
+  Lane *lane0 = nullptr;
+  for (int i = 0; i < 8; i++) us8[i] = 0;
+  if (!device(dev).ok() || !lane(*dev, 0, lane0).ok()) return;
+  std::lock_guard<std::mutex> g(lane0->mu);
+  for (int i = 0; i < 8; i++) us8[i] = lane0->enc_stage_us[i];
 }
 
 void prores_last_encode_timing(double *us4) {

@@ -353,10 +353,14 @@ Err lane(Device &d, int index, Lane *&out) {
     if (e.ok()) e = check(hipStreamCreateWithFlags(&L->aux, hipStreamNonBlocking), "hipStreamCreate");
     for (auto &ev : L->ev)
       if (e.ok()) e = check(hipEventCreate(&ev), "hipEventCreate");
+    for (auto &ev : L->eev)
+      if (e.ok()) e = check(hipEventCreate(&ev), "hipEventCreate");
     if (e.ok()) e = check(hipEventCreateWithFlags(&L->fork, hipEventDisableTiming), "hipEventCreate");
     if (e.ok()) e = check(hipEventCreateWithFlags(&L->join, hipEventDisableTiming), "hipEventCreate");
     if (!e.ok()) {
       for (auto &ev : L->ev)
+        if (ev) (void)hipEventDestroy(ev);
+      for (auto &ev : L->eev)
         if (ev) (void)hipEventDestroy(ev);
       if (L->fork) (void)hipEventDestroy(L->fork);
       if (L->join) (void)hipEventDestroy(L->join);

@@ -79,11 +79,17 @@ struct Lane {
   hipStream_t aux = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // The encoder's stage markers: before the transform, after the transform,
+  // the alpha coding, the probes, the window, the share-out, the offsets;
+  // before and after the pack.
+  static constexpr int ENC_EVENTS = 9;
+  hipEvent_t eev[ENC_EVENTS] = {};
   std::mutex mu;
   void *dec = nullptr;  // decode workspace (decode_host.cpp)
   void *enc = nullptr;  // encode workspace (encode_host.cpp)
   double dec_us[5] = {0, 0, 0, 0, 0};  // last decode batch by stage
   double enc_us[4] = {0, 0, 0, 0};     // last encode batch by stage
+  double enc_stage_us[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // see prores_last_encode_stages
   uint64_t enc_counters[4] = {0, 0, 0, 0};  // last encode batch: see prores_last_encode_counters
 };
 constexpr int MAX_LANES = 8;

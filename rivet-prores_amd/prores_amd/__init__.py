@@ -153,7 +153,7 @@ EXPORTS = [
     "prores_server_submit_decode", "prores_server_submit_encode", "prores_server_poll",
     "prores_server_release", "prores_server_get_stats", "prores_last_encode_counters",
     "prores_entropy_tables", "prores_memcpy_error", "prores_decode_host_batch", "prores_find_frames",
-    "prores_fnv1a", "prores_graph_counters",
+    "prores_fnv1a", "prores_graph_counters", "prores_last_encode_stages",
 ]
 
 _lib = None
@@ -209,6 +209,7 @@ def lib():
         L.prores_last_encode_timing.argtypes = [P(C.c_double)]
         L.prores_last_encode_counters.argtypes = [P(C.c_uint64)]
         L.prores_graph_counters.argtypes = [P(C.c_uint64)]
+        L.prores_last_encode_stages.argtypes = [P(C.c_double)]
         L.prores_version.restype = C.c_char_p
         L.prores_find_frames.argtypes = [C.c_char_p, C.c_size_t, P(C.c_uint64), P(C.c_uint64), C.c_size_t]
         L.prores_find_frames.restype = C.c_size_t
@@ -589,6 +590,16 @@ def graph_counters():
     out = (C.c_uint64 * 3)()
     lib().prores_graph_counters(out)
     return list(out)
+
+
+ENCODE_STAGES = ["fdct", "alpha", "probes", "window", "shareout", "offsets", "pack", "all"]
+
+
+def last_encode_stages():
+    """Microseconds of the last encode batch by kernel (ENCODE_STAGES)."""
+    us = (C.c_double * 8)()
+    lib().prores_last_encode_stages(us)
+    return dict(zip(ENCODE_STAGES, us))
 
 
 def last_encode_timing():
