@@ -208,6 +208,18 @@ def fnv_pictures(P, pictures):
     return f"{h:016x}"
 
 
+def calibration_copy(P, mb):
+    """One device-to-device copy of `mb` MiB: a dispatch of known traffic,
+    for scaling the profiler's byte counters."""
+    if mb:
+        nbytes = mb << 20
+        ca, cb = P.DeviceBuffer(nbytes), P.DeviceBuffer(nbytes)
+        P.copy_device(cb.ptr, ca.ptr, nbytes)
+        log(f"bench: calibration copy of {nbytes} bytes")
+        ca.free()
+        cb.free()
+
+
 def timed(fn, steps):
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -590,6 +602,8 @@ def main():
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="after the timed steps, one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, for scaling profiler byte counters)")
+    ap.add_argument("--legs-only", action="store_true",
+                    help="run only the --legs (no headline decode, no CPU baseline): for profiling one leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -598,6 +612,22 @@ def main():
 
     import prores_amd as P
     from prores_amd import synth
+
+    if args.legs_only:
+        legs = [x for x in args.legs.split(",") if x and x != "none"]
+        check = not args.no_cpu
+        out = {}
+        if "roundtrip" in legs:
+            out["roundtrip_1080p"] = leg_roundtrip_1080p(P, synth, args.steps, args.warmup, check)[0]
+        if "mixed" in legs:
+            out["mixed_decode"] = leg_mixed_decode(P, synth, args.steps, args.warmup, check)
+        if "encode" in legs:
+            out["batch_encode"] = leg_batch_encode(P, synth, args.steps, args.warmup, check, [])
+        if "serve" in legs:
+            out["serving"] = leg_serving(P, synth, args.serve_seconds, args.serve_rate, check)
+        calibration_copy(P, args.calibration_copy_mb)
+        print(json.dumps(out), flush=True)
+        return
 
     log(f"bench: generating and GPU-encoding {args.frames} frames of {args.width}x{args.height} 4444 XQ")
     frames, cfg, in_bufs, packets, dev_packets, enc_s = make_inputs(P, synth, args.frames, args.width, args.height)
@@ -643,13 +673,7 @@ def main():
     plain_s = (time.perf_counter() - t0) / args.steps
     log(f"bench: plain {plain_s * 1e3:.2f} ms/step")
 
-    if args.calibration_copy_mb:
-        nbytes = args.calibration_copy_mb << 20
-        ca, cb = P.DeviceBuffer(nbytes), P.DeviceBuffer(nbytes)
-        P.copy_device(cb.ptr, ca.ptr, nbytes)
-        log(f"bench: calibration copy of {nbytes} bytes")
-        ca.free()
-        cb.free()
+    calibration_copy(P, args.calibration_copy_mb)
 
     # Algorithmic bytes (the packet read, the planes written).
     packet_bytes = sum(len(p) for p in packets)

@@ -5,6 +5,7 @@
 namespace prores_hip {
 
 constexpr uint32_t ENC_MAXK = 8;         // window columns of the size table
+constexpr uint32_t ENC_PROBES = 9;       // qi = 224, then at most 8 bisection steps (224 < 2^8)
 constexpr uint32_t ENC_DEEP = 224;       // size cache columns per slice (column t: qi = uniform - t)
 constexpr uint32_t SHARE_WAVES = 8;      // waves of a pe_shareout workgroup
 constexpr uint32_t DEEP_STEP = 4;        // quantisers coded per cache fill
@@ -38,7 +39,8 @@ struct EncSlice {
   uint64_t coef;          // element offset of the slice's f32 coefficients
   uint32_t pic;
   uint16_t mb_x, mb_y;
-  uint32_t mbs, pad;
+  uint32_t mbs;
+  uint32_t mbs_from;      // macroblocks from this slice to the end of its picture
 };
 
 // The picture-level rate-control state (src/encode.rs:685-794).
@@ -51,8 +53,11 @@ struct EncState {
   uint32_t probe;          // qi of the pending probe, 0 when none
   uint32_t lo, hi;
   uint32_t fits_at_all, uniform;
-  uint32_t window_k;       // columns computed: uniform, uniform-1, ...
+  uint32_t window_k;       // columns of the size table: uniform, uniform-1, ...
   uint32_t fallback;       // sizes the share-out computed itself (diagnostic)
+  uint32_t round;          // probes made
+  uint32_t probe_qi[ENC_PROBES];          // the qi of each probe
+  unsigned long long probe_sum[ENC_PROBES];  // and the picture's total bytes at it
 };
 
 }  // namespace prores_hip

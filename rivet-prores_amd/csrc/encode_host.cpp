@@ -224,6 +224,11 @@ Err plan_encode(const prores_config &cfg, const prores_frame &f, uint64_t dev_da
       }
     }
     P.n_slices = (uint32_t)(b.slices.size() - P.slice_base);
+    uint32_t from = 0;  // macroblocks from each slice to the end of the picture
+    for (size_t k = b.slices.size(); k-- > P.slice_base;) {
+      from += b.slices[k].mbs;
+      b.slices[k].mbs_from = from;
+    }
     b.pics.push_back(P);
     EncState st;
     memset(&st, 0, sizeof st);
@@ -236,7 +241,7 @@ Err plan_encode(const prores_config &cfg, const prores_frame &f, uint64_t dev_da
 
 struct EncWorkspace {
   DevBuf frames, pics, slices, states, qtabs, hdrs, coefs, alpha_scratch, alpha_len, size_tab, deep, used, qidx,
-      slice_out, frame_size, out, planes, alpha_in;
+      slice_out, frame_size, out, planes, alpha_in, probe_sizes;
 };
 EncWorkspace &enc_workspace(Lane &L) {
   if (!L.enc) L.enc = new EncWorkspace();
@@ -264,10 +269,13 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   if (!(e = upload(L, ws.states, b.states)).ok()) return e;
   if (!(e = upload(L, ws.qtabs, b.qtabs)).ok()) return e;
   if (!(e = upload(L, ws.hdrs, b.hdrs)).ok()) return e;
-  if (!(e = ws.coefs.reserve(b.n_coefs * 4 + 64)).ok()) return e;
+  // A chunk of 64 coefficients of padding: the size count loads the next
+  // chunk of a slice while it counts the current one.
+  if (!(e = ws.coefs.reserve(b.n_coefs * 4 + 512)).ok()) return e;
   if (!(e = ws.alpha_scratch.reserve(G * ALPHA_SLICE_CAP + 64)).ok()) return e;
   if (!(e = ws.alpha_len.reserve(G * 4 + 64)).ok()) return e;
   if (!(e = ws.size_tab.reserve(G * ENC_MAXK * 4 + 64)).ok()) return e;
+  if (!(e = ws.probe_sizes.reserve(G * ENC_PROBES * 4 + 64)).ok()) return e;
   if (!(e = ws.deep.reserve(G * ENC_DEEP * 4 + 64)).ok()) return e;
   if (!(e = ws.used.reserve(G * 4 + 64)).ok()) return e;
   if (!(e = ws.qidx.reserve(G * 4 + 64)).ok()) return e;
@@ -277,6 +285,7 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   void *qtabs = ws.qtabs.p, *hdrs = ws.hdrs.p, *coefs = ws.coefs.p, *ascr = ws.alpha_scratch.p;
   void *alen = ws.alpha_len.p, *stab = ws.size_tab.p, *used = ws.used.p, *qidx = ws.qidx.p;
   void *sout = ws.slice_out.p, *fsize = ws.frame_size.p, *tables = d.tables, *deep = ws.deep.p;
+  void *psz = ws.probe_sizes.p;
   uint32_t g32 = (uint32_t)G, np32 = NP, window = WINDOW;
   hipEvent_t *ev = L.eev;
   (void)hipEventRecord(ev[0], L.stream);
@@ -291,21 +300,21 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   }
   (void)hipEventRecord(ev[2], L.stream);
   // qi = 224, then at most 8 bisection steps (224 < 2^8).
-  for (int probe = 0; probe < 9; probe++) {
-    void *a1[] = {&frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states};
+  for (uint32_t probe = 0; probe < ENC_PROBES; probe++) {
+    void *a1[] = {&tables, &frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &psz};
     if (!(e = launch(L, d.k_size, (g32 + 3) / 4, 1, 256, a1)).ok()) return e;
     void *a2[] = {&pics, &states, &np32, &window};
     if (!(e = launch(L, d.k_search, (np32 + 63) / 64, 1, 64, a2)).ok()) return e;
   }
   (void)hipEventRecord(ev[3], L.stream);
   {
-    void *args[] = {&frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &stab};
+    void *args[] = {&tables, &frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &psz, &stab};
     if (!(e = launch(L, d.k_window, (g32 + 3) / 4, 1, 256, args)).ok()) return e;
   }
   (void)hipEventRecord(ev[4], L.stream);
   if (G && !(e = check(hipMemsetAsync(ws.deep.p, 0, G * ENC_DEEP * 4, L.stream), "memset")).ok()) return e;
   {
-    void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &states, &stab, &deep, &used, &qidx};
+    void *args[] = {&tables, &frames, &pics, &slices, &coefs, &qtabs, &alen, &states, &stab, &deep, &used, &qidx};
     if (!(e = launch(L, d.k_shareout, np32, 1, 64 * SHARE_WAVES, args)).ok()) return e;
   }
   (void)hipEventRecord(ev[5], L.stream);

@@ -8,11 +8,12 @@
 //                coded in parallel (src/encode.rs:838-868, 457-470).
 //   pe_size      one wavefront per slice: bytes at the picture's pending probe
 //                qi (quantise + exact bit count, src/encode.rs:671-681,
-//                398-409), summed per picture.
+//                398-409), kept per slice and summed per picture.
 //   pe_search    one lane per picture: one step of the binary search over qi
 //                (src/encode.rs:685-713).
-//   pe_window    one wavefront per slice: sizes at uniform, uniform-1, ... in
-//                one pass over the coefficients.
+//   pe_window    one wavefront per slice: the share-out's size table, from
+//                the probes at uniform and uniform-1 and, for the slices that
+//                may step further, the next finer quantisers.
 //   pe_shareout  one workgroup per picture: the sequential slice-by-slice
 //                share-out (src/encode.rs:769-793) solved 64 slices at a time
 //                by fixed-point iteration of its triangular recurrence (one
@@ -135,107 +136,123 @@ __device__ __forceinline__ void component(uint32_t c, uint32_t mbs, uint32_t cpm
   off = c == 0 ? 0u : (c == 1 ? 256u * mbs : (256u + 64u * cpm) * mbs);
 }
 
-// The bits one lane's coefficient contributes in a 64-coefficient chunk of a
-// component's scanned array, and the chunk-to-chunk state of the run/level
-// contexts (src/encode.rs:366-409).
+// The run/level context state pe_pack carries from one 64-coefficient chunk
+// of a component's scanned array to the next (src/encode.rs:413-437).
 struct Ctx {
   uint32_t idx, run, lvl;  // last nonzero AC index, its run and level symbol
 };
 
-// Length of lane `L`'s codewords in the chunk starting at `base`.
-__device__ __forceinline__ uint32_t chunk_len(int32_t q, uint32_t i, uint32_t base, uint32_t L,
-                                              uint32_t nb, Ctx &cx) {
-  uint32_t len = 0;
-  if (base == 0) {  // the DC coefficients (src/encode.rs:383-395)
-    int32_t d1 = __shfl(q, L ? L - 1 : 0, 64);
-    int32_t d2 = __shfl(q, L > 1 ? L - 2 : 0, 64);
-    if (L < nb) {
-      if (L == 0) {
-        len = book_len(FIRST_DC, signed_to_symbol(q));
-      } else {
-        int32_t diff = (int32_t)((uint32_t)q - (uint32_t)d1);
-        int32_t pd = L == 1 ? 3 : (int32_t)((uint32_t)d1 - (uint32_t)d2);
-        int32_t n = pd < 0 ? (int32_t)(0u - (uint32_t)diff) : diff;
-        uint32_t a = pd < 0 ? 0u - (uint32_t)pd : (uint32_t)pd;
-        len = book_len(dc_book(a < 3 ? a : 3), signed_to_symbol(n));
-      }
-    }
-  }
-  const bool nz = i >= nb && q != 0;
-  const uint64_t mask = __ballot(nz);
-  if (mask) {
-    const uint64_t lower = mask & ((1ull << L) - 1);
-    const int P = lower ? 63 - (int)__builtin_clzll(lower) : -1;
-    const uint32_t prev_idx = P >= 0 ? base + (uint32_t)P : cx.idx;
-    const uint32_t run = i - prev_idx - 1;
-    const uint32_t lvl = (q < 0 ? 0u - (uint32_t)q : (uint32_t)q) - 1;
-    uint32_t prun = __shfl(run, P >= 0 ? P : (int)L, 64);
-    uint32_t plvl = __shfl(lvl, P >= 0 ? P : (int)L, 64);
-    if (P < 0) {
-      prun = cx.run;
-      plvl = cx.lvl;
-    }
-    if (nz)
-      len += book_len(run_book(prun < 15 ? prun : 15), run) + book_len(lvl_book(plvl < 8 ? plvl : 8), lvl) + 1;
-    const int last = 63 - (int)__builtin_clzll(mask);
-    cx.idx = base + (uint32_t)last;
-    cx.run = __shfl(run, last, 64);
-    cx.lvl = __shfl(lvl, last, 64);
-  }
-  return len;
+// Length of the combination code (lq, kr, ke) of n (Codebook::len,
+// src/vlc.rs:155-172) without branches: the Rice length below the threshold,
+// else the exp-Golomb length, lq + 1 + 2 bitlen(n - th + 2^ke) - ke - 1
+// (n <= 2^30, so the sum fits 32 bits; the unused side may wrap).
+__device__ __forceinline__ uint32_t code_len(uint32_t lq, uint32_t kr, uint32_t ke, uint32_t n) {
+  const uint32_t th = (lq + 1) << kr;
+  const uint32_t rice = (n >> kr) + 1 + kr;
+  const uint32_t eg = lq + 2 * (32 - (uint32_t)__clz((int)(n - th + (1u << ke)))) - ke;
+  return n < th ? rice : eg;
+}
+__device__ __forceinline__ uint32_t run_code_len(uint32_t ctx, uint32_t n) {
+  const uint32_t nib = (uint32_t)(RUN_P >> (4 * ctx)) & 15, kr = nib >> 2;
+  return code_len(nib & 3, kr, kr + 1, n);
+}
+__device__ __forceinline__ uint32_t level_code_len(uint32_t ctx, uint32_t n) {
+  const uint32_t nib = (uint32_t)(LVL_P >> (4 * ctx)) & 15, kr = nib >> 2;
+  return code_len(nib & 3, kr, kr + 1 + (ctx == 0 ? 1u : 0u), n);
 }
 
-// Sizes of one slice at K quantisers (K <= MAXK), one wavefront: total bytes
-// and whether every size field can hold them (src/encode.rs:671-681).
-template <int MAXK>
-__device__ void slice_sizes(const float *__restrict__ coef, uint32_t mbs, uint32_t cpm,
-                            const float *__restrict__ qt, const uint32_t *qis, uint32_t K,
-                            uint32_t hb, uint32_t alen, uint32_t *size, bool *fits) {
+// The encoder's code-length tables (DevTables::enc_run_len, enc_level_len)
+// into LDS, by the whole workgroup.
+__device__ __forceinline__ void load_len_tables(const DevTables *__restrict__ T, uint32_t *lens) {
+  const uint32_t *src = (const uint32_t *)&T->enc_run_len[0][0];
+  for (uint32_t i = threadIdx.x; i < ENC_LEN_BYTES / 4; i += blockDim.x) lens[i] = src[i];
+  __syncthreads();
+}
+
+// Bytes of one slice at quantiser qi, one wavefront, and whether every size
+// field can hold them (src/encode.rs:671-681): per component, the exact bit
+// count of coefficient_bits (src/encode.rs:366-409) over the quantised
+// coefficients (src/dsp/scalar.rs:19-39), a 64-coefficient chunk of the
+// scanned array a step, one coefficient a lane.
+// - The inverse quantiser of a lane's scan position comes from a register
+//   (lane k holds the component's entry k) by a lane permute, not a load.
+// - The next chunk's coefficients are loaded while this one is counted (the
+//   arena has a chunk of padding after its last slice).
+// - Run and level contexts travel as one packed word (min(run, 15) |
+//   min(level, 8) << 4): one permute from the previous nonzero lane, and the
+//   chunk's last nonzero passes its word to the next chunk by a readlane.
+// - Code lengths come from the LDS tables `lens` (runs below 64, levels
+//   below 256); a chunk with a larger symbol also takes the closed form.
+// No conversion here saturates: a coefficient of fdct_load's output is at
+// most 64 * 256 * (1/2)^2 = 4096 in magnitude and an inverse quantiser at
+// most 8 / 2, so |x| + r < 2^15.
+__device__ void slice_bytes(const float *__restrict__ coef, uint32_t mbs, uint32_t cpm,
+                            const float *__restrict__ qt, uint32_t qi, uint32_t hb, uint32_t alen,
+                            const uint8_t *lens, uint32_t &size, bool &fits) {
   const uint32_t L = threadIdx.x & 63;
-#pragma unroll
-  for (int k = 0; k < MAXK; k++) {
-    size[k] = hb + alen;
-    fits[k] = true;
-  }
+  const uint64_t below = (1ull << L) - 1;
+  const uint8_t *run_lens = lens, *level_lens = lens + 16 * 64;
+  size = hb + alen;
+  fits = true;
   for (uint32_t c = 0; c < 3; c++) {
     uint32_t nb, off;
     component(c, mbs, cpm, nb, off);
-    const uint32_t lg = ilog2(nb);
-    const float *cf = coef + off;
-    uint32_t lanebits[MAXK];
-    Ctx cx[MAXK];
-    const float *inv[MAXK];
-#pragma unroll
-    for (int k = 0; k < MAXK; k++) {
-      lanebits[k] = 0;
-      cx[k] = Ctx{nb - 1, 4, 1};
-      inv[k] = qt + ((k < (int)K ? qis[k] : qis[0]) * 2 + (c > 0 ? 1 : 0)) * 64;
-    }
-    for (uint32_t base = 0; base < nb * 64; base += 64) {
-      const uint32_t i = base + L;
-      const float f = cf[i];
-      const uint32_t s = i >> lg;
-      const float r = s == 0 ? 0.5f : 0.42f;
-#pragma unroll
-      for (int k = 0; k < MAXK; k++) {
-        if (k < (int)K) {
-          const int32_t q = quant(f, inv[k][s], r);
-          lanebits[k] += chunk_len(q, i, base, L, nb, cx[k]);
+    const uint32_t lg = ilog2(nb), n = nb * 64;
+    const float *cf = coef + off + L;
+    const float invL = qt[(qi * 2 + (c > 0 ? 1u : 0u)) * 64 + L];
+    uint32_t bits = 0;
+    uint32_t last_idx = nb - 1, ctx = 4u | 1u << 4;  // no nonzero yet: run context 4, level context 1
+    float f = cf[0];
+    for (uint32_t base = 0; base < n; base += 64) {
+      const float fn = cf[base + 64];
+      const uint32_t i = base + L, s = i >> lg;
+      const float inv = __shfl(invL, (int)s, 64);
+      const float x = f * inv;
+      const uint32_t m = (uint32_t)(int32_t)floorf(fabsf(x) + (s == 0 ? 0.5f : 0.42f));
+      bool nz = m != 0;
+      if (base == 0) {  // the DC coefficients (src/encode.rs:383-395), nb <= 32 of the first chunk
+        const int32_t q = x < 0.0f ? -(int32_t)m : (int32_t)m;
+        const int32_t d1 = __shfl(q, L ? (int)L - 1 : 0, 64);
+        const int32_t d2 = __shfl(q, L > 1 ? (int)L - 2 : 0, 64);
+        if (L < nb) {
+          if (L == 0) {
+            bits += book_len(FIRST_DC, signed_to_symbol(q));
+          } else {
+            const int32_t diff = (int32_t)((uint32_t)q - (uint32_t)d1);
+            const int32_t pd = L == 1 ? 3 : (int32_t)((uint32_t)d1 - (uint32_t)d2);
+            const int32_t nn = pd < 0 ? (int32_t)(0u - (uint32_t)diff) : diff;
+            const uint32_t a = pd < 0 ? 0u - (uint32_t)pd : (uint32_t)pd;
+            bits += book_len(dc_book(a < 3 ? a : 3), signed_to_symbol(nn));
+          }
         }
+        nz = nz && L >= nb;
       }
-    }
-#pragma unroll
-    for (int k = 0; k < MAXK; k++) {
-      if (k < (int)K) {
-        const uint32_t bytes = (wave_sum(lanebits[k]) + 7) / 8;
-        size[k] += bytes;
-        if (bytes > 65535) fits[k] = false;
+      const uint64_t mask = __ballot(nz);
+      if (mask) {
+        const uint64_t lower = mask & below;
+        const int P = lower ? 63 - (int)__builtin_clzll(lower) : -1;
+        const uint32_t prev_idx = P >= 0 ? base + (uint32_t)P : last_idx;
+        const uint32_t run = i - prev_idx - 1, lvl = m - 1;
+        const uint32_t mine = (run < 15 ? run : 15u) | (lvl < 8 ? lvl : 8u) << 4;
+        uint32_t pc = __shfl(mine, P >= 0 ? P : (int)L, 64);
+        if (P < 0) pc = ctx;
+        uint32_t pair = run_lens[(pc & 15) << 6 | (run & 63)] + level_lens[(pc >> 4) << 8 | (lvl & 255)] + 1;
+        const bool big = nz && (run >= 64 || lvl >= 256);
+        if (__ballot(big)) {
+          if (big) pair = run_code_len(pc & 15, run) + level_code_len(pc >> 4, lvl) + 1;
+        }
+        if (nz) bits += pair;
+        const uint32_t last = 63 - (uint32_t)__builtin_clzll(mask);
+        last_idx = base + last;
+        ctx = __builtin_amdgcn_readlane(mine, last);
       }
+      f = fn;
     }
+    const uint32_t bytes = (wave_sum(bits) + 7) / 8;
+    size += bytes;
+    if (bytes > 65535) fits = false;
   }
-#pragma unroll
-  for (int k = 0; k < MAXK; k++)
-    if (size[k] > 65535) fits[k] = false;
+  if (size > 65535) fits = false;
 }
 
 __device__ __forceinline__ uint16_t rescale(uint32_t a, uint32_t identity, uint64_t m) {
@@ -432,22 +449,28 @@ extern "C" __global__ void __launch_bounds__(64)
 // ---------------------------------------------------------------------------
 // One wavefront per slice, four per workgroup.
 extern "C" __global__ void __launch_bounds__(256)
-    pe_size(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
+    pe_size(const DevTables *__restrict__ T, const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
             const EncSlice *__restrict__ slices, uint32_t n_slices, const float *__restrict__ coefs,
             const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
-            EncState *__restrict__ state) {
+            EncState *__restrict__ state, uint32_t *__restrict__ probe_sizes) {
+  __shared__ uint32_t lens[ENC_LEN_BYTES / 4];
+  load_len_tables(T, lens);
   const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (g >= n_slices) return;
   const EncSlice S = slices[g];
   const uint32_t qi = state[S.pic].probe;
   if (qi == 0) return;
+  const uint32_t round = state[S.pic].round;
   const EncPic &P = pics[S.pic];
   const EncFrame &F = frames[P.frame];
   uint32_t size[1];
   bool fits[1];
-  slice_sizes<1>(coefs + S.coef, S.mbs, F.chroma == 2 ? 2u : 4u, qtabs + (size_t)P.qtab * 225 * 128,
-                 &qi, 1, P.header_bytes, alpha_len[g], size, fits);
+  slice_bytes(coefs + S.coef, S.mbs, F.chroma == 2 ? 2u : 4u, qtabs + (size_t)P.qtab * 225 * 128, qi,
+              P.header_bytes, alpha_len[g], (const uint8_t *)lens, size[0], fits[0]);
   if ((threadIdx.x & 63) == 0) {
+    // Every probe's sizes are kept: the share-out starts from the uniform
+    // quantiser's and the next finer one's, both of which the search probed.
+    probe_sizes[(size_t)round * n_slices + g] = size[0] | (fits[0] ? 0u : 0x80000000u);
     atomicAdd(&state[S.pic].sum, (unsigned long long)size[0]);
     if (!fits[0]) atomicAdd(&state[S.pic].notfit, 1u);
   }
@@ -467,6 +490,9 @@ extern "C" __global__ void __launch_bounds__(64)
     st.budget = b > over ? b - over : 0;
   }
   const bool ok = st.notfit == 0 && st.sum <= st.budget;
+  st.probe_qi[st.round] = st.probe;
+  st.probe_sum[st.round] = st.sum;
+  st.round++;
   if (st.phase == 0) {
     st.fits_at_all = ok;
     if (ok) {
@@ -498,32 +524,62 @@ extern "C" __global__ void __launch_bounds__(64)
   st.notfit = 0;
 }
 
+// One wavefront per slice: the size table's columns, qi = uniform - t.
+// Columns 0 and 1 are the probes at uniform and uniform - 1 (whenever
+// uniform > 1 the search probed both: `hi` only ever takes probed values, and
+// `lo` reached uniform by lo = mid + 1). Further columns are coded only for
+// the slices the reference precomputes them for (src/encode.rs:741-766): a
+// slice whose previous column fits and is at most `most`, its size at
+// uniform plus the slack shared from it on if no slice before it takes any.
+// A column left 0 is coded by the share-out if it turns out to need it.
 extern "C" __global__ void __launch_bounds__(256)
-    pe_window(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
+    pe_window(const DevTables *__restrict__ T, const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
               const EncSlice *__restrict__ slices, uint32_t n_slices, const float *__restrict__ coefs,
               const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
-              const EncState *__restrict__ state, uint32_t *__restrict__ size_tab) {
+              const EncState *__restrict__ state, const uint32_t *__restrict__ probe_sizes,
+              uint32_t *__restrict__ size_tab) {
+  __shared__ uint32_t lens[ENC_LEN_BYTES / 4];
+  load_len_tables(T, lens);
   const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (g >= n_slices) return;
+  const uint32_t L = threadIdx.x & 63;
   const EncSlice S = slices[g];
   const EncState &st = state[S.pic];
-  const EncPic &P = pics[S.pic];
-  const EncFrame &F = frames[P.frame];
-  const uint32_t K = st.window_k;
-  uint32_t qis[ENC_MAXK];
+  const uint32_t U = st.uniform, K = st.window_k, R = st.round;
+  uint32_t ru = 0, ru1 = 0;
+  for (uint32_t r = 0; r < R; r++) {
+    if (st.probe_qi[r] == U) ru = r;
+    if (st.probe_qi[r] == U - 1) ru1 = r;
+  }
+  uint32_t col[ENC_MAXK];
 #pragma unroll
-  for (uint32_t k = 0; k < ENC_MAXK; k++) qis[k] = k < K ? st.uniform - k : st.uniform;
-  uint32_t size[ENC_MAXK];
-  bool fits[ENC_MAXK];
-  slice_sizes<ENC_MAXK>(coefs + S.coef, S.mbs, F.chroma == 2 ? 2u : 4u,
-                        qtabs + (size_t)P.qtab * 225 * 128, qis, K, P.header_bytes, alpha_len[g],
-                        size, fits);
-  const uint32_t L = threadIdx.x & 63;
-  if (L < K) {
+  for (uint32_t t = 0; t < ENC_MAXK; t++) col[t] = 0;
+  col[0] = probe_sizes[(size_t)ru * n_slices + g];
+  if (K > 1) {
+    col[1] = probe_sizes[(size_t)ru1 * n_slices + g];
+    const EncPic &P = pics[S.pic];
+    const EncFrame &F = frames[P.frame];
+    const float *qt = qtabs + (size_t)P.qtab * 225 * 128;
+    const uint32_t cpm = F.chroma == 2 ? 2u : 4u, alen = alpha_len[g];
+    // The search found uniform fitting, so its total is within the budget.
+    const uint64_t slack = st.budget - st.probe_sum[ru];
+    const uint64_t most = (uint64_t)(col[0] & 0x7FFFFFFFu) + slack * S.mbs / S.mbs_from;
+#pragma unroll
+    for (uint32_t t = 2; t < ENC_MAXK; t++) {
+      const uint32_t prev = col[t - 1];
+      if (t >= K || prev == 0 || (prev >> 31) || (uint64_t)(prev & 0x7FFFFFFFu) > most) break;
+      const uint32_t q = U - t;
+      uint32_t sz[1];
+      bool ft[1];
+      slice_bytes(coefs + S.coef, S.mbs, cpm, qt, q, P.header_bytes, alen, (const uint8_t *)lens, sz[0], ft[0]);
+      col[t] = sz[0] | (ft[0] ? 0u : 0x80000000u);
+    }
+  }
+  if (L < ENC_MAXK) {
     uint32_t v = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < ENC_MAXK; k++)
-      if (k == L) v = size[k] | (fits[k] ? 0u : 0x80000000u);
+    for (uint32_t t = 0; t < ENC_MAXK; t++)
+      if (t == L) v = col[t];
     size_tab[(size_t)g * ENC_MAXK + L] = v;
   }
 }
@@ -536,13 +592,15 @@ extern "C" __global__ void __launch_bounds__(256)
 // the prefix sums of the decisions, and the loop stops at the fixed point,
 // which is the sequential answer (lane j is exact after j rounds).
 extern "C" __global__ void __launch_bounds__(64 * SHARE_WAVES)
-    pe_shareout(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
+    pe_shareout(const DevTables *__restrict__ T, const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
                 const EncSlice *__restrict__ slices, const float *__restrict__ coefs,
                 const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
                 EncState *__restrict__ state, const uint32_t *__restrict__ size_tab,
                 uint32_t *__restrict__ deep, uint32_t *__restrict__ used, uint32_t *__restrict__ qidx) {
   __shared__ uint32_t req_g[64], req_t[64];
   __shared__ uint32_t n_req, done;
+  __shared__ uint32_t lens[ENC_LEN_BYTES / 4];
+  load_len_tables(T, lens);
   const uint32_t p = blockIdx.x, tid = threadIdx.x, L = tid & 63, wave = tid >> 6;
   const EncPic &P = pics[p];
   EncState &st = state[p];
@@ -623,11 +681,10 @@ extern "C" __global__ void __launch_bounds__(64 * SHARE_WAVES)
           if (walking) {
             const uint32_t t = U - qi + 1;  // the column of qi - 1
             uint32_t v;
-            if (t < K) {
-              v = t == 1 ? w1 : t == 2 ? w2 : t == 3 ? w3 : t == 4 ? w4 : t == 5 ? w5 : t == 6 ? w6 : w7;
-            } else {
-              v = deep[(size_t)g * ENC_DEEP + t];
-            }
+            v = 0;
+            if (t < K) v = t == 1 ? w1 : t == 2 ? w2 : t == 3 ? w3 : t == 4 ? w4 : t == 5 ? w5 : t == 6 ? w6 : w7;
+            // A column the window left out (0), or one past it: the cache.
+            if (v == 0) v = deep[(size_t)g * ENC_DEEP + t];
             if (v == 0) {
               need = true;
             } else if ((v >> 31) || (v & 0x7FFFFFFFu) > allowed) {
@@ -689,8 +746,8 @@ extern "C" __global__ void __launch_bounds__(64 * SHARE_WAVES)
       const uint32_t q = U - t - k;
       uint32_t sz[1];
       bool ft[1];
-      slice_sizes<1>(coefs + slices[gr].coef, slices[gr].mbs, cpm, qt, &q, 1, P.header_bytes, alpha_len[gr],
-                     sz, ft);
+      slice_bytes(coefs + slices[gr].coef, slices[gr].mbs, cpm, qt, q, P.header_bytes, alpha_len[gr],
+                  (const uint8_t *)lens, sz[0], ft[0]);
       if (L == 0) deep[(size_t)gr * ENC_DEEP + t + k] = sz[0] | (ft[0] ? 0u : 0x80000000u);
     }
     served += nr;

@@ -242,6 +242,20 @@ void build_tables(DevTables &t) {
       t.scan[1][8 * u + v] = SCAN_P[8 * v + u];
     }
   fill_fast(t);
+  for (int c = 0; c < 16; c++)
+    for (uint32_t n = 0; n < 64; n++) {
+      uint64_t bits;
+      uint32_t len;
+      code_of(RUN_BOOKS[c], n, bits, len);
+      t.enc_run_len[c][n] = (uint8_t)len;
+    }
+  for (int c = 0; c < 9; c++)
+    for (uint32_t n = 0; n < 256; n++) {
+      uint64_t bits;
+      uint32_t len;
+      code_of(LEVEL_BOOKS[c], n, bits, len);
+      t.enc_level_len[c][n] = (uint8_t)len;
+    }
 }
 
 // ---------------------------------------------------------------------------

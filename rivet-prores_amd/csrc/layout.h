@@ -38,7 +38,13 @@ struct DevTables {
   // Books are the six distinct level and six distinct run codebooks.
   uint16_t fast_level[6][1024];
   uint16_t fast_run[6][512];
+  // The encoder's code lengths (Codebook::len, src/vlc.rs:155-172) by
+  // context, for the symbols the size count meets most: runs below 64 and
+  // levels (|q| - 1) below 256; larger symbols take the closed form.
+  uint8_t enc_run_len[16][64];
+  uint8_t enc_level_len[9][256];
 };
+constexpr uint32_t ENC_LEN_BYTES = 16 * 64 + 9 * 256;
 constexpr uint32_t FAST_VALID = 0x8000;
 constexpr uint32_t FAST_RUN_BASE = 6 * 1024 * 2;  // byte offset of fast_run after fast_level
 
