@@ -241,7 +241,7 @@ Err plan_encode(const prores_config &cfg, const prores_frame &f, uint64_t dev_da
 
 struct EncWorkspace {
   DevBuf frames, pics, slices, states, qtabs, hdrs, coefs, alpha_scratch, alpha_len, size_tab, deep, used, qidx,
-      slice_out, frame_size, out, planes, alpha_in, probe_sizes;
+      slice_out, frame_size, out, planes, alpha_in, probe_sizes, chunk_bounds;
 };
 EncWorkspace &enc_workspace(Lane &L) {
   if (!L.enc) L.enc = new EncWorkspace();
@@ -276,6 +276,7 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   if (!(e = ws.alpha_len.reserve(G * 4 + 64)).ok()) return e;
   if (!(e = ws.size_tab.reserve(G * ENC_MAXK * 4 + 64)).ok()) return e;
   if (!(e = ws.probe_sizes.reserve(G * ENC_PROBES * 4 + 64)).ok()) return e;
+  if (!(e = ws.chunk_bounds.reserve(b.n_coefs / 64 * 4 + 512)).ok()) return e;
   if (!(e = ws.deep.reserve(G * ENC_DEEP * 4 + 64)).ok()) return e;
   if (!(e = ws.used.reserve(G * 4 + 64)).ok()) return e;
   if (!(e = ws.qidx.reserve(G * 4 + 64)).ok()) return e;
@@ -285,12 +286,12 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   void *qtabs = ws.qtabs.p, *hdrs = ws.hdrs.p, *coefs = ws.coefs.p, *ascr = ws.alpha_scratch.p;
   void *alen = ws.alpha_len.p, *stab = ws.size_tab.p, *used = ws.used.p, *qidx = ws.qidx.p;
   void *sout = ws.slice_out.p, *fsize = ws.frame_size.p, *tables = d.tables, *deep = ws.deep.p;
-  void *psz = ws.probe_sizes.p;
+  void *psz = ws.probe_sizes.p, *cbnd = ws.chunk_bounds.p;
   uint32_t g32 = (uint32_t)G, np32 = NP, window = WINDOW;
   hipEvent_t *ev = L.eev;
   (void)hipEventRecord(ev[0], L.stream);
   {
-    void *args[] = {&tables, &frames, &pics, &slices, &coefs};
+    void *args[] = {&tables, &frames, &pics, &slices, &qtabs, &coefs, &cbnd};
     if (!(e = launch(L, d.k_fdct, g32, 1, 256, args)).ok()) return e;
   }
   (void)hipEventRecord(ev[1], L.stream);
@@ -301,20 +302,21 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   (void)hipEventRecord(ev[2], L.stream);
   // qi = 224, then at most 8 bisection steps (224 < 2^8).
   for (uint32_t probe = 0; probe < ENC_PROBES; probe++) {
-    void *a1[] = {&tables, &frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &psz};
+    void *a1[] = {&tables, &frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &psz, &cbnd};
     if (!(e = launch(L, d.k_size, (g32 + 3) / 4, 1, 256, a1)).ok()) return e;
     void *a2[] = {&pics, &states, &np32, &window};
     if (!(e = launch(L, d.k_search, (np32 + 63) / 64, 1, 64, a2)).ok()) return e;
   }
   (void)hipEventRecord(ev[3], L.stream);
   {
-    void *args[] = {&tables, &frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &psz, &stab};
+    void *args[] = {&tables, &frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &psz, &cbnd, &stab};
     if (!(e = launch(L, d.k_window, (g32 + 3) / 4, 1, 256, args)).ok()) return e;
   }
   (void)hipEventRecord(ev[4], L.stream);
   if (G && !(e = check(hipMemsetAsync(ws.deep.p, 0, G * ENC_DEEP * 4, L.stream), "memset")).ok()) return e;
   {
-    void *args[] = {&tables, &frames, &pics, &slices, &coefs, &qtabs, &alen, &states, &stab, &deep, &used, &qidx};
+    void *args[] = {&tables, &frames, &pics, &slices, &coefs, &qtabs, &alen, &states, &stab, &deep, &used, &qidx,
+                    &cbnd};
     if (!(e = launch(L, d.k_shareout, np32, 1, 64 * SHARE_WAVES, args)).ok()) return e;
   }
   (void)hipEventRecord(ev[5], L.stream);

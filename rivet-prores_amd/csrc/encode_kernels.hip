@@ -127,6 +127,8 @@ __device__ __forceinline__ int64_t excl_scan64(int64_t v, uint32_t lane) {
 }
 
 __device__ __forceinline__ uint32_t ilog2(uint32_t v) { return 31 - __builtin_clz(v); }
+// Table 15 of RDD 36 (src/tables.rs:35-38).
+__device__ __forceinline__ uint32_t qscale(uint32_t q) { return q <= 128 ? q : 128 + 4 * (q - 128); }
 
 // Component c of a slice: its block count and its offset inside the slice's
 // coefficients (Y', then Cb, then Cr; src/encode.rs:557-564).
@@ -169,15 +171,24 @@ __device__ __forceinline__ void load_len_tables(const DevTables *__restrict__ T,
   __syncthreads();
 }
 
+// A chunk of 64 coefficients whose bound (pe_fdct: the largest
+// |F| * fl(8 / W) over the chunk) is at most qScale(qi) * ZERO_CHUNK_LIMIT
+// quantises to zeros at qi. In f32, fl(|F| fl(8 / (W q))) is at most
+// bound / q * (1 + 5 * 2^-24), so below 0.58 - 10^-4, and adding r = 0.42f
+// stays under 1: floor gives 0 for every AC coefficient of the chunk. The
+// first chunk of a component (its DC coefficients) is always coded.
+constexpr float ZERO_CHUNK_LIMIT = 0.5798f;
+
 // Bytes of one slice at quantiser qi, one wavefront, and whether every size
 // field can hold them (src/encode.rs:671-681): per component, the exact bit
 // count of coefficient_bits (src/encode.rs:366-409) over the quantised
 // coefficients (src/dsp/scalar.rs:19-39), a 64-coefficient chunk of the
 // scanned array a step, one coefficient a lane.
+// - Chunks that quantise to zeros (by their bounds `cb`) are skipped: they
+//   add no bits and leave the run/level contexts as they are.
 // - The inverse quantiser of a lane's scan position comes from a register
 //   (lane k holds the component's entry k) by a lane permute, not a load.
-// - The next chunk's coefficients are loaded while this one is counted (the
-//   arena has a chunk of padding after its last slice).
+// - The next coded chunk's coefficients are loaded while this one is counted.
 // - Run and level contexts travel as one packed word (min(run, 15) |
 //   min(level, 8) << 4): one permute from the previous nonzero lane, and the
 //   chunk's last nonzero passes its word to the next chunk by a readlane.
@@ -186,31 +197,44 @@ __device__ __forceinline__ void load_len_tables(const DevTables *__restrict__ T,
 // No conversion here saturates: a coefficient of fdct_load's output is at
 // most 64 * 256 * (1/2)^2 = 4096 in magnitude and an inverse quantiser at
 // most 8 / 2, so |x| + r < 2^15.
-__device__ void slice_bytes(const float *__restrict__ coef, uint32_t mbs, uint32_t cpm,
-                            const float *__restrict__ qt, uint32_t qi, uint32_t hb, uint32_t alen,
+__device__ void slice_bytes(const float *__restrict__ coef, const float *__restrict__ cb, uint32_t mbs,
+                            uint32_t cpm, const float *__restrict__ qt, uint32_t qi, uint32_t hb, uint32_t alen,
                             const uint8_t *lens, uint32_t &size, bool &fits) {
   const uint32_t L = threadIdx.x & 63;
   const uint64_t below = (1ull << L) - 1;
   const uint8_t *run_lens = lens, *level_lens = lens + 16 * 64;
+  // Chunks that may hold a nonzero at this quantiser: bit k of live[k / 64]
+  // (a slice has at most 96 chunks, and no component crosses chunk 64:
+  // slices have 1, 2, 4 or 8 macroblocks).
+  const uint32_t TB = (4 + 2 * cpm) * mbs;
+  const float qlim = (float)qscale(qi) * ZERO_CHUNK_LIMIT;
+  const uint64_t live0 = __ballot(L < TB && cb[L < TB ? L : 0] > qlim);
+  const uint64_t live1 = __ballot(L + 64 < TB && cb[L + 64 < TB ? L + 64 : 0] > qlim);
   size = hb + alen;
   fits = true;
+  uint32_t k0 = 0;  // the component's first chunk in the slice
   for (uint32_t c = 0; c < 3; c++) {
     uint32_t nb, off;
     component(c, mbs, cpm, nb, off);
-    const uint32_t lg = ilog2(nb), n = nb * 64;
+    const uint32_t lg = ilog2(nb);
+    uint64_t todo = (((k0 >> 6) ? live1 : live0) >> (k0 & 63)) & ((1ull << nb) - 1);
+    todo |= 1;  // the DC chunk
     const float *cf = coef + off + L;
     const float invL = qt[(qi * 2 + (c > 0 ? 1u : 0u)) * 64 + L];
     uint32_t bits = 0;
     uint32_t last_idx = nb - 1, ctx = 4u | 1u << 4;  // no nonzero yet: run context 4, level context 1
+    uint32_t j = 0;
     float f = cf[0];
-    for (uint32_t base = 0; base < n; base += 64) {
-      const float fn = cf[base + 64];
-      const uint32_t i = base + L, s = i >> lg;
+    for (;;) {
+      todo &= todo - 1;  // chunk j is the lowest bit
+      const uint32_t jn = todo ? (uint32_t)__builtin_ctzll(todo) : j;
+      const float fn = cf[jn * 64];
+      const uint32_t base = j * 64, i = base + L, s = i >> lg;
       const float inv = __shfl(invL, (int)s, 64);
       const float x = f * inv;
       const uint32_t m = (uint32_t)(int32_t)floorf(fabsf(x) + (s == 0 ? 0.5f : 0.42f));
       bool nz = m != 0;
-      if (base == 0) {  // the DC coefficients (src/encode.rs:383-395), nb <= 32 of the first chunk
+      if (j == 0) {  // the DC coefficients (src/encode.rs:383-395), nb <= 32 of the first chunk
         const int32_t q = x < 0.0f ? -(int32_t)m : (int32_t)m;
         const int32_t d1 = __shfl(q, L ? (int)L - 1 : 0, 64);
         const int32_t d2 = __shfl(q, L > 1 ? (int)L - 2 : 0, 64);
@@ -246,11 +270,14 @@ __device__ void slice_bytes(const float *__restrict__ coef, uint32_t mbs, uint32
         last_idx = base + last;
         ctx = __builtin_amdgcn_readlane(mine, last);
       }
+      if (!todo) break;
+      j = jn;
       f = fn;
     }
     const uint32_t bytes = (wave_sum(bits) + 7) / 8;
     size += bytes;
     if (bytes > 65535) fits = false;
+    k0 += nb;
   }
   if (size > 65535) fits = false;
 }
@@ -265,7 +292,7 @@ __device__ __forceinline__ uint16_t rescale(uint32_t a, uint32_t identity, uint6
 extern "C" __global__ void __launch_bounds__(256)
     pe_fdct(const DevTables *__restrict__ T, const EncFrame *__restrict__ frames,
             const EncPic *__restrict__ pics, const EncSlice *__restrict__ slices,
-            float *__restrict__ coefs) {
+            const float *__restrict__ qtabs, float *__restrict__ coefs, float *__restrict__ cbound) {
   __shared__ float outc[96 * 64];
   __shared__ float tmp[32 * 64];
   const uint32_t g = blockIdx.x, tid = threadIdx.x;
@@ -331,6 +358,29 @@ extern "C" __global__ void __launch_bounds__(256)
   }
   float *dst = coefs + S.coef;
   for (uint32_t i = tid; i < TB * 64; i += 256) dst[i] = outc[i];
+  // Each chunk's bound for the size count: the largest |F| * fl(8 / W) of
+  // its 64 coefficients (the quantiser table's row for qi = 1), four
+  // threads a chunk.
+  const float *qt1 = qtabs + (size_t)P.qtab * 225 * 128 + 2 * 64;
+  for (uint32_t k0 = 0; k0 < TB; k0 += 64) {
+    const uint32_t k = k0 + (tid >> 2), part = tid & 3;
+    float mx = 0.0f;
+    if (k < TB) {
+      const uint32_t c = k < nbY ? 0u : (k < nbY + nbC ? 1u : 2u);
+      const uint32_t kc = c == 0 ? 0u : (c == 1 ? nbY : nbY + nbC), nb = c == 0 ? nbY : nbC;
+      const uint32_t lg = 31 - __builtin_clz(nb);
+      const float *inv1 = qt1 + (c > 0 ? 64 : 0);
+      for (uint32_t e = 16 * part; e < 16 * part + 16; e++) {
+        const float v = fabsf(outc[64 * k + e]) * inv1[(64 * (k - kc) + e) >> lg];
+        mx = v > mx ? v : mx;
+      }
+    }
+    const float m1 = __shfl_xor(mx, 1, 64);
+    mx = m1 > mx ? m1 : mx;
+    const float m2 = __shfl_xor(mx, 2, 64);
+    mx = m2 > mx ? m2 : mx;
+    if (k < TB && part == 0) cbound[S.coef / 64 + k] = mx;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -452,7 +502,7 @@ extern "C" __global__ void __launch_bounds__(256)
     pe_size(const DevTables *__restrict__ T, const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
             const EncSlice *__restrict__ slices, uint32_t n_slices, const float *__restrict__ coefs,
             const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
-            EncState *__restrict__ state, uint32_t *__restrict__ probe_sizes) {
+            EncState *__restrict__ state, uint32_t *__restrict__ probe_sizes, const float *__restrict__ cbound) {
   __shared__ uint32_t lens[ENC_LEN_BYTES / 4];
   load_len_tables(T, lens);
   const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -465,8 +515,9 @@ extern "C" __global__ void __launch_bounds__(256)
   const EncFrame &F = frames[P.frame];
   uint32_t size[1];
   bool fits[1];
-  slice_bytes(coefs + S.coef, S.mbs, F.chroma == 2 ? 2u : 4u, qtabs + (size_t)P.qtab * 225 * 128, qi,
-              P.header_bytes, alpha_len[g], (const uint8_t *)lens, size[0], fits[0]);
+  slice_bytes(coefs + S.coef, cbound + S.coef / 64, S.mbs, F.chroma == 2 ? 2u : 4u,
+              qtabs + (size_t)P.qtab * 225 * 128, qi, P.header_bytes, alpha_len[g], (const uint8_t *)lens, size[0],
+              fits[0]);
   if ((threadIdx.x & 63) == 0) {
     // Every probe's sizes are kept: the share-out starts from the uniform
     // quantiser's and the next finer one's, both of which the search probed.
@@ -537,7 +588,7 @@ extern "C" __global__ void __launch_bounds__(256)
               const EncSlice *__restrict__ slices, uint32_t n_slices, const float *__restrict__ coefs,
               const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
               const EncState *__restrict__ state, const uint32_t *__restrict__ probe_sizes,
-              uint32_t *__restrict__ size_tab) {
+              const float *__restrict__ cbound, uint32_t *__restrict__ size_tab) {
   __shared__ uint32_t lens[ENC_LEN_BYTES / 4];
   load_len_tables(T, lens);
   const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -571,7 +622,8 @@ extern "C" __global__ void __launch_bounds__(256)
       const uint32_t q = U - t;
       uint32_t sz[1];
       bool ft[1];
-      slice_bytes(coefs + S.coef, S.mbs, cpm, qt, q, P.header_bytes, alen, (const uint8_t *)lens, sz[0], ft[0]);
+      slice_bytes(coefs + S.coef, cbound + S.coef / 64, S.mbs, cpm, qt, q, P.header_bytes, alen,
+                  (const uint8_t *)lens, sz[0], ft[0]);
       col[t] = sz[0] | (ft[0] ? 0u : 0x80000000u);
     }
   }
@@ -596,7 +648,8 @@ extern "C" __global__ void __launch_bounds__(64 * SHARE_WAVES)
                 const EncSlice *__restrict__ slices, const float *__restrict__ coefs,
                 const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
                 EncState *__restrict__ state, const uint32_t *__restrict__ size_tab,
-                uint32_t *__restrict__ deep, uint32_t *__restrict__ used, uint32_t *__restrict__ qidx) {
+                uint32_t *__restrict__ deep, uint32_t *__restrict__ used, uint32_t *__restrict__ qidx,
+                const float *__restrict__ cbound) {
   __shared__ uint32_t req_g[64], req_t[64];
   __shared__ uint32_t n_req, done;
   __shared__ uint32_t lens[ENC_LEN_BYTES / 4];
@@ -746,8 +799,8 @@ extern "C" __global__ void __launch_bounds__(64 * SHARE_WAVES)
       const uint32_t q = U - t - k;
       uint32_t sz[1];
       bool ft[1];
-      slice_bytes(coefs + slices[gr].coef, slices[gr].mbs, cpm, qt, q, P.header_bytes, alpha_len[gr],
-                  (const uint8_t *)lens, sz[0], ft[0]);
+      slice_bytes(coefs + slices[gr].coef, cbound + slices[gr].coef / 64, slices[gr].mbs, cpm, qt, q,
+                  P.header_bytes, alpha_len[gr], (const uint8_t *)lens, sz[0], ft[0]);
       if (L == 0) deep[(size_t)gr * ENC_DEEP + t + k] = sz[0] | (ft[0] ? 0u : 0x80000000u);
     }
     served += nr;
