@@ -72,25 +72,6 @@ struct GraphKey {
   bool operator==(const GraphKey &o) const { return memcmp(this, &o, sizeof *this) == 0; }
 };
 
-// Page-locked host memory, grow-only: a lane's staging for the batch plan
-// going up and the per-picture results coming back, so that each is one
-// asynchronous copy.
-struct HostBuf {
-  void *p = nullptr;
-  size_t cap = 0;
-  Err reserve(size_t bytes) {
-    if (bytes <= cap) return Err{};
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    cap = 0;
-    const size_t want = bytes + bytes / 4 + 4096;
-    const hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
-    if (e != hipSuccess) return device_error("hipHostMalloc", e);
-    cap = want;
-    return Err{};
-  }
-};
-
 struct Workspace {
   DevBuf slices, slice_err, coefs, wide, packet, planes, items, alpha_order, gather, records, order;
   // The batch plan (frames, pictures, row slots, IDCT segments, header) in

@@ -14,7 +14,6 @@ constexpr uint32_t ALPHA_SLICE_CAP = 4608;  // bytes: 2048 values x 18 bits, the
 struct EncFrame {
   uint64_t in_plane[3];   // u16* Y', Cb, Cr (tightly packed)
   uint64_t in_alpha;      // u16* or 0 (alpha coded only when alpha_bits != 0)
-  uint64_t out;           // device address of the frame's output bytes
   uint32_t plane_w[3];
   uint32_t width, height, chroma, bit_depth, interlaced;
   uint32_t alpha_bits;    // 0, 8, 16: as coded
@@ -29,10 +28,21 @@ struct EncFrame {
 struct EncPic {
   uint64_t budget;        // per-picture budget before alpha and overhead
   uint64_t out_off;       // picture offset inside the frame (pe_offsets)
+  uint64_t coef_base;     // element offset of the picture's first slice coefficients
   uint32_t frame, n_slices, slice_base, height;
   uint32_t first_row, row_step, header_bytes, log2_slice_mbs;
   uint32_t qtab;          // index of the 225 x 2 x 64 inverse-quantiser table
   uint32_t interlaced;
+  // Slice layout: `rows` rows of `row_slices` slices, laid out as the
+  // EncRowSlot entries from `layout_base`; `row_mbs` macroblocks a row.
+  uint32_t rows, row_slices, layout_base, row_mbs;
+};
+
+// Slice j of a picture's row: its first macroblock, its macroblocks and the
+// macroblocks of the row before it (pe_slices expands the slice records).
+struct EncRowSlot {
+  uint16_t mb_x, mbs;
+  uint32_t mbs_before;
 };
 
 struct EncSlice {

@@ -5,6 +5,7 @@
 // run on the device. One launch sequence covers a whole batch of frames.
 #include <stdlib.h>
 
+#include <algorithm>
 #include <map>
 
 #include "enc_layout.h"
@@ -97,36 +98,64 @@ void write_frame_header(const Header &hd, std::vector<uint8_t> &out) {  // src/h
 struct EncBatch {
   std::vector<EncFrame> frames;
   std::vector<EncPic> pics;
-  std::vector<EncSlice> slices;
+  std::vector<EncRowSlot> slots;  // the pictures' row layouts
+  std::map<std::pair<uint32_t, uint32_t>, uint32_t> slot_index;  // (width in MBs, log2) -> first slot
+  uint64_t n_slices = 0;
+  uint32_t max_pic_slices = 0;
   std::vector<EncState> states;
-  std::vector<float> qtabs;
   std::vector<uint8_t> hdrs;
+  // The distinct quantiser-table keys of the batch (luma and chroma
+  // weights, interlaced); a picture's qtab names one until run_encode maps
+  // it to the table's slot in the lane's device table cache.
+  std::vector<std::vector<uint8_t>> qkeys;
   std::map<std::vector<uint8_t>, uint32_t> qtab_index;
   std::vector<int> job_frame;  // batch frame index per job, -1 when the job failed its checks
+  // Each frame's output arena guess: its target_frame_bytes plus a margin,
+  // or the most its slices could take if that is less.
+  std::vector<uint64_t> targets;
   uint64_t n_coefs = 0;
 };
 
+constexpr size_t QTAB_FLOATS = 225 * 128;
+
+// 8 / (W qScale) at scanned positions for luma and chroma and every
+// quantization_index (index 0 as 1), IEEE single division (src/encode.rs:
+// 303-324), for a key (luma weights, chroma weights, interlaced), into
+// `out` (QTAB_FLOATS floats). Tables computed before are kept, process-wide:
+// the same matrices recur from frame to frame.
+void quantiser_values(const std::vector<uint8_t> &key, float *out) {
+  static std::mutex mu;
+  static std::map<std::vector<uint8_t>, std::vector<float>> *cache =
+      new std::map<std::vector<uint8_t>, std::vector<float>>();  // never destroyed: encodes may run at exit
+  std::lock_guard<std::mutex> g(mu);
+  auto c = cache->find(key);
+  if (c == cache->end()) {
+    if (cache->size() >= 4096) cache->clear();
+    std::vector<float> t(QTAB_FLOATS);
+    for (uint32_t qi = 0; qi <= 224; qi++) {
+      const uint32_t q = qscale(qi ? qi : 1);
+      for (int k = 0; k < 64; k++) {
+        const uint32_t s = key[128] ? SCAN_P[8 * (k % 8) + k / 8] : SCAN_P[k];
+        t[qi * 128 + s] = 8.0f / (float)((uint32_t)key[k] * q);
+        t[qi * 128 + 64 + s] = 8.0f / (float)((uint32_t)key[64 + k] * q);
+      }
+    }
+    c = cache->emplace(key, std::move(t)).first;
+  }
+  memcpy(out, c->second.data(), QTAB_FLOATS * sizeof(float));
+}
+
+// The batch-local index of a frame's quantiser table key.
 uint32_t quantiser_table(EncBatch &b, const Header &hd) {
-  // 8 / (W qScale) at scanned positions for luma and chroma and every
-  // quantization_index (index 0 as 1), IEEE single division (src/encode.rs:303-324).
   std::vector<uint8_t> key(129);
   hd.luma_weights(key.data());
   hd.chroma_weights(key.data() + 64);
   key[128] = hd.h.interlace != 0;
   auto it = b.qtab_index.find(key);
   if (it != b.qtab_index.end()) return it->second;
-  const uint32_t index = (uint32_t)b.qtab_index.size();
+  const uint32_t index = (uint32_t)b.qkeys.size();
   b.qtab_index[key] = index;
-  size_t base = b.qtabs.size();
-  b.qtabs.resize(base + 225 * 128);
-  for (uint32_t qi = 0; qi <= 224; qi++) {
-    const uint32_t q = qscale(qi ? qi : 1);
-    for (int k = 0; k < 64; k++) {
-      const uint32_t s = key[128] ? SCAN_P[8 * (k % 8) + k / 8] : SCAN_P[k];
-      b.qtabs[base + qi * 128 + s] = 8.0f / (float)((uint32_t)key[k] * q);
-      b.qtabs[base + qi * 128 + 64 + s] = 8.0f / (float)((uint32_t)key[64 + k] * q);
-    }
-  }
+  b.qkeys.push_back(key);
   return index;
 }
 
@@ -190,10 +219,26 @@ Err plan_encode(const prores_config &cfg, const prores_frame &f, uint64_t dev_da
   const uint32_t qtab = quantiser_table(b, hd);
   F.first_pic = (uint32_t)b.pics.size();
   F.n_pics = npics;
-  F.first_slice = (uint32_t)b.slices.size();
+  F.first_slice = (uint32_t)b.n_slices;
   const uint32_t fi = (uint32_t)b.frames.size();
   const uint32_t wmb = (f.width + 15) / 16, cpm = f.chroma == 2 ? 2u : 4u;
   const std::vector<uint32_t> sizes = slice_sizes(wmb, cfg.log2_slice_mbs);
+  uint32_t layout_base;
+  {
+    const auto key = std::make_pair(wmb, cfg.log2_slice_mbs);
+    auto it = b.slot_index.find(key);
+    if (it != b.slot_index.end()) {
+      layout_base = it->second;
+    } else {
+      layout_base = (uint32_t)b.slots.size();
+      b.slot_index[key] = layout_base;
+      uint32_t mx = 0;
+      for (uint32_t m : sizes) {
+        b.slots.push_back(EncRowSlot{(uint16_t)mx, (uint16_t)m, mx});
+        mx += m;
+      }
+    }
+  }
   for (uint32_t index = 0; index < npics; index++) {
     EncPic P;
     memset(&P, 0, sizeof P);
@@ -205,30 +250,16 @@ Err plan_encode(const prores_config &cfg, const prores_frame &f, uint64_t dev_da
     P.log2_slice_mbs = cfg.log2_slice_mbs;
     P.qtab = qtab;
     P.interlaced = F.interlaced;
-    P.slice_base = (uint32_t)b.slices.size();
-    const uint32_t hmb = (P.height + 15) / 16;
-    const uint32_t pic = (uint32_t)b.pics.size();
-    for (uint32_t my = 0; my < hmb; my++) {
-      uint32_t mx = 0;
-      for (uint32_t m : sizes) {
-        EncSlice s;
-        memset(&s, 0, sizeof s);
-        s.coef = b.n_coefs;
-        s.pic = pic;
-        s.mb_x = (uint16_t)mx;
-        s.mb_y = (uint16_t)my;
-        s.mbs = m;
-        b.slices.push_back(s);
-        b.n_coefs += 64ull * (4 + 2 * cpm) * m;
-        mx += m;
-      }
-    }
-    P.n_slices = (uint32_t)(b.slices.size() - P.slice_base);
-    uint32_t from = 0;  // macroblocks from each slice to the end of the picture
-    for (size_t k = b.slices.size(); k-- > P.slice_base;) {
-      from += b.slices[k].mbs;
-      b.slices[k].mbs_from = from;
-    }
+    P.slice_base = (uint32_t)b.n_slices;
+    P.rows = (P.height + 15) / 16;
+    P.row_slices = (uint32_t)sizes.size();
+    P.layout_base = layout_base;
+    P.row_mbs = wmb;
+    P.n_slices = P.rows * P.row_slices;
+    P.coef_base = b.n_coefs;
+    b.n_coefs += 64ull * (4 + 2 * cpm) * wmb * P.rows;
+    b.n_slices += P.n_slices;
+    b.max_pic_slices = std::max(b.max_pic_slices, P.n_slices);
     b.pics.push_back(P);
     EncState st;
     memset(&st, 0, sizeof st);
@@ -236,24 +267,26 @@ Err plan_encode(const prores_config &cfg, const prores_frame &f, uint64_t dev_da
     b.states.push_back(st);
   }
   b.frames.push_back(F);
+  const uint64_t most = F.hdr_len + 8ull * npics + (b.n_slices - F.first_slice) * (2 + 65535);
+  b.targets.push_back(std::min<uint64_t>(target + 4096, most));
   return Err{};
 }
 
 struct EncWorkspace {
-  DevBuf frames, pics, slices, states, qtabs, hdrs, coefs, alpha_scratch, alpha_len, size_tab, deep, used, qidx,
-      slice_out, frame_size, out, planes, alpha_in, probe_sizes, chunk_bounds;
+  DevBuf plan, slices, coefs, alpha_scratch, alpha_len, size_tab, deep, used, qidx, slice_out, frame_size, out, planes,
+      alpha_in, probe_sizes, chunk_bounds, out_addr;
+  // The plan (frames, pictures, slices, rate-control states, quantiser
+  // tables, frame headers) goes up in one copy from page-locked staging; the
+  // frame sizes and the arena's overflow flag come back in one.
+  HostBuf hplan, hres;
+  // Quantiser tables resident on the device, one QTAB_FLOATS slot a key,
+  // uploaded the first time a batch on this lane needs them.
+  DevBuf qarena;
+  std::map<std::vector<uint8_t>, uint32_t> qslot;
 };
 EncWorkspace &enc_workspace(Lane &L) {
   if (!L.enc) L.enc = new EncWorkspace();
   return *(EncWorkspace *)L.enc;
-}
-
-template <class T>
-Err upload(Lane &L, DevBuf &buf, const std::vector<T> &v) {
-  Err e = buf.reserve(v.size() * sizeof(T) + 16);
-  if (!e.ok() || v.empty()) return e;
-  return check(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, L.stream),
-               "hipMemcpyAsync(encode plan)");
 }
 
 // Runs the kernels over a planned batch; on return the packets are in the
@@ -261,16 +294,68 @@ Err upload(Lane &L, DevBuf &buf, const std::vector<T> &v) {
 Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, std::vector<uint64_t> &sizes) {
   EncWorkspace &ws = enc_workspace(L);
   Err e;
-  const uint64_t G = b.slices.size();
+  const uint64_t G = b.n_slices;
+  if (G > 0xFFFFFFF0ull) return invalid("a batch of more than 2^32 slices");
   const uint32_t NP = (uint32_t)b.pics.size(), NF = (uint32_t)b.frames.size();
-  if (!(e = upload(L, ws.frames, b.frames)).ok()) return e;
-  if (!(e = upload(L, ws.pics, b.pics)).ok()) return e;
-  if (!(e = upload(L, ws.slices, b.slices)).ok()) return e;
-  if (!(e = upload(L, ws.states, b.states)).ok()) return e;
-  if (!(e = upload(L, ws.qtabs, b.qtabs)).ok()) return e;
-  if (!(e = upload(L, ws.hdrs, b.hdrs)).ok()) return e;
-  // A chunk of 64 coefficients of padding: the size count loads the next
-  // chunk of a slice while it counts the current one.
+  // The plan's parts at 256-byte offsets of one buffer.
+  size_t at = 0;
+  auto part = [&](size_t bytes) {
+    const size_t o = at;
+    at += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_frames = part(b.frames.size() * sizeof(EncFrame)), o_pics = part(b.pics.size() * sizeof(EncPic)),
+               o_slots = part(b.slots.size() * sizeof(EncRowSlot)),
+               o_states = part(b.states.size() * sizeof(EncState)),
+               o_hdrs = part(b.hdrs.size()), plan_bytes = at;
+  // The batch's quantiser tables in the lane's device cache: keys not there
+  // get the next slots (the arena grows by clearing the cache: every table
+  // of the batch is then uploaded again) and go up with the plan.
+  std::vector<uint32_t> slot_of(b.qkeys.size());
+  std::vector<size_t> fresh;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    fresh.clear();
+    uint32_t next = (uint32_t)ws.qslot.size();
+    for (size_t k = 0; k < b.qkeys.size(); k++) {
+      auto it = ws.qslot.find(b.qkeys[k]);
+      if (it != ws.qslot.end()) {
+        slot_of[k] = it->second;
+      } else {
+        slot_of[k] = next++;
+        fresh.push_back(k);
+      }
+    }
+    if ((size_t)next * QTAB_FLOATS * 4 <= ws.qarena.cap) break;
+    ws.qslot.clear();
+    if (!(e = ws.qarena.reserve((size_t)std::max<uint32_t>(next, 32) * QTAB_FLOATS * 4)).ok()) return e;
+  }
+  const uint32_t first_fresh = (uint32_t)ws.qslot.size();
+  for (size_t k : fresh) ws.qslot[b.qkeys[k]] = slot_of[k];
+  for (EncPic &P : b.pics) P.qtab = slot_of[P.qtab];
+  const size_t o_fresh = part(fresh.size() * QTAB_FLOATS * 4), staged = at;
+  if (!(e = ws.plan.reserve(plan_bytes + 64)).ok() || !(e = ws.hplan.reserve(staged + 64)).ok()) return e;
+  {
+    uint8_t *h = (uint8_t *)ws.hplan.p;
+    auto put = [&](size_t o, const void *src, size_t n) {
+      if (n) memcpy(h + o, src, n);
+    };
+    put(o_frames, b.frames.data(), b.frames.size() * sizeof(EncFrame));
+    put(o_pics, b.pics.data(), b.pics.size() * sizeof(EncPic));
+    put(o_slots, b.slots.data(), b.slots.size() * sizeof(EncRowSlot));
+    put(o_states, b.states.data(), b.states.size() * sizeof(EncState));
+    put(o_hdrs, b.hdrs.data(), b.hdrs.size());
+    for (size_t i = 0; i < fresh.size(); i++) quantiser_values(b.qkeys[fresh[i]], (float *)(h + o_fresh) + i * QTAB_FLOATS);
+    if (!(e = check(hipMemcpyAsync(ws.plan.p, ws.hplan.p, plan_bytes, hipMemcpyHostToDevice, L.stream),
+                    "hipMemcpyAsync(encode plan)")).ok())
+      return e;
+    // New slots are consecutive from first_fresh.
+    if (!fresh.empty() &&
+        !(e = check(hipMemcpyAsync((uint8_t *)ws.qarena.p + (size_t)first_fresh * QTAB_FLOATS * 4, h + o_fresh,
+                                   fresh.size() * QTAB_FLOATS * 4, hipMemcpyHostToDevice, L.stream),
+                    "hipMemcpyAsync(quantiser tables)")).ok())
+      return e;
+  }
+  if (!(e = ws.slices.reserve(G * sizeof(EncSlice) + 64)).ok()) return e;
   if (!(e = ws.coefs.reserve(b.n_coefs * 4 + 512)).ok()) return e;
   if (!(e = ws.alpha_scratch.reserve(G * ALPHA_SLICE_CAP + 64)).ok()) return e;
   if (!(e = ws.alpha_len.reserve(G * 4 + 64)).ok()) return e;
@@ -282,14 +367,21 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   if (!(e = ws.qidx.reserve(G * 4 + 64)).ok()) return e;
   if (!(e = ws.slice_out.reserve(G * 8 + 64)).ok()) return e;
   if (!(e = ws.frame_size.reserve(NF * 8 + 64)).ok()) return e;
-  void *frames = ws.frames.p, *pics = ws.pics.p, *slices = ws.slices.p, *states = ws.states.p;
-  void *qtabs = ws.qtabs.p, *hdrs = ws.hdrs.p, *coefs = ws.coefs.p, *ascr = ws.alpha_scratch.p;
+  if (!(e = ws.out_addr.reserve(NF * 8 + 64)).ok()) return e;
+  uint8_t *const pl = (uint8_t *)ws.plan.p;
+  void *frames = pl + o_frames, *pics = pl + o_pics, *slots = pl + o_slots, *states = pl + o_states;
+  void *slices = ws.slices.p;
+  void *qtabs = ws.qarena.p, *hdrs = pl + o_hdrs, *coefs = ws.coefs.p, *ascr = ws.alpha_scratch.p;
   void *alen = ws.alpha_len.p, *stab = ws.size_tab.p, *used = ws.used.p, *qidx = ws.qidx.p;
   void *sout = ws.slice_out.p, *fsize = ws.frame_size.p, *tables = d.tables, *deep = ws.deep.p;
   void *psz = ws.probe_sizes.p, *cbnd = ws.chunk_bounds.p;
   uint32_t g32 = (uint32_t)G, np32 = NP, window = WINDOW;
   hipEvent_t *ev = L.eev;
   (void)hipEventRecord(ev[0], L.stream);
+  {
+    void *args[] = {&frames, &pics, &slots, &slices};
+    if (!(e = launch(L, d.k_slices, (b.max_pic_slices + 255) / 256, np32, 256, args)).ok()) return e;
+  }
   {
     void *args[] = {&tables, &frames, &pics, &slices, &qtabs, &coefs, &cbnd};
     if (!(e = launch(L, d.k_fdct, g32, 1, 256, args)).ok()) return e;
@@ -325,11 +417,36 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
     if (!(e = launch(L, d.k_offsets, NF, 1, 256, args)).ok()) return e;
   }
   (void)hipEventRecord(ev[6], L.stream);
-  sizes.assign(NF, 0);
-  if (NF && !(e = check(hipMemcpyAsync(sizes.data(), ws.frame_size.p, NF * 8, hipMemcpyDeviceToHost, L.stream),
-                        "frame sizes")).ok())
+  // The packets' places in the output arena are found on the device; the
+  // arena holds every frame at its target (a frame over it is one that does
+  // not fit at qi 224) or the largest batch seen, else the pack runs again
+  // over a larger one.
+  uint64_t want = 0;
+  for (uint32_t f = 0; f < NF; f++) want += (b.targets[f] + 255) & ~255ull;
+  if (!(e = ws.out.reserve(want + 256)).ok()) return e;  // grow-only: keeps the largest batch's arena
+  void *oaddr = ws.out_addr.p, *oflow = (uint8_t *)ws.out_addr.p + NF * 8;
+  if (!(e = ws.hres.reserve(NF * 8 + 64)).ok()) return e;
+  (void)hipEventRecord(ev[7], L.stream);
+  auto pack = [&](uint64_t capacity) -> Err {
+    Err x;
+    uint64_t base = (uint64_t)ws.out.p;
+    uint32_t nf = NF;
+    void *aargs[] = {&fsize, &nf, &base, &capacity, &oaddr, &oflow};
+    if (!(x = launch(L, d.k_arena, 1, 1, 256, aargs)).ok()) return x;
+    void *args[] = {&frames, &pics,  &slices, &coefs, &qtabs, &alen,  &ascr,
+                    &used,   &qidx,  &sout,   &fsize, &hdrs,  &oaddr, &oflow};
+    if (!(x = launch(L, d.k_pack, g32, 1, 64, args)).ok()) return x;
+    return check(hipMemcpyAsync(ws.hres.p, fsize, NF * 8, hipMemcpyDeviceToHost, L.stream), "frame sizes");
+  };
+  if (!(e = pack(ws.out.cap - 256)).ok()) return e;
+  uint8_t *const hres = (uint8_t *)ws.hres.p;
+  if (!(e = check(hipMemcpyAsync(hres + NF * 8, oflow, 4, hipMemcpyDeviceToHost, L.stream), "arena flag")).ok())
     return e;
+  (void)hipEventRecord(ev[8], L.stream);
   if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
+  uint32_t overflow = 0;
+  memcpy(&overflow, hres + NF * 8, 4);
+  sizes.assign((const uint64_t *)hres, (const uint64_t *)hres + NF);
   offsets.assign(NF, 0);
   uint64_t total = 0;
   for (uint32_t f = 0; f < NF; f++) {
@@ -337,20 +454,16 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
     offsets[f] = total;
     total += (sizes[f] + 255) & ~255ull;
   }
-  if (!(e = ws.out.reserve(total + 256)).ok()) return e;
-  for (uint32_t f = 0; f < NF; f++) b.frames[f].out = (uint64_t)ws.out.p + offsets[f];
-  if (!(e = upload(L, ws.frames, b.frames)).ok()) return e;
-  (void)hipEventRecord(ev[7], L.stream);
-  {
-    void *args[] = {&frames, &pics, &slices, &coefs, &qtabs, &alen, &ascr, &used, &qidx, &sout, &fsize, &hdrs};
-    if (!(e = launch(L, d.k_pack, g32, 1, 64, args)).ok()) return e;
+  if (overflow) {
+    if (!(e = ws.out.reserve(total + 256)).ok()) return e;
+    if (!(e = pack(ws.out.cap - 256)).ok()) return e;
+    (void)hipEventRecord(ev[8], L.stream);
+    if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
   }
-  (void)hipEventRecord(ev[8], L.stream);
-  if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
   {
     // Diagnostics: sizes the share-out computed past the window.
     std::vector<EncState> st(NP);
-    if (NP && copy_on(L, st.data(), ws.states.p, NP * sizeof(EncState), hipMemcpyDeviceToHost).ok()) {
+    if (NP && copy_on(L, st.data(), states, NP * sizeof(EncState), hipMemcpyDeviceToHost).ok()) {
       uint64_t total = 0, most = 0, uni = 0;
       for (const EncState &x : st) {
         total += x.fallback;

@@ -1,5 +1,7 @@
 // gfx950 encode kernels for the ProRes slice engine (src/encode.rs).
 //
+//   pe_slices    one thread per slice: the slice records from each
+//                picture's row layout.
 //   pe_fdct      one workgroup per slice: edge-replicated 8x8 gathers, the
 //                f32 forward DCT in the reference's order (src/dct.rs:70-93,
 //                src/dsp/scalar.rs:14-17), scattered to scanned-interleaved
@@ -20,6 +22,7 @@
 //                wave); sizes past the table are coded by all the waves, four
 //                quantisers a request, one per wave, and cached.
 //   pe_offsets   one workgroup per frame: slice and picture offsets.
+//   pe_arena     one workgroup: each frame's place in the output arena.
 //   pe_pack      one wavefront per slice: quantise at the chosen qi and place
 //                every codeword with a wave prefix sum over their lengths.
 //
@@ -287,6 +290,29 @@ __device__ __forceinline__ uint16_t rescale(uint32_t a, uint32_t identity, uint6
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// The slice records, one thread a slice, blockIdx.y the picture: each
+// picture's rows of slices laid out as its row slots, coefficients
+// contiguous in row-major order.
+extern "C" __global__ void __launch_bounds__(256)
+    pe_slices(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
+              const EncRowSlot *__restrict__ slots, EncSlice *__restrict__ slices) {
+  const EncPic &P = pics[blockIdx.y];
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= P.n_slices) return;
+  const uint32_t row = k / P.row_slices, j = k - row * P.row_slices;
+  const EncRowSlot rs = slots[P.layout_base + j];
+  const uint32_t per_mb = 64 * (frames[P.frame].chroma == 2 ? 8u : 12u);
+  EncSlice S;
+  S.coef = P.coef_base + ((uint64_t)row * P.row_mbs + rs.mbs_before) * per_mb;
+  S.pic = blockIdx.y;
+  S.mb_x = rs.mb_x;
+  S.mb_y = (uint16_t)row;
+  S.mbs = rs.mbs;
+  S.mbs_from = (P.rows - row - 1) * P.row_mbs + (P.row_mbs - rs.mbs_before);
+  slices[P.slice_base + k] = S;
+}
 
 // ---------------------------------------------------------------------------
 extern "C" __global__ void __launch_bounds__(256)
@@ -854,6 +880,35 @@ extern "C" __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
+// One workgroup: each frame's place in the output arena, its packet at a
+// 256-byte boundary after the previous frame's, and whether the packets fit
+// the arena's capacity (if not, pe_pack writes nothing and the host runs it
+// again over a larger arena).
+extern "C" __global__ void __launch_bounds__(256)
+    pe_arena(const uint64_t *__restrict__ frame_size, uint32_t n_frames, uint64_t base, uint64_t capacity,
+             uint64_t *__restrict__ out_addr, uint32_t *__restrict__ overflow) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ uint64_t wsum[4];
+  uint64_t carry = 0;
+  for (uint32_t b = 0; b < n_frames; b += 256) {
+    const uint32_t k = b + tid;
+    const uint64_t v = k < n_frames ? (frame_size[k] + 255) & ~255ull : 0ull;
+    uint64_t incl = v;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      const uint64_t o = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += o;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint64_t before = 0;
+    for (uint32_t w = 0; w < wave; w++) before += wsum[w];
+    if (k < n_frames) out_addr[k] = base + carry + before + incl - v;
+    carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+  if (tid == 0) *overflow = carry > capacity ? 1u : 0u;
+}
+
 namespace {
 constexpr uint32_t RING = 512;  // words of the per-slice bit ring
 
@@ -887,13 +942,15 @@ extern "C" __global__ void __launch_bounds__(64)
             const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
             const uint8_t *__restrict__ alpha_scratch, const uint32_t *__restrict__ used,
             const uint32_t *__restrict__ qidx, const uint64_t *__restrict__ slice_out,
-            const uint64_t *__restrict__ frame_size, const uint8_t *__restrict__ hdr_table) {
+            const uint64_t *__restrict__ frame_size, const uint8_t *__restrict__ hdr_table,
+            const uint64_t *__restrict__ out_addr, const uint32_t *__restrict__ overflow) {
   __shared__ uint32_t ring[RING];
+  if (*overflow) return;  // the arena is too small: the host grows it and packs again
   const uint32_t g = blockIdx.x, L = threadIdx.x;
   const EncSlice S = slices[g];
   const EncPic &P = pics[S.pic];
   const EncFrame &F = frames[P.frame];
-  GLOBAL uint8_t *out = (GLOBAL uint8_t *)(uintptr_t)F.out;
+  GLOBAL uint8_t *out = (GLOBAL uint8_t *)(uintptr_t)out_addr[P.frame];
   const uint32_t k = g - P.slice_base;
   const uint32_t u = used[g], qi = qidx[g], hb = P.header_bytes;
   if (L == 0) {

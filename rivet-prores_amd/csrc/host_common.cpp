@@ -321,6 +321,18 @@ Err copy_on(Lane &L, void *dst, const void *src, size_t bytes, hipMemcpyKind kin
   return e.ok() ? s : e;
 }
 
+Err HostBuf::reserve(size_t bytes) {
+  if (bytes <= cap) return Err{};
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+  const size_t want = bytes + bytes / 4 + 4096;
+  const hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+  if (e != hipSuccess) return device_error("hipHostMalloc", e);
+  cap = want;
+  return Err{};
+}
+
 Err launch_on(hipStream_t s, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args,
               unsigned lds) {
   if (gx == 0 || gy == 0) return Err{};
@@ -395,11 +407,11 @@ Err ensure_encoder(Device &d) {
   if (d.enc_mod) return Err{};
   Err e = check(hipModuleLoadData(&d.enc_mod, prores_encode_co), "hipModuleLoadData(encode)");
   if (!e.ok()) return e;
-  const char *const names[] = {"pe_fdct",     "pe_alpha",   "pe_size", "pe_search",
-                               "pe_window",   "pe_shareout", "pe_offsets", "pe_pack"};
-  hipFunction_t *const fns[] = {&d.k_fdct,   &d.k_alpha_enc, &d.k_size,    &d.k_search,
-                                &d.k_window, &d.k_shareout,  &d.k_offsets, &d.k_pack};
-  e = load_functions(d.enc_mod, names, fns, 8);
+  const char *const names[] = {"pe_fdct",     "pe_alpha",   "pe_size", "pe_search", "pe_window",
+                               "pe_shareout", "pe_offsets", "pe_pack", "pe_arena", "pe_slices"};
+  hipFunction_t *const fns[] = {&d.k_fdct,     &d.k_alpha_enc, &d.k_size, &d.k_search, &d.k_window,
+                                &d.k_shareout, &d.k_offsets,   &d.k_pack, &d.k_arena,  &d.k_slices};
+  e = load_functions(d.enc_mod, names, fns, 10);
   if (!e.ok()) {
     (void)hipModuleUnload(d.enc_mod);
     d.enc_mod = nullptr;

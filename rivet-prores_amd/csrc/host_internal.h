@@ -102,7 +102,7 @@ struct Device {
                 k_headers = nullptr, k_order = nullptr;
   hipFunction_t k_fdct = nullptr, k_alpha_enc = nullptr, k_size = nullptr, k_search = nullptr,
                 k_shareout = nullptr, k_offsets = nullptr, k_pack = nullptr,
-                k_window = nullptr;
+                k_window = nullptr, k_arena = nullptr, k_slices = nullptr;
   DevTables *tables = nullptr;
   std::mutex mu;  // lane creation, encoder loading, the batch tuning
   Lane *lanes[MAX_LANES] = {nullptr};
@@ -125,6 +125,16 @@ struct DevBuf {
   T *as() const {
     return (T *)p;
   }
+};
+
+// Page-locked host memory, grow-only: a lane's staging for a batch plan
+// going up and its results coming back, so that each is one asynchronous
+// copy. The caller holds the lane's lock and has waited for the last copy
+// out of it before it grows or refills.
+struct HostBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  Err reserve(size_t bytes);
 };
 
 // `lds`: bytes of dynamic LDS per workgroup.

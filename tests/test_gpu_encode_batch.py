@@ -94,3 +94,49 @@ def test_mixed_encode_batch():
         ptr, n = r
         got = P.download_ptr(ptr, n).tobytes()
         assert got == w, (i, len(got), len(w), next((k for k in range(min(len(got), len(w))) if got[k] != w[k]), None))
+
+
+_FRESH = r"""
+import sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import prores_amd as P
+data = np.fromfile(sys.argv[2], dtype=np.uint16)
+f = P.Frame(1920, 1080, 2, 10, data)
+items = []
+bufs = []
+for target in (64, 64, 200):
+    dd = P.DeviceBuffer(data.nbytes + 64)
+    dd.upload(data)
+    bufs.append(dd)
+    items.append((P.Config(P.Profile.Hq, target_frame_bytes=target or None), f, dd.ptr, 0))
+out = P.encode_device_batch(items)
+for r in out:
+    if isinstance(r, Exception):
+        raise r
+    sys.stdout.write(P.download_ptr(*r).tobytes().hex() + "\n")
+"""
+
+
+def test_output_arena_grows_in_a_fresh_process(tmp_path):
+    """A first batch whose packets outgrow the output arena's first guess
+    (frames whose targets are far below what qi 224 takes) is packed again
+    over a larger arena; its packets equal the oracle's. It runs in a fresh
+    process, where the arena starts at the batch's guess."""
+    import os
+    import subprocess
+    import sys
+
+    f = O.test_frame(1920, 1080, 2, 10, 12, 31)
+    path = tmp_path / "frame.u16"
+    f.data.tofile(path)
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "rivet-prores_amd")
+    print("case: fresh-process encode with targets 64, 64, 200", flush=True)
+    r = subprocess.run([sys.executable, "-c", _FRESH, pkg, str(path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = [bytes.fromhex(x) for x in r.stdout.split()]
+    want = [O.encode(f, O.config(3, target=t)) for t in (64, 64, 200)]
+    # qi 224 does not fit these targets: the packets take more than twice the
+    # arena's first guess (each target plus 4 KiB, and a quarter more).
+    assert sum(len(w) for w in want) > 2 * 3 * (200 + 4096)
+    assert got == want
