@@ -173,9 +173,13 @@ int32_t prores_profile_from_fourcc(const char code[4]); /* -1 if none */
  * No counterpart in the reference (it runs on CPU threads). For callers that
  * keep bitstreams and planes in GPU memory (the batched path, the serving
  * loop, the benchmark): packets and outputs are device pointers. The host
- * copy of a packet, where the caller has one, is read for its headers only;
- * without one (host_packet NULL) the library gathers the header bytes on the
- * device (one small launch and a download of 192 bytes a packet).
+ * copy of a packet, where the caller has one, is read for its headers only.
+ * A batch with a job without one (host_packet NULL) is planned on the
+ * device: the frame and picture headers (src/header.rs:82-290) are parsed
+ * and the batch laid out by a kernel, so the launch sequence starts with no
+ * host round trip, and header errors come back with the results, the
+ * reference's messages. The device plan is laid out for the largest batch
+ * the library has seen; a larger one is run again over a grown workspace.
  * A job without dev_packet or dev_data, or without dev_alpha for a frame
  * that has alpha, fails with PRORES_ERR_CONFIG and decodes nothing.
  */

@@ -64,22 +64,31 @@ struct Batch {
 // arguments are workspace addresses and launch shapes, so it replays while
 // those stay the same.
 struct GraphKey {
-  void *ptrs[17];
+  void *ptrs[22];
   unsigned grids[4];
-  unsigned ewg, idct_lds;
+  unsigned ewg, idct_lds, n_jobs, idct_cap;
   prores_decode_tuning t;
-  bool alpha;
+  PlanCaps caps;
+  bool alpha, device_plan;
   bool operator==(const GraphKey &o) const { return memcmp(this, &o, sizeof *this) == 0; }
 };
 
 struct Workspace {
-  DevBuf slices, slice_err, coefs, wide, packet, planes, items, alpha_order, gather, records, order;
-  // The batch plan (frames, pictures, row slots, IDCT segments, header) in
-  // one device buffer, each part at a fixed offset while it fits its
-  // capacity; the per-picture status and first-error words in another.
+  DevBuf slices, slice_err, coefs, wide, packet, planes, items, alpha_order, order;
+  // The batch plan (frames, pictures, row slots, IDCT segments, slice-table
+  // chunks, entropy item costs, header) in one device buffer, each part at a
+  // fixed offset while it fits its capacity; the per-picture status and
+  // first-error words in another.
   DevBuf plan, picres;
   HostBuf hplan, hres;
   size_t cap_frames = 0, cap_pics = 0, cap_rows = 0, cap_segs = 0, cap_chunks = 0, cap_cost = 0, cap_np = 0;
+  // Device-planned batches: the jobs, pd_plan's scratch, and its results
+  // (job records, totals, picture status and first-error words) in one
+  // buffer that comes back in one copy; the capacities it plans into (the
+  // largest batch seen, grow-only).
+  DevBuf jobs, scratch, dres;
+  PlanCaps dcap{};
+  uint32_t idct_cap_slices = IDCT_MAX_SLICES;
   hipGraphExec_t graph = nullptr;
   GraphKey key;
   bool graph_failed = false;    // the capture for `key` failed
@@ -91,35 +100,17 @@ Workspace &workspace(Lane &L) {
   return *(Workspace *)L.dec;
 }
 
-// Where plan_frame reads a frame's headers: the whole packet in host memory,
-// or the HDR_BLOB record pd_headers gathered from a packet that is only in
-// device memory.
-struct HeaderSource {
-  const uint8_t *host = nullptr;
-  const uint8_t *record = nullptr;
-  const uint8_t *frame() const { return host ? host : record; }
-  // The picture header at byte `pos` of the packet (its first HDR_PIC_BYTES
-  // bytes are readable), or null if the record holds no picture there.
-  const uint8_t *picture(size_t pos) const {
-    if (host) return host + pos;
-    uint64_t at[2];
-    memcpy(at, record + HDR_POS_AT, sizeof at);
-    for (uint32_t i = 0; i < 2; i++)
-      if (at[i] == pos) return record + HDR_PIC_AT + i * HDR_PIC_BYTES;
-    return nullptr;
-  }
-};
-
-// Lays out one frame's pictures. `dev_packet` is the device address of the
+// Lays out one frame's pictures from its packet in host memory (`host`, or
+// the header `parsed` already). `dev_packet` is the device address of the
 // packet; `dev_data` / `dev_alpha` the device output planes.
-void plan_frame(const HeaderSource &src, size_t len, uint64_t dev_packet, uint32_t want_depth,
+void plan_frame(const uint8_t *host, size_t len, uint64_t dev_packet, uint32_t want_depth,
                 uint16_t *dev_data, uint16_t *dev_alpha, Batch &b, const Header *parsed) {
   FramePlan fp;
   Header hd;
   if (parsed) {
     hd = *parsed;
   } else {
-    fp.early = parse_frame_header(src.frame(), len, hd);
+    fp.early = parse_frame_header(host, len, hd);
     if (fp.early.ok()) fp.early = check_plausible_size(hd);
   }
   // The device writes through these: a job without them decodes nothing.
@@ -173,9 +164,7 @@ void plan_frame(const HeaderSource &src, size_t len, uint64_t dev_packet, uint32
       break;
     }
     PicHeader ph;
-    const uint8_t *at = src.picture(pos);
-    Err e = at ? parse_picture_header(at, h.frame_size - pos, ph)
-               : Err{PRORES_ERR_DEVICE, "the gathered header bytes hold no picture header here"};
+    Err e = parse_picture_header(host + pos, h.frame_size - pos, ph);
     if (!e.ok()) {
       fp.deferred = e;
       break;
@@ -215,14 +204,6 @@ void plan_frame(const HeaderSource &src, size_t len, uint64_t dev_packet, uint32
     pos += ph.picture_size;
   }
   b.plans.push_back(fp);
-}
-
-template <class T>
-Err upload(Lane &L, DevBuf &buf, const std::vector<T> &v) {
-  Err e = buf.reserve(v.size() * sizeof(T) + 16);
-  if (!e.ok() || v.empty()) return e;
-  return check(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, L.stream),
-               "hipMemcpyAsync(plan)");
 }
 
 // Workgroups of kernel `f` that fit on the device at once (persistent grids).
@@ -286,8 +267,229 @@ Err validate_tuning(const prores_decode_tuning &t) {
   return Err{};
 }
 
-// Runs the kernels over a planned batch and turns the device's error records
-// into one Result per frame (`results`, in frame order).
+// The device addresses and launch shapes of one batch's launch sequence:
+// pd_plan first for a device-planned batch, then the slice table, the entropy
+// order, the colour entropy decode, the alpha decode on the second stream
+// and the IDCT.
+struct Launch {
+  void *B = nullptr, *frames = nullptr, *pics = nullptr, *rows = nullptr, *chunks = nullptr, *slices = nullptr;
+  void *status = nullptr, *serr = nullptr, *ferr = nullptr, *segs = nullptr, *items = nullptr, *aord = nullptr;
+  void *cost = nullptr, *tables = nullptr, *coefs = nullptr, *wide = nullptr, *order = nullptr;
+  unsigned grids[4] = {0, 0, 0, 0};
+  unsigned ewg = 64, idct_lds = 0;
+  uint32_t pers = 0, n_slots = 0;
+  bool alpha = false;
+  // pd_plan's arguments (device_plan only).
+  bool device_plan = false;
+  void *jobs = nullptr, *scratch = nullptr, *recs = nullptr, *totals = nullptr;
+  uint32_t n_jobs = 0, want_depth = 0, idct_cap = 0, chunk = 1;
+  PlanCaps caps{};
+};
+
+Err enqueue_sequence(Device &d, Lane &L, Launch &A, bool timed) {
+  Err x;
+  const unsigned ewaves = A.ewg / 64;
+  if (timed) (void)hipEventRecord(L.ev[0], L.stream);
+  if (A.device_plan) {
+    void *args[] = {&A.jobs, &A.n_jobs, &A.want_depth, &A.idct_cap, &A.chunk, &A.caps, &A.scratch, &A.recs,
+                    &A.totals, &A.frames, &A.pics, &A.rows, &A.segs, &A.chunks, &A.cost, &A.B};
+    if (!(x = launch(L, d.k_plan, 1, 1, 256, args)).ok()) return x;
+  }
+  {
+    void *args[] = {&A.B, &A.frames, &A.pics, &A.rows, &A.chunks, &A.slices, &A.status, &A.serr, &A.ferr, &A.segs,
+                    &A.items, &A.aord, &A.cost};
+    if (!(x = launch(L, d.k_slice_table, A.grids[0], 1, ST_CHUNK, args)).ok()) return x;
+  }
+  if (A.order) {
+    void *args[] = {&A.B, &A.cost, &A.order, &A.n_slots};
+    if (!(x = launch(L, d.k_order, 1, 1, 1024, args)).ok()) return x;
+  }
+  if (timed && !(x = check(hipEventRecord(L.ev[1], L.stream), "hipEventRecord")).ok()) return x;
+  // Alpha needs only the slice table: the second stream waits for the work
+  // so far and decodes alpha beside the colour entropy decode and the IDCT,
+  // which leave most of the machine's wave slots free. The entropy decode is
+  // launched first and its workgroups (most of a CU's LDS each) are placed
+  // before the alpha workgroups.
+  bool forked = false, joined = false;
+  if (A.alpha) {
+    // A timed batch forks at its stage event: one marker less between the
+    // order and entropy launches.
+    hipEvent_t fork = timed ? L.ev[1] : L.fork;
+    if (!timed && !(x = check(hipEventRecord(fork, L.stream), "hipEventRecord")).ok()) return x;
+    if (!(x = check(hipStreamWaitEvent(L.aux, fork, 0), "hipStreamWaitEvent")).ok()) return x;
+    forked = true;
+  }
+  {
+    void *args[] = {&A.tables, &A.B, &A.frames, &A.pics, &A.slices, &A.status, &A.coefs, &A.wide, &A.serr, &A.ferr,
+                    &A.order, &A.pers};
+    x = launch(L, d.k_entropy, A.grids[1], 1, A.ewg, args, ewaves * ENTROPY_LDS_PER_WAVE);
+  }
+  if (x.ok() && forked) {
+    void *args[] = {&A.B, &A.frames, &A.pics, &A.slices, &A.status, &A.serr, &A.ferr, &A.aord, &A.pers};
+    const Err a = launch_on(L.aux, d.k_alpha, A.grids[2], 1, 64, args);
+    if (timed) (void)hipEventRecord(L.ev[5], L.aux);
+    Err j = check(hipEventRecord(L.join, L.aux), "hipEventRecord");
+    if (!j.ok()) return j;  // the stream joins nothing: fail before more work
+    joined = true;
+    x = a;
+  }
+  if (x.ok()) {
+    if (timed) (void)hipEventRecord(L.ev[2], L.stream);
+    void *args[] = {&A.tables, &A.B, &A.frames, &A.pics, &A.slices, &A.status, &A.serr, &A.coefs, &A.wide, &A.items,
+                    &A.pers};
+    x = launch(L, d.k_idct, A.grids[3], 1, 256, args, A.idct_lds);
+    if (timed) (void)hipEventRecord(L.ev[3], L.stream);
+  }
+  // Whatever happened, the main stream waits for the alpha work before
+  // anything else on it can reuse the workspace.
+  if (joined) {
+    Err j = check(hipStreamWaitEvent(L.stream, L.join, 0), "hipStreamWaitEvent");
+    if (x.ok()) x = j;
+  }
+  return x;
+}
+
+// Launches the sequence (replaying a captured graph when the tuning asks
+// for one), copies `res_bytes` of results from `res` into the lane's
+// page-locked result buffer, waits, and records the stage times.
+Err run_sequence(Device &d, Lane &L, Launch &A, const prores_decode_tuning &t, const void *res, size_t res_bytes) {
+  Workspace &ws = workspace(L);
+  Err e;
+  // A capture that fails (another thread's work invalidated it, or the
+  // runtime refused a node) leaves the stream out of capture mode with
+  // nothing launched: the batch then runs as stream launches.
+  bool replay = t.graph != 0;
+  if (replay) {
+    GraphKey key;
+    memset(&key, 0, sizeof key);
+    void *ptrs[22] = {A.B,    A.pics,  A.rows,  A.slices, A.status, A.frames, A.tables, A.coefs,
+                      A.wide, A.serr,  A.ferr,  A.items,  A.segs,   A.aord,   A.cost,   A.order,
+                      A.chunks, A.jobs, A.scratch, A.recs, A.totals, nullptr};
+    memcpy(key.ptrs, ptrs, sizeof ptrs);
+    memcpy(key.grids, A.grids, sizeof A.grids);
+    key.ewg = A.ewg;
+    key.idct_lds = A.idct_lds;
+    key.n_jobs = A.n_jobs;
+    key.idct_cap = A.idct_cap;
+    key.t = t;
+    key.caps = A.caps;
+    key.alpha = A.alpha;
+    key.device_plan = A.device_plan;
+    // Captured once per key; a key whose capture failed runs as stream
+    // launches without trying again.
+    if (!(key == ws.key) || (!ws.graph && !ws.graph_failed)) {
+      if (ws.graph) {
+        (void)hipGraphExecDestroy(ws.graph);
+        ws.graph = nullptr;
+      }
+      hipGraph_t graph = nullptr;
+      bool captured = false;
+      if (hipStreamBeginCapture(L.stream, hipStreamCaptureModeRelaxed) == hipSuccess) {
+        const Err ce = enqueue_sequence(d, L, A, false);
+        const hipError_t ee = hipStreamEndCapture(L.stream, &graph);
+        captured = ce.ok() && ee == hipSuccess && graph;
+      }
+      if (captured && hipGraphInstantiate(&ws.graph, graph, nullptr, nullptr, 0) != hipSuccess) {
+        ws.graph = nullptr;
+        captured = false;
+      }
+      if (graph) (void)hipGraphDestroy(graph);
+      if (!captured) (void)hipGetLastError();  // the failed capture's error is not this batch's
+      ws.key = key;
+      ws.graph_failed = !captured;
+      ws.graph_captures += captured;
+      ws.graph_failures += !captured;
+    }
+    replay = ws.graph != nullptr;
+  }
+  if (replay) {
+    (void)hipEventRecord(L.ev[0], L.stream);
+    if (!(e = check(hipGraphLaunch(ws.graph, L.stream), "hipGraphLaunch")).ok()) return e;
+    ws.graph_replays++;
+    (void)hipEventRecord(L.ev[4], L.stream);
+  } else {
+    if (!(e = enqueue_sequence(d, L, A, true)).ok()) return e;
+    (void)hipEventRecord(L.ev[4], L.stream);
+  }
+  if (!(e = ws.hres.reserve(res_bytes + 64)).ok()) return e;
+  if (res_bytes && !(e = check(hipMemcpyAsync(ws.hres.p, res, res_bytes, hipMemcpyDeviceToHost, L.stream),
+                               "hipMemcpyAsync(results)")).ok())
+    return e;
+  if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
+  // Stages: slice table, colour entropy, alpha (from the end of the slice
+  // table to its own end, beside the other two), IDCT, whole batch.
+  float ms[4] = {0, 0, 0, 0}, all = 0;
+  if (!replay) {
+    (void)hipEventElapsedTime(&ms[0], L.ev[0], L.ev[1]);
+    (void)hipEventElapsedTime(&ms[1], L.ev[1], L.ev[2]);
+    if (A.alpha) (void)hipEventElapsedTime(&ms[2], L.ev[1], L.ev[5]);
+    (void)hipEventElapsedTime(&ms[3], L.ev[2], L.ev[3]);
+  }
+  (void)hipEventElapsedTime(&all, L.ev[0], L.ev[4]);
+  for (int i = 0; i < 4; i++) L.dec_us[i] = ms[i] * 1000.0;
+  L.dec_us[4] = all * 1000.0;
+  return Err{};
+}
+
+// Grids, workgroup size and LDS of a launch for batches of up to `slices`
+// slices, `chunks` slice-table chunks and `idct_items` IDCT items.
+void launch_shape(Device &d, const prores_decode_tuning &t, uint64_t slices, uint64_t chunks, uint64_t idct_items,
+                  uint32_t max_count, Launch &A) {
+  const uint32_t entropy_items = (uint32_t)(3 * ((slices + 63) / 64));
+  A.idct_lds = max_count * 128 * (unsigned)sizeof(float);
+  A.ewg = entropy_wg(d, t, entropy_items);
+  const unsigned ewaves = A.ewg / 64;
+  A.pers = t.persistent;
+  if (t.persistent) {
+    A.grids[0] = resident_grid(d, d.k_slice_table, ST_CHUNK);
+    A.grids[1] = resident_grid(d, d.k_entropy, A.ewg, ewaves * ENTROPY_LDS_PER_WAVE);
+    A.grids[2] = resident_grid(d, d.k_alpha, 64);
+    A.grids[3] = resident_grid(d, d.k_idct, 256, A.idct_lds);
+  } else {
+    A.grids[0] = (unsigned)chunks;
+    A.grids[1] = (unsigned)((entropy_items + ewaves - 1) / ewaves);
+    A.grids[2] = (unsigned)slices;
+    A.grids[3] = (unsigned)idct_items;
+  }
+  // Entropy items longest first (pd_entropy_order) when the sort fits: the
+  // grid's slots, or the queue's items (0: the kernel reads the count from
+  // the batch, so a captured graph replays for any batch of its shape).
+  A.n_slots = t.persistent ? 0u : A.grids[1] * ewaves;
+  A.order = nullptr;
+}
+
+// The picture results of a batch: the first error of each frame's good
+// pictures in picture order (a slice table past picture_size, else the
+// failing slice's first error word), then the frame's deferred error.
+Err picture_result(Lane &L, Workspace &ws, const uint32_t *st, const uint32_t *fe, uint32_t first_pic,
+                   uint32_t n_pics, const DecPic *host_pics, const void *dev_pics, Err deferred, Err &out) {
+  Err r, e;
+  for (uint32_t p = first_pic; p < first_pic + n_pics && r.ok(); p++) {
+    if (st[p] == PIC_TABLE_ERR) {
+      r = invalid("a slice runs past picture_size");
+    } else if (fe[p] != 0xFFFFFFFFu) {
+      DecPic P;
+      if (host_pics) {
+        P = host_pics[p];
+      } else if (!(e = copy_on(L, &P, (const DecPic *)dev_pics + p, sizeof P, hipMemcpyDeviceToHost)).ok()) {
+        return e;
+      }
+      uint32_t words[4];
+      const uint64_t g = P.slice_base + (uint64_t)fe[p];
+      if (!(e = copy_on(L, words, (uint8_t *)ws.slice_err.p + 16 * g, 16, hipMemcpyDeviceToHost)).ok()) return e;
+      for (int c = 0; c < 4; c++)
+        if (words[c]) {
+          r = slice_error(words[c]);
+          break;
+        }
+    }
+  }
+  out = r.ok() ? deferred : r;
+  return Err{};
+}
+
+// Runs the kernels over a batch the host planned and turns the device's
+// error records into one Result per frame (`results`, in frame order).
 Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::vector<Err> &results) {
   Workspace &ws = workspace(L);
   Err e;
@@ -301,7 +503,6 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   // largest item.
   uint32_t max_count = 1;
   for (const IdctItem &it : segs) max_count = it.count > max_count ? it.count : max_count;
-  const unsigned idct_lds = max_count * 128 * (unsigned)sizeof(float);
   DecBatch hdr;
   memset(&hdr, 0, sizeof hdr);
   hdr.n_pics = (uint32_t)np;
@@ -353,7 +554,6 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   }
   fit(ws.cap_np, np * 4);
   if (!(e = ws.picres.reserve(2 * ws.cap_np)).ok()) return e;
-  if (!(e = ws.hres.reserve(2 * ws.cap_np)).ok()) return e;
   if (!(e = ws.items.reserve(n_items * sizeof(IdctItem) + 64)).ok()) return e;
   if (!(e = ws.slices.reserve(G * sizeof(SliceRec) + 64)).ok()) return e;
   if (!(e = ws.slice_err.reserve(G * 16 + 64)).ok()) return e;
@@ -361,163 +561,37 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   if (!(e = ws.wide.reserve(b.n_coefs * 4 + 64)).ok()) return e;
   if (!(e = ws.alpha_order.reserve(G * sizeof(AlphaRec) + 64)).ok()) return e;
 
+  Launch A;
   uint8_t *const pl = (uint8_t *)ws.plan.p;
-  void *B = pl + o_hdr, *pics = pl + o_pics, *rows = pl + o_rows, *slices = ws.slices.p, *status = ws.picres.p;
-  void *frames = pl, *tables = d.tables, *coefs = ws.coefs.p, *wide = ws.wide.p;
-  void *serr = ws.slice_err.p, *ferr = (uint8_t *)ws.picres.p + ws.cap_np, *itemp = ws.items.p,
-       *segp = pl + o_segs, *chp = pl + o_chunks;
-  void *aord = ws.alpha_order.p, *costp = pl + o_cost;
-  uint32_t pers = t.persistent;
-  const unsigned ewg = entropy_wg(d, t, hdr.n_entropy_items), ewaves = ewg / 64;
-  unsigned grids[4];
-  if (t.persistent) {
-    grids[0] = resident_grid(d, d.k_slice_table, ST_CHUNK);
-    grids[1] = resident_grid(d, d.k_entropy, ewg, ewaves * ENTROPY_LDS_PER_WAVE);
-    grids[2] = resident_grid(d, d.k_alpha, 64);
-    grids[3] = resident_grid(d, d.k_idct, 256, idct_lds);
-  } else {
-    grids[0] = hdr.n_chunks;
-    grids[1] = (unsigned)((hdr.n_entropy_items + ewaves - 1) / ewaves);
-    grids[2] = (unsigned)G;
-    grids[3] = (unsigned)n_items;
-  }
-  // Entropy items longest first (pd_entropy_order) when the sort fits: the
-  // grid's slots, or the queue's items (0: the kernel reads the count from
-  // the batch, so a captured graph replays for any batch of its shape).
-  const uint32_t slots = t.persistent ? hdr.n_entropy_items : grids[1] * ewaves;
-  uint32_t n_slots = t.persistent ? 0u : slots;
-  void *ordp = nullptr;
+  A.B = pl + o_hdr;
+  A.frames = pl;
+  A.pics = pl + o_pics;
+  A.rows = pl + o_rows;
+  A.segs = pl + o_segs;
+  A.chunks = pl + o_chunks;
+  A.cost = pl + o_cost;
+  A.slices = ws.slices.p;
+  A.status = ws.picres.p;
+  A.ferr = (uint8_t *)ws.picres.p + ws.cap_np;
+  A.serr = ws.slice_err.p;
+  A.items = ws.items.p;
+  A.aord = ws.alpha_order.p;
+  A.tables = d.tables;
+  A.coefs = ws.coefs.p;
+  A.wide = ws.wide.p;
+  A.alpha = b.any_alpha;
+  launch_shape(d, t, G, hdr.n_chunks, n_items, max_count, A);
+  const uint32_t slots = t.persistent ? hdr.n_entropy_items : A.n_slots;
   if (slots <= ORDER_MAX) {
     if (!(e = ws.order.reserve((size_t)ORDER_MAX * 4 + 64)).ok()) return e;
-    ordp = ws.order.p;
-  }
-  auto enqueue = [&](bool timed) -> Err {
-    Err x;
-    if (timed) (void)hipEventRecord(L.ev[0], L.stream);
-    {
-      void *args[] = {&B, &frames, &pics, &rows, &chp, &slices, &status, &serr, &ferr, &segp, &itemp, &aord, &costp};
-      if (!(x = launch(L, d.k_slice_table, grids[0], 1, ST_CHUNK, args)).ok()) return x;
-    }
-    if (ordp) {
-      void *args[] = {&B, &costp, &ordp, &n_slots};
-      if (!(x = launch(L, d.k_order, 1, 1, 1024, args)).ok()) return x;
-    }
-    if (timed && !(x = check(hipEventRecord(L.ev[1], L.stream), "hipEventRecord")).ok()) return x;
-    // Alpha needs only the slice table: the second stream waits for the work
-    // so far and decodes alpha beside the colour entropy decode and the
-    // IDCT, which leave most of the machine's wave slots free. The entropy
-    // decode is launched first and its workgroups (most of a CU's LDS each)
-    // are placed before the alpha workgroups.
-    bool forked = false, joined = false;
-    if (b.any_alpha) {
-      // A timed batch forks at its stage event: one marker less between
-      // the order and entropy launches.
-      hipEvent_t fork = timed ? L.ev[1] : L.fork;
-      if (!timed && !(x = check(hipEventRecord(fork, L.stream), "hipEventRecord")).ok()) return x;
-      if (!(x = check(hipStreamWaitEvent(L.aux, fork, 0), "hipStreamWaitEvent")).ok()) return x;
-      forked = true;
-    }
-    {
-      void *args[] = {&tables, &B, &frames, &pics, &slices, &status, &coefs, &wide, &serr, &ferr, &ordp, &pers};
-      x = launch(L, d.k_entropy, grids[1], 1, ewg, args, ewaves * ENTROPY_LDS_PER_WAVE);
-    }
-    if (x.ok() && forked) {
-      void *args[] = {&B, &frames, &pics, &slices, &status, &serr, &ferr, &aord, &pers};
-      const Err a = launch_on(L.aux, d.k_alpha, grids[2], 1, 64, args);
-      if (timed) (void)hipEventRecord(L.ev[5], L.aux);
-      Err j = check(hipEventRecord(L.join, L.aux), "hipEventRecord");
-      if (!j.ok()) return j;  // the stream joins nothing: fail before more work
-      joined = true;
-      x = a;
-    }
-    if (x.ok()) {
-      if (timed) (void)hipEventRecord(L.ev[2], L.stream);
-      void *args[] = {&tables, &B, &frames, &pics, &slices, &status, &serr, &coefs, &wide, &itemp, &pers};
-      x = launch(L, d.k_idct, grids[3], 1, 256, args, idct_lds);
-      if (timed) (void)hipEventRecord(L.ev[3], L.stream);
-    }
-    // Whatever happened, the main stream waits for the alpha work before
-    // anything else on it can reuse the workspace.
-    if (joined) {
-      Err j = check(hipStreamWaitEvent(L.stream, L.join, 0), "hipStreamWaitEvent");
-      if (x.ok()) x = j;
-    }
-    return x;
-  };
-  // A capture that fails (another thread's work invalidated it, or the
-  // runtime refused a node) leaves the stream out of capture mode with
-  // nothing launched: the batch then runs as stream launches.
-  bool replay = t.graph != 0;
-  if (np && replay) {
-    GraphKey key;
-    memset(&key, 0, sizeof key);
-    void *ptrs[17] = {B, pics, rows, slices, status, frames, tables, coefs, wide, serr, ferr, itemp, segp, aord,
-                      costp, ordp, chp};
-    memcpy(key.ptrs, ptrs, sizeof ptrs);
-    memcpy(key.grids, grids, sizeof grids);
-    key.ewg = ewg;
-    key.idct_lds = idct_lds;
-    key.t = t;
-    key.alpha = b.any_alpha;
-    // Captured once per key; a key whose capture failed runs as stream
-    // launches without trying again.
-    if (!(key == ws.key) || (!ws.graph && !ws.graph_failed)) {
-      if (ws.graph) {
-        (void)hipGraphExecDestroy(ws.graph);
-        ws.graph = nullptr;
-      }
-      hipGraph_t graph = nullptr;
-      bool captured = false;
-      if (hipStreamBeginCapture(L.stream, hipStreamCaptureModeRelaxed) == hipSuccess) {
-        const Err ce = enqueue(false);
-        const hipError_t ee = hipStreamEndCapture(L.stream, &graph);
-        captured = ce.ok() && ee == hipSuccess && graph;
-      }
-      if (captured && hipGraphInstantiate(&ws.graph, graph, nullptr, nullptr, 0) != hipSuccess) {
-        ws.graph = nullptr;
-        captured = false;
-      }
-      if (graph) (void)hipGraphDestroy(graph);
-      if (!captured) (void)hipGetLastError();  // the failed capture's error is not this batch's
-      ws.key = key;
-      ws.graph_failed = !captured;
-      ws.graph_captures += captured;
-      ws.graph_failures += !captured;
-    }
-    replay = ws.graph != nullptr;
+    A.order = ws.order.p;
   }
   if (np == 0) {
     L.dec_us[0] = L.dec_us[1] = L.dec_us[2] = L.dec_us[3] = L.dec_us[4] = 0;
-  } else if (replay) {
-    (void)hipEventRecord(L.ev[0], L.stream);
-    if (!(e = check(hipGraphLaunch(ws.graph, L.stream), "hipGraphLaunch")).ok()) return e;
-    ws.graph_replays++;
-    (void)hipEventRecord(L.ev[4], L.stream);
-  } else {
-    if (!(e = enqueue(true)).ok()) return e;
-    (void)hipEventRecord(L.ev[4], L.stream);
-  }
-  // Status and first-error words come back in one copy.
-  if (np && !(e = check(hipMemcpyAsync(ws.hres.p, ws.picres.p, 2 * ws.cap_np, hipMemcpyDeviceToHost, L.stream),
-                        "hipMemcpyAsync(results)")).ok())
+  } else if (!(e = run_sequence(d, L, A, t, ws.picres.p, 2 * ws.cap_np)).ok()) {
     return e;
-  if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
-  const uint32_t *st = (const uint32_t *)ws.hres.p, *fe = st + ws.cap_np / 4;
-  if (np) {
-    // Stages: slice table, colour entropy, alpha (from the end of the slice
-    // table to its own end, beside the other two), IDCT, whole batch.
-    float ms[4] = {0, 0, 0, 0}, all = 0;
-    if (!replay) {
-      (void)hipEventElapsedTime(&ms[0], L.ev[0], L.ev[1]);
-      (void)hipEventElapsedTime(&ms[1], L.ev[1], L.ev[2]);
-      if (b.any_alpha) (void)hipEventElapsedTime(&ms[2], L.ev[1], L.ev[5]);
-      (void)hipEventElapsedTime(&ms[3], L.ev[2], L.ev[3]);
-    }
-    (void)hipEventElapsedTime(&all, L.ev[0], L.ev[4]);
-    for (int i = 0; i < 4; i++) L.dec_us[i] = ms[i] * 1000.0;
-    L.dec_us[4] = all * 1000.0;
   }
-
+  const uint32_t *st = (const uint32_t *)ws.hres.p, *fe = st + ws.cap_np / 4;
   results.assign(b.plans.size(), Err{});
   for (size_t f = 0; f < b.plans.size(); f++) {
     const FramePlan &fp = b.plans[f];
@@ -525,26 +599,174 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
       results[f] = fp.early;
       continue;
     }
-    Err r;
-    for (uint32_t p = fp.first_pic; p < fp.first_pic + fp.n_pics && r.ok(); p++) {
-      if (st[p] == PIC_TABLE_ERR) {
-        r = invalid("a slice runs past picture_size");
-      } else if (fe[p] != 0xFFFFFFFFu) {
-        uint32_t words[4];
-        const uint64_t g = b.pics[p].slice_base + (uint64_t)fe[p];
-        if (!(e = copy_on(L, words, (uint8_t *)ws.slice_err.p + 16 * g, 16, hipMemcpyDeviceToHost)).ok())
-          return e;
-        for (int c = 0; c < 4; c++)
-          if (words[c]) {
-            r = slice_error(words[c]);
-            break;
-          }
-      }
-    }
-    if (r.ok()) r = fp.deferred;
-    results[f] = r;
+    if (!(e = picture_result(L, ws, st, fe, fp.first_pic, fp.n_pics, b.pics.data(), nullptr, fp.deferred,
+                             results[f])).ok())
+      return e;
   }
   return Err{};
+}
+
+// The reference's message for a pd_plan error (the host checks' text,
+// host_common.cpp parse_frame_header / check_plausible_size /
+// parse_picture_header, and plan_frame).
+Err plan_error(uint32_t code, const uint64_t *a, uint32_t darg) {
+  switch (code) {
+    case F_SHORT: return invalid(fmtstr("%llu bytes is too short for a ProRes frame header", (unsigned long long)a[0]));
+    case F_ICPF: return invalid("no 'icpf' frame identifier");
+    case F_SIZE_EXCEEDS:
+      return invalid(fmtstr("frame_size %llu exceeds the %llu bytes given", (unsigned long long)a[0],
+                            (unsigned long long)a[1]));
+    case F_SIZE_SMALL: return invalid("frame_size leaves no room for the frame header");
+    case F_HDR_SIZE: return invalid(fmtstr("frame_header_size %llu is out of range", (unsigned long long)a[0]));
+    case F_VERSION: return unsupported(fmtstr("bitstream_version %llu", (unsigned long long)a[0]));
+    case F_DIMS:
+      return invalid(fmtstr("a %llu%s%llu frame", (unsigned long long)a[0], TIMES, (unsigned long long)a[1]));
+    case F_CHROMA: return invalid(fmtstr("chroma_format %llu is reserved", (unsigned long long)a[0]));
+    case F_INTERLACE: return invalid("interlace_mode 3 is reserved");
+    case F_ALPHA: return invalid(fmtstr("alpha_channel_type %llu is reserved", (unsigned long long)a[0]));
+    case F_MATRIX: return invalid("a quantisation matrix runs past frame_header_size");
+    case F_IMPLAUSIBLE:
+      return invalid(fmtstr("frame_size %llu is too small for a %llu%s%llu frame (at least %llu bytes)",
+                            (unsigned long long)a[0], (unsigned long long)a[1], TIMES, (unsigned long long)a[2],
+                            (unsigned long long)a[3]));
+    case F_NO_PACKET: return config_error("a decode job has no device packet");
+    case F_NO_PLANES: return config_error("a decode job has no device planes");
+    case F_NO_ALPHA: return config_error("a decode job has no alpha plane for a frame with alpha");
+    case P_START_PAST: return invalid("a picture starts past the end of the frame");
+    case P_HDR_SHORT: return invalid("a picture header runs past the end of the frame");
+    case P_HDR_SIZE: return invalid(fmtstr("picture_header_size %u is out of range", darg));
+    case P_SIZE: return invalid(fmtstr("picture_size %u is out of range", darg));
+    case P_TABLE: return invalid("the slice table runs past picture_size");
+    default: return Err{};
+  }
+}
+
+// A batch of jobs planned on the device (pd_plan): the jobs go up, the
+// launch sequence runs with no host round trip before it, and the results
+// (each job's record, the batch totals, the picture status words) come back
+// in one copy. The workspace is laid out for the capacities of the largest
+// batch seen on the lane; a batch over them does no work and runs again over
+// a workspace grown to its totals.
+Err run_device_batch(Device &d, Lane &L, uint32_t bit_depth, size_t n, const prores_dev_decode_job *jobs,
+                     const prores_decode_tuning &t, std::vector<Err> &results) {
+  Workspace &ws = workspace(L);
+  Err e;
+  if (!(e = validate_tuning(t)).ok()) return e;
+  if (n > (1u << 24)) return invalid("a batch of more than 2^24 frames");
+  results.assign(n, Err{});
+  if (!n) return Err{};
+  // Jobs up, from the page-locked plan staging.
+  if (!(e = ws.hplan.reserve(n * sizeof(DevJob) + 64)).ok() || !(e = ws.jobs.reserve(n * sizeof(DevJob) + 64)).ok())
+    return e;
+  {
+    DevJob *hj = (DevJob *)ws.hplan.p;
+    for (size_t i = 0; i < n; i++)
+      hj[i] = DevJob{(uint64_t)jobs[i].dev_packet, jobs[i].len, (uint64_t)jobs[i].dev_data, (uint64_t)jobs[i].dev_alpha};
+    if (!(e = check(hipMemcpyAsync(ws.jobs.p, ws.hplan.p, n * sizeof(DevJob), hipMemcpyHostToDevice, L.stream),
+                    "hipMemcpyAsync(jobs)")).ok())
+      return e;
+  }
+  // First capacities: a 4:4:4 frame of 2160p's slices at 8 macroblocks a
+  // job; the totals of a batch over them set the next.
+  PlanCaps &c = ws.dcap;
+  if (!c.slices) {
+    c.slices = (uint32_t)std::min<uint64_t>(4096ull * n, 0xFFFFFF00u);
+    c.coefs = 32400ull * 768 * n;
+  }
+  const uint32_t np = (uint32_t)(2 * n);
+  PlanTotals T;
+  for (int attempt = 0;; attempt++) {
+    c.pics = np;
+    c.rows = std::max<uint32_t>(c.rows, 64 * np);
+    c.segs = std::max<uint32_t>(c.segs, c.rows);
+    c.chunks = std::max<uint32_t>(c.chunks, c.slices / ST_CHUNK + np);
+    c.items = std::max<uint32_t>(c.items, c.slices);
+    c.item_slices = ws.idct_cap_slices;
+    // The plan buffer: frames (one a job), pictures, row slots, segments,
+    // chunks, item costs, header.
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t o_pics = al(n * sizeof(DecFrame)), o_rows = o_pics + al((size_t)c.pics * sizeof(DecPic)),
+                 o_segs = o_rows + al((size_t)c.rows * sizeof(RowSlot)),
+                 o_chunks = o_segs + al((size_t)c.segs * sizeof(IdctItem)),
+                 o_cost = o_chunks + al((size_t)c.chunks * 8), o_hdr = o_cost + al(((size_t)c.slices + 63) / 64 * 4),
+                 plan_bytes = o_hdr + sizeof(DecBatch);
+    // Results: job records, totals, picture status, first errors.
+    const size_t r_tot = al(n * sizeof(PlanRec)), r_st = r_tot + al(sizeof(PlanTotals)), r_fe = r_st + al(np * 4),
+                 res_bytes = r_fe + np * 4;
+    if (!(e = ws.plan.reserve(plan_bytes)).ok() || !(e = ws.dres.reserve(res_bytes + 64)).ok() ||
+        !(e = ws.scratch.reserve(n * sizeof(PlanJob) + 64)).ok() ||
+        !(e = ws.items.reserve((size_t)c.items * sizeof(IdctItem) + 64)).ok() ||
+        !(e = ws.slices.reserve((size_t)c.slices * sizeof(SliceRec) + 64)).ok() ||
+        !(e = ws.slice_err.reserve((size_t)c.slices * 16 + 64)).ok() ||
+        !(e = ws.coefs.reserve(c.coefs * 2 + 64)).ok() || !(e = ws.wide.reserve(c.coefs * 4 + 64)).ok() ||
+        !(e = ws.alpha_order.reserve((size_t)c.slices * sizeof(AlphaRec) + 64)).ok())
+      return e;
+    Launch A;
+    uint8_t *const pl = (uint8_t *)ws.plan.p, *rs = (uint8_t *)ws.dres.p;
+    A.device_plan = true;
+    A.jobs = ws.jobs.p;
+    A.n_jobs = (uint32_t)n;
+    A.want_depth = bit_depth;
+    A.idct_cap = t.idct_blocks;
+    A.chunk = t.persistent ? t.chunk : 1;
+    A.caps = c;
+    A.scratch = ws.scratch.p;
+    A.recs = rs;
+    A.totals = rs + r_tot;
+    A.B = pl + o_hdr;
+    A.frames = pl;
+    A.pics = pl + o_pics;
+    A.rows = pl + o_rows;
+    A.segs = pl + o_segs;
+    A.chunks = pl + o_chunks;
+    A.cost = pl + o_cost;
+    A.slices = ws.slices.p;
+    A.status = rs + r_st;
+    A.ferr = rs + r_fe;
+    A.serr = ws.slice_err.p;
+    A.items = ws.items.p;
+    A.aord = ws.alpha_order.p;
+    A.tables = d.tables;
+    A.coefs = ws.coefs.p;
+    A.wide = ws.wide.p;
+    A.alpha = true;  // whether any frame has alpha is known only on the device
+    launch_shape(d, t, c.slices, c.chunks, c.items, c.item_slices, A);
+    const uint32_t slots = t.persistent ? 3 * ((c.slices + 63) / 64) : A.n_slots;
+    if (slots <= ORDER_MAX) {
+      if (!(e = ws.order.reserve((size_t)ORDER_MAX * 4 + 64)).ok()) return e;
+      A.order = ws.order.p;
+    }
+    if (!(e = run_sequence(d, L, A, t, ws.dres.p, res_bytes)).ok()) return e;
+    const uint8_t *h = (const uint8_t *)ws.hres.p;
+    memcpy(&T, h + r_tot, sizeof T);
+    // The next batches' IDCT items may hold as many slices as this one's
+    // wanted (the dynamic LDS is sized for it).
+    ws.idct_cap_slices = std::max<uint32_t>(std::min<uint32_t>(T.wanted_count, IDCT_MAX_SLICES), 1);
+    if (T.overflow) {
+      if (attempt) return Err{PRORES_ERR_DEVICE, "a device-planned batch outgrew the workspace sized for it"};
+      if (T.slices > 0xFFFFFFF0u || T.chunks > 0xFFFFFFF0u) return invalid("a batch of more than 2^32 slices");
+      auto grow = [](uint32_t v) { return (uint32_t)std::min<uint64_t>((uint64_t)v + v / 4 + 64, 0xFFFFFFF0u); };
+      c.slices = std::max(c.slices, grow(T.slices));
+      c.rows = std::max(c.rows, grow(T.rows));
+      c.segs = std::max(c.segs, grow(T.segs));
+      c.chunks = std::max(c.chunks, grow(T.chunks));
+      c.items = std::max(c.items, grow(T.items));
+      c.coefs = std::max<uint64_t>(c.coefs, T.coefs + T.coefs / 4 + 4096);
+      continue;
+    }
+    const PlanRec *R = (const PlanRec *)h;
+    const uint32_t *st = (const uint32_t *)(h + r_st), *fe = (const uint32_t *)(h + r_fe);
+    for (size_t i = 0; i < n; i++) {
+      if (R[i].code != F_NONE) {
+        results[i] = plan_error(R[i].code, R[i].arg, 0);
+        continue;
+      }
+      if (!(e = picture_result(L, ws, st, fe, R[i].first_pic, R[i].n_pics, nullptr, A.pics,
+                               plan_error(R[i].deferred, R[i].arg, R[i].darg), results[i])).ok())
+        return e;
+    }
+    return Err{};
+  }
 }
 
 // Host memory of the frames the library allocates: page-locked
@@ -718,7 +940,7 @@ static prores_status decode_common(const prores_decoder *d, const uint8_t *data,
     Batch b;
     uint16_t *dd = ws.planes.as<uint16_t>();
     uint16_t *da = alpha_n ? (uint16_t *)((uint8_t *)ws.planes.p + plane_bytes) : nullptr;
-    plan_frame(HeaderSource{data, nullptr}, len, (uint64_t)ws.packet.p, d->bit_depth, dd, da, b, &hd);
+    plan_frame(data, len, (uint64_t)ws.packet.p, d->bit_depth, dd, da, b, &hd);
     std::vector<Err> results;
     prores_decode_tuning t;
     prores_decode_tuning_default(&t);
@@ -752,47 +974,28 @@ prores_status prores_decode_into(const prores_decoder *d, const uint8_t *data, s
 
 namespace prores_hip {
 
-// Plans a batch of device jobs. The header bytes of jobs whose packet is
-// only in device memory (a null host_packet) are gathered first: one
-// pd_headers launch over those packets and one download of HDR_BLOB bytes
-// each.
-Err plan_jobs(Device &d, Lane &L, uint32_t bit_depth, size_t n, const prores_dev_decode_job *jobs, Batch &b) {
-  std::vector<uint64_t> gather;  // device packets, then their lengths
+// A batch is planned on the device when any job's packet is only in device
+// memory (host_packet NULL); else the host plans it from the host copies.
+bool device_planned(size_t n, const prores_dev_decode_job *jobs) {
   for (size_t i = 0; i < n; i++)
-    if (!jobs[i].host_packet && jobs[i].dev_packet) gather.push_back((uint64_t)jobs[i].dev_packet);
-  const size_t m = gather.size();
-  std::vector<uint8_t> records;
-  if (m) {
-    for (size_t i = 0; i < n; i++)
-      if (!jobs[i].host_packet && jobs[i].dev_packet) gather.push_back(jobs[i].len);
-    Workspace &ws = workspace(L);
-    Err e = upload(L, ws.gather, gather);
-    if (e.ok()) e = ws.records.reserve(m * HDR_BLOB + 64);
-    if (!e.ok()) return e;
-    void *pk = ws.gather.p, *ln = (uint64_t *)ws.gather.p + m, *rec = ws.records.p;
-    uint32_t count = (uint32_t)m;
-    void *args[] = {&pk, &ln, &rec, &count};
-    if (!(e = launch(L, d.k_headers, (unsigned)m, 1, 64, args)).ok()) return e;
-    records.resize(m * HDR_BLOB);
-    e = check(hipMemcpyAsync(records.data(), ws.records.p, records.size(), hipMemcpyDeviceToHost, L.stream),
-              "D2H header records");
-    if (e.ok()) e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize(headers)");
-    if (!e.ok()) return e;
-  }
-  size_t k = 0;
+    if (!jobs[i].host_packet) return true;
+  return false;
+}
+
+// One batch of device jobs either way, into one Result per job.
+Err run_jobs(Device &d, Lane &L, uint32_t bit_depth, size_t n, const prores_dev_decode_job *jobs,
+             const prores_decode_tuning &t, std::vector<Err> &results) {
+  if (device_planned(n, jobs)) return run_device_batch(d, L, bit_depth, n, jobs, t, results);
+  Batch b;
+  b.frames.reserve(n);
+  b.pics.reserve(2 * n);
+  b.plans.reserve(n);
+  b.rows.reserve(2 * n * 64);
   for (size_t i = 0; i < n; i++) {
     const prores_dev_decode_job &j = jobs[i];
-    HeaderSource src{j.host_packet, nullptr};
-    if (!j.host_packet && j.dev_packet) src.record = &records[HDR_BLOB * k++];
-    if (!src.frame()) {
-      FramePlan fp;
-      fp.early = config_error("a decode job has neither a host nor a device packet");
-      b.plans.push_back(fp);
-      continue;
-    }
-    plan_frame(src, j.len, (uint64_t)j.dev_packet, bit_depth, j.dev_data, j.dev_alpha, b, nullptr);
+    plan_frame(j.host_packet, j.len, (uint64_t)j.dev_packet, bit_depth, j.dev_data, j.dev_alpha, b, nullptr);
   }
-  return Err{};
+  return run_batch(d, L, b, t, results);
 }
 
 Err decode_batch_on(Device &d, Lane &L, uint32_t bit_depth, size_t n, const prores_dev_decode_job *jobs,
@@ -804,14 +1007,8 @@ Err decode_batch_on(Device &d, Lane &L, uint32_t bit_depth, size_t n, const pror
     std::lock_guard<std::mutex> g(d.mu);
     t = d.tuning;
   }
-  Batch b;
-  b.frames.reserve(n);
-  b.pics.reserve(2 * n);
-  b.plans.reserve(n);
-  b.rows.reserve(2 * n * 64);
-  Err e = plan_jobs(d, L, bit_depth, n, jobs, b);
   std::vector<Err> results;
-  if (e.ok()) e = run_batch(d, L, b, t, results);
+  Err e = run_jobs(d, L, bit_depth, n, jobs, t, results);
   if (!e.ok()) {
     for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
     return e;
@@ -919,7 +1116,7 @@ prores_status prores_decode_host_batch(const prores_decoder *d, size_t n, const 
         uint16_t *da = items[i].alpha_n ? (uint16_t *)((uint8_t *)ws.planes.p + aoff[i - a]) : nullptr;
         e = check(hipMemcpyAsync(dp, data[i], items[i].hd.h.frame_size, hipMemcpyHostToDevice, L->stream),
                   "H2D packet");
-        plan_frame(HeaderSource{data[i], nullptr}, len[i], (uint64_t)dp, d->bit_depth, dd, da, bt, &items[i].hd);
+        plan_frame(data[i], len[i], (uint64_t)dp, d->bit_depth, dd, da, bt, &items[i].hd);
         idx.push_back(i);
       }
       std::vector<Err> results;
@@ -1019,12 +1216,6 @@ prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const pr
     return (prores_status)e.kind;
   }
   std::lock_guard<std::mutex> g(lane0->mu);
-  Batch b;
-  e = plan_jobs(*dev, *lane0, d->bit_depth, n, jobs, b);
-  if (!e.ok()) {
-    fill_error(err, e);
-    return (prores_status)e.kind;
-  }
   if (reps < 1) reps = 1;
   std::vector<prores_decode_tuning> cands;
   for (uint32_t pers = 0; pers < 2; pers++)
@@ -1042,7 +1233,7 @@ prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const pr
   for (const prores_decode_tuning &c : cands) {
     times.clear();
     for (uint32_t r = 0; r <= reps; r++) {
-      if (!(e = run_batch(*dev, *lane0, b, c, results)).ok()) {
+      if (!(e = run_jobs(*dev, *lane0, d->bit_depth, n, jobs, c, results)).ok()) {
         fill_error(err, e);
         return (prores_status)e.kind;
       }

@@ -1,13 +1,15 @@
 // gfx950 decode kernels for the ProRes slice engine.
 //
-// Four launches per batch of pictures (and, for packets the caller holds only
-// in device memory, a fifth before them, pd_headers, that gathers the header
-// bytes the host plans the batch from):
+// Five launches per batch of pictures (and, for a batch with packets the
+// caller holds only in device memory, a sixth before them, pd_plan, that
+// parses the frame and picture headers and lays out the batch on the device):
 // Packet and slice offsets are absolute device addresses.
+//   pd_plan         one workgroup: header parse and batch layout (above).
 //   pd_slice_table  one workgroup per 256 slices of a picture: the u16
 //                   big-endian slice-size table becomes byte offsets (an
 //                   exclusive prefix), plus the bounds check of the table walk
 //                   (src/decode.rs:202-221).
+//   pd_entropy_order one workgroup: the entropy items ordered by cost.
 //   pd_entropy      one lane per colour component of a slice: the RDD 36
 //                   adaptive Rice/exp-Golomb DC and run/level decode
 //                   (src/decode.rs:484-540, src/vlc.rs) into a coefficient
@@ -1180,7 +1182,7 @@ extern "C" __global__ void __launch_bounds__(ENTROPY_MAX_WG)
   if (!persistent) {
     const uint32_t b = blockIdx.x, w = threadIdx.x >> 6, ew = blockDim.x >> 6, nb = gridDim.x;
     uint32_t item = b * ew + w;
-    if (order) {
+    if (order && B->ordered) {
       // `order` holds the items longest first. Tier k of them (nb items)
       // goes one to a workgroup, odd tiers in reverse, so every workgroup
       // (one to a CU) gets one item of each tier; wave w takes tier
@@ -1200,7 +1202,7 @@ extern "C" __global__ void __launch_bounds__(ENTROPY_MAX_WG)
     if (first >= n_items) break;
     const uint32_t last = first + chunk < n_items ? first + chunk : n_items;
     for (uint32_t k = first; k < last; k++) {
-      const uint32_t item = order ? order[k] : k;  // longest first
+      const uint32_t item = order && B->ordered ? order[k] : k;  // longest first
       const uint32_t g = (item / 3) * 64 + lane;
       entropy_stream(g < n_slices, g, item % 3, lines, list, &B->wide, ftab, frames, pics, slices, pic_status, coefs,
                      coef_wide, slice_err, pic_first_err);
@@ -1218,7 +1220,7 @@ extern "C" __global__ void __launch_bounds__(ENTROPY_MAX_WG)
 // n_slots <= ORDER_MAX.
 constexpr uint32_t ORDER_BUCKETS = 2048, ORDER_BUCKET_BYTES = 32;
 extern "C" __global__ void __launch_bounds__(1024)
-    pd_entropy_order(const DecBatch *__restrict__ B, const uint32_t *__restrict__ group_cost,
+    pd_entropy_order(DecBatch *__restrict__ B, const uint32_t *__restrict__ group_cost,
                      uint32_t *__restrict__ order, uint32_t n_slots) {
   __shared__ uint32_t cnt[ORDER_BUCKETS];
   __shared__ uint16_t bucket[ORDER_MAX];
@@ -1226,6 +1228,9 @@ extern "C" __global__ void __launch_bounds__(1024)
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t G = B->n_slices, groups = (G + 63) / 64, n_items = 3 * groups;
   if (n_slots == 0) n_slots = n_items;
+  // A batch of more items than the order holds goes unordered (a device
+  // plan's count is known only here).
+  if (n_slots > ORDER_MAX || n_items > n_slots) return;
   for (uint32_t i = tid; i < ORDER_BUCKETS; i += blockDim.x) cnt[i] = 0;
   __syncthreads();
   // Costs, one a group of 64 slices (its largest slice, from
@@ -1257,6 +1262,7 @@ extern "C" __global__ void __launch_bounds__(1024)
   __syncthreads();
   for (uint32_t i = tid; i < n_items; i += blockDim.x) order[atomicAdd(&cnt[bucket[i]], 1u)] = i;
   for (uint32_t i = n_items + tid; i < n_slots; i += blockDim.x) order[i] = i;
+  if (tid == 0) B->ordered = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1587,34 +1593,453 @@ extern "C" __global__ void __launch_bounds__(256)
   }
 }
 
-// Gathers the bytes the host reads to plan a frame (the frame header, and
-// the picture headers that Decoder::decode walks with `pos`:
-// src/decode.rs:85-102, src/header.rs:82-170, 262-290) from packets that are
-// only in device memory, into one HDR_BLOB record a packet. One wave a
-// packet; every read is bounded by the packet's length. The host repeats the
-// header checks on the gathered bytes, so nothing is validated here beyond
-// the bounds.
-extern "C" __global__ void __launch_bounds__(64)
-pd_headers(const uint64_t *packets, const uint64_t *lens, uint8_t *records, uint32_t n) {
-  const uint32_t j = blockIdx.x, t = threadIdx.x;
-  if (j >= n) return;
-  const uint8_t *p = (const uint8_t *)packets[j];
-  const uint64_t len = lens[j];
-  uint8_t *out = records + (uint64_t)j * HDR_BLOB;
-  for (uint32_t i = t; i < HDR_FRAME_BYTES; i += 64) out[i] = i < len ? p[i] : (uint8_t)0;
-  // The first picture follows the frame header (8 + frame_header_size);
-  // an interlaced frame's second follows the first (+ picture_size).
-  uint64_t pos[2] = {~0ull, ~0ull};
-  if (len >= 28) {
-    pos[0] = 8u + ((uint32_t)p[8] << 8 | p[9]);
-    const uint64_t q = pos[0];
-    if (q + HDR_PIC_BYTES <= len && ((p[20] >> 2) & 3) != 0)
-      pos[1] = q + ((uint32_t)p[q + 1] << 24 | (uint32_t)p[q + 2] << 16 | (uint32_t)p[q + 3] << 8 | p[q + 4]);
+// ---------------------------------------------------------------------------
+// pd_plan: a batch of packets held in device memory, planned on the device.
+// One workgroup. For each job (one thread a job): FrameHeader::parse
+// (src/header.rs:82-170), check_plausible_size (src/decode.rs:107-128), the
+// job's device buffers, then each picture's PictureHeader::parse
+// (src/header.rs:262-290) and slice-table extent (src/decode.rs:202-210),
+// with the reference's checks in its order; then prefix sums over the jobs
+// place every good picture, and each job writes its DecFrame, DecPics, row
+// slots, IDCT row segments and slice-table chunks, exactly as the host
+// planner lays out a batch of host packets. The entropy item costs are
+// cleared and the DecBatch written (queues zero). A batch over the
+// workspace's capacities writes nothing but the totals, marked overflow, and
+// a DecBatch with no work.
+namespace {
+
+__device__ __forceinline__ uint32_t rd16(const GLOBAL uint8_t *p) { return (uint32_t)p[0] << 8 | p[1]; }
+__device__ __forceinline__ uint32_t rd32(const GLOBAL uint8_t *p) {
+  return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3];
+}
+
+// Slices of a row of wmb macroblocks at 2^log2 (header::slice_sizes,
+// src/header.rs:308-320): wmb >> log2 full slices, then one for each set bit
+// of the remainder, largest first. Slice j's first macroblock and size.
+__device__ __forceinline__ uint32_t row_slice_count(uint32_t wmb, uint32_t log2) {
+  return (wmb >> log2) + __builtin_popcount(wmb & ((1u << log2) - 1));
+}
+__device__ __forceinline__ void row_slice(uint32_t wmb, uint32_t log2, uint32_t j, uint32_t &mb_x, uint32_t &mbs) {
+  const uint32_t full = wmb >> log2;
+  if (j < full) {
+    mb_x = j << log2;
+    mbs = 1u << log2;
+    return;
   }
-  if (t < 2 * HDR_PIC_BYTES) {
-    const uint64_t q = pos[t / HDR_PIC_BYTES], i = t % HDR_PIC_BYTES;
-    out[HDR_PIC_AT + t] = q != ~0ull && q + HDR_PIC_BYTES <= len ? p[q + i] : (uint8_t)0;
-  } else if (t < 2 * HDR_PIC_BYTES + 2) {
-    ((uint64_t *)(out + HDR_POS_AT))[t - 2 * HDR_PIC_BYTES] = pos[t - 2 * HDR_PIC_BYTES];
+  uint32_t x = full << log2, rem = wmb & ((1u << log2) - 1);
+  for (uint32_t k = full; k < j; k++) {  // the tail slices before j (at most 2 of them)
+    const uint32_t b = 1u << (31 - __builtin_clz(rem));
+    x += b;
+    rem -= b;
+  }
+  mb_x = x;
+  mbs = 1u << (31 - __builtin_clz(rem));
+}
+
+// The IDCT row segments of a row (the host planner's cut): consecutive
+// slices while their blocks fit `cap` and they are at most `max_slices`; a
+// larger slice is a segment of its own. Counts them, or writes them at `out`.
+__device__ uint32_t row_segments(uint32_t wmb, uint32_t log2, uint32_t cpm, uint32_t cap, uint32_t max_slices,
+                                 IdctItem *out, uint32_t &max_count, uint32_t &wanted) {
+  const uint32_t n = row_slice_count(wmb, log2);
+  IdctItem cur{0, 0, 0, 0};
+  uint32_t segs = 0, lcount = 0, lblocks = 0;  // the segment the cut would make without max_slices
+  max_count = 0;
+  wanted = 0;
+  for (uint32_t j = 0; j < n; j++) {
+    uint32_t mb_x, mbs;
+    row_slice(wmb, log2, j, mb_x, mbs);
+    const uint32_t tb = mbs * (4 + 2 * cpm);
+    if (cur.count && (cur.blocks + tb > cap || cur.count == max_slices)) {
+      if (out) out[segs] = cur;
+      max_count = cur.count > max_count ? cur.count : max_count;
+      segs++;
+      cur = IdctItem{j, 0, 0, 0};
+    }
+    if (lcount && (lblocks + tb > cap || lcount == IDCT_MAX_SLICES)) lcount = lblocks = 0;
+    lcount++;
+    lblocks += tb;
+    wanted = lcount > wanted ? lcount : wanted;
+    cur.count++;
+    cur.blocks += tb;
+  }
+  if (out) out[segs] = cur;
+  max_count = cur.count > max_count ? cur.count : max_count;
+  return segs + 1;
+}
+
+// Parses job j's headers into its record and PlanJob, with the reference's
+// checks in its order.
+__device__ void plan_parse(const DevJob &J, PlanRec &R, PlanJob &P, uint32_t cap, uint32_t max_slices) {
+  R.code = F_NONE;
+  R.deferred = F_NONE;
+  R.n_pics = 0;
+  R.first_pic = 0;
+  R.arg[0] = R.arg[1] = R.arg[2] = R.arg[3] = 0;
+  R.darg = 0;
+  P.n_pics = 0;
+  P.slices = P.rows = P.segs = P.items = P.chunks = P.max_count = P.wanted = 0;
+  P.coefs = 0;
+  P.base_pic = P.base_slice = P.base_row = P.base_seg = P.base_item = P.base_chunk = 0;
+  P.base_coef = 0;
+  if (!J.packet) {
+    R.code = F_NO_PACKET;
+    return;
+  }
+  const GLOBAL uint8_t *d = (const GLOBAL uint8_t *)(uintptr_t)J.packet;
+  const uint64_t len = J.len;
+  if (len < 28) {
+    R.code = F_SHORT;
+    R.arg[0] = len;
+    return;
+  }
+  const uint32_t frame_size = rd32(d);
+  if (!(d[4] == 'i' && d[5] == 'c' && d[6] == 'p' && d[7] == 'f')) {
+    R.code = F_ICPF;
+    return;
+  }
+  if ((uint64_t)frame_size > len) {
+    R.code = F_SIZE_EXCEEDS;
+    R.arg[0] = frame_size;
+    R.arg[1] = len;
+    return;
+  }
+  if (frame_size < 28) {
+    R.code = F_SIZE_SMALL;
+    return;
+  }
+  const GLOBAL uint8_t *h = d + 8;
+  const uint32_t avail = frame_size - 8, header_size = rd16(h);
+  if (header_size < 20 || header_size > avail) {
+    R.code = F_HDR_SIZE;
+    R.arg[0] = header_size;
+    return;
+  }
+  if (h[3] > 1) {
+    R.code = F_VERSION;
+    R.arg[0] = h[3];
+    return;
+  }
+  const uint32_t width = rd16(h + 8), height = rd16(h + 10);
+  if (width == 0 || height == 0) {
+    R.code = F_DIMS;
+    R.arg[0] = width;
+    R.arg[1] = height;
+    return;
+  }
+  const uint32_t chroma = h[12] >> 6, interlace = (h[12] >> 2) & 3;
+  if (chroma != 2 && chroma != 3) {
+    R.code = F_CHROMA;
+    R.arg[0] = chroma;
+    return;
+  }
+  if (interlace == 3) {
+    R.code = F_INTERLACE;
+    return;
+  }
+  const uint32_t alpha = h[17] & 15;
+  if (alpha > 2) {
+    R.code = F_ALPHA;
+    R.arg[0] = alpha;
+    return;
+  }
+  uint32_t mpos = 20;
+  if (h[19] & 2) {
+    if (mpos + 64 > header_size) {
+      R.code = F_MATRIX;
+      return;
+    }
+    mpos += 64;
+  }
+  if ((h[19] & 1) && mpos + 64 > header_size) {
+    R.code = F_MATRIX;
+    return;
+  }
+  // check_plausible_size: 11 bytes a slice at 8-macroblock slices.
+  const uint32_t wmb = (width + 15) / 16, n_pics = interlace ? 2u : 1u;
+  const uint64_t per_row = row_slice_count(wmb, 3);
+  uint64_t need = 8 + (uint64_t)header_size;
+  uint32_t heights[2];
+  for (uint32_t i = 0; i < n_pics; i++) {
+    const bool top = interlace == 0 || ((interlace == 1) == (i == 0));
+    heights[i] = interlace == 0 ? height : (top ? (height + 1) / 2 : height / 2);
+    need += 8 + 11 * per_row * (((uint64_t)heights[i] + 15) / 16);
+  }
+  if ((uint64_t)frame_size < need) {
+    R.code = F_IMPLAUSIBLE;
+    R.arg[0] = frame_size;
+    R.arg[1] = width;
+    R.arg[2] = height;
+    R.arg[3] = need;
+    return;
+  }
+  // The device writes through these: a job without them decodes nothing.
+  if (!J.data) {
+    R.code = F_NO_PLANES;
+    return;
+  }
+  if (alpha && !J.alpha) {
+    R.code = F_NO_ALPHA;
+    return;
+  }
+  P.wmb = wmb;
+  P.cpm = chroma == 2 ? 2u : 4u;
+  uint64_t pos = 8 + (uint64_t)header_size;
+  for (uint32_t i = 0; i < n_pics; i++) {
+    if (pos > frame_size) {
+      R.deferred = P_START_PAST;
+      break;
+    }
+    const uint64_t plen = frame_size - pos;
+    if (plen < 8) {
+      R.deferred = P_HDR_SHORT;
+      break;
+    }
+    const GLOBAL uint8_t *ph = d + pos;
+    const uint32_t hs = ph[0] >> 3, ps = rd32(ph + 1);
+    if (hs < 8 || hs > plen) {
+      R.deferred = P_HDR_SIZE;
+      R.darg = hs;
+      break;
+    }
+    if (ps < hs || ps > plen) {
+      R.deferred = P_SIZE;
+      R.darg = ps;
+      break;
+    }
+    const uint32_t log2 = (ph[7] >> 4) & 3, hmb = (heights[i] + 15) / 16;
+    const uint32_t rs = row_slice_count(wmb, log2);
+    const uint64_t n = (uint64_t)rs * hmb;
+    if ((uint64_t)hs + 2 * n > ps) {
+      R.deferred = P_TABLE;
+      break;
+    }
+    PlanPic &Q = P.pic[P.n_pics];
+    Q.pos = pos;
+    Q.header_size = hs;
+    Q.picture_size = ps;
+    Q.log2 = log2;
+    Q.row_slices = rs;
+    Q.height = heights[i];
+    Q.hmb = hmb;
+    uint32_t mc, want;
+    Q.n_segs = row_segments(wmb, log2, P.cpm, cap, max_slices, nullptr, mc, want);
+    Q.max_count = mc;
+    P.max_count = mc > P.max_count ? mc : P.max_count;
+    P.wanted = want > P.wanted ? want : P.wanted;
+    P.slices += (uint32_t)n;
+    P.rows += rs;
+    P.segs += Q.n_segs;
+    P.items += hmb * Q.n_segs;
+    P.chunks += n ? (uint32_t)((n + ST_CHUNK - 1) / ST_CHUNK) : 1u;
+    P.coefs += (uint64_t)wmb * (4 + 2 * P.cpm) * 64 * hmb;
+    P.n_pics++;
+    pos += ps;
+  }
+  R.n_pics = P.n_pics;
+}
+
+}  // namespace
+
+// The plan buffer's parts: frames (one a job), pictures, row slots, IDCT row
+// segments, slice-table chunks (uint2), entropy item costs, DecBatch.
+extern "C" __global__ void __launch_bounds__(256)
+    pd_plan(const DevJob *__restrict__ jobs, uint32_t n_jobs, uint32_t want_depth, uint32_t idct_cap,
+            uint32_t chunk, PlanCaps caps, PlanJob *__restrict__ scratch, PlanRec *__restrict__ recs,
+            PlanTotals *__restrict__ totals, DecFrame *__restrict__ frames, DecPic *__restrict__ pics,
+            RowSlot *__restrict__ rows, IdctItem *__restrict__ segs, uint2 *__restrict__ chunks,
+            uint32_t *__restrict__ group_cost, DecBatch *__restrict__ B) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t max_slices = caps.item_slices < IDCT_MAX_SLICES ? caps.item_slices : IDCT_MAX_SLICES;
+  for (uint32_t j = tid; j < n_jobs; j += 256) {
+    const DevJob J = jobs[j];
+    plan_parse(J, recs[j], scratch[j], idct_cap, max_slices);
+  }
+  __syncthreads();
+  // Exclusive prefix sums over the jobs, 256 at a time: pictures, slices,
+  // row slots, segments, IDCT items, chunks (u32) and coefficients (u64).
+  __shared__ uint32_t wsum[4][6];
+  __shared__ uint64_t wcoef[4];
+  __shared__ uint32_t carry[6], maxc[2];
+  __shared__ uint64_t ccarry;
+  if (tid < 6) carry[tid] = 0;
+  if (tid < 2) maxc[tid] = 0;
+  if (tid == 0) ccarry = 0;
+  __syncthreads();
+  for (uint32_t b = 0; b < n_jobs; b += 256) {
+    const uint32_t j = b + tid;
+    uint32_t v[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t cv = 0;
+    if (j < n_jobs) {
+      const PlanJob &P = scratch[j];
+      v[0] = P.n_pics;
+      v[1] = P.slices;
+      v[2] = P.rows;
+      v[3] = P.segs;
+      v[4] = P.items;
+      v[5] = P.chunks;
+      cv = P.coefs;
+      atomicMax(&maxc[0], P.max_count);
+      atomicMax(&maxc[1], P.wanted);
+    }
+    uint32_t incl[6];
+    uint64_t cincl = cv;
+    for (int k = 0; k < 6; k++) incl[k] = v[k];
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      for (int k = 0; k < 6; k++) {
+        const uint32_t o = __shfl_up(incl[k], d, 64);
+        if (lane >= d) incl[k] += o;
+      }
+      const uint64_t co = __shfl_up(cincl, d, 64);
+      if (lane >= d) cincl += co;
+    }
+    if (lane == 63) {
+      for (int k = 0; k < 6; k++) wsum[wave][k] = incl[k];
+      wcoef[wave] = cincl;
+    }
+    __syncthreads();
+    uint32_t before[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t cbefore = 0;
+    for (uint32_t w = 0; w < wave; w++) {
+      for (int k = 0; k < 6; k++) before[k] += wsum[w][k];
+      cbefore += wcoef[w];
+    }
+    if (j < n_jobs) {
+      // The job's bases, kept in its scratch record for the layout pass.
+      PlanJob &P = scratch[j];
+      P.base_pic = carry[0] + before[0] + incl[0] - v[0];
+      P.base_slice = carry[1] + before[1] + incl[1] - v[1];
+      P.base_row = carry[2] + before[2] + incl[2] - v[2];
+      P.base_seg = carry[3] + before[3] + incl[3] - v[3];
+      P.base_item = carry[4] + before[4] + incl[4] - v[4];
+      P.base_chunk = carry[5] + before[5] + incl[5] - v[5];
+      P.base_coef = ccarry + cbefore + cincl - cv;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int k = 0; k < 6; k++) carry[k] += wsum[0][k] + wsum[1][k] + wsum[2][k] + wsum[3][k];
+      ccarry += wcoef[0] + wcoef[1] + wcoef[2] + wcoef[3];
+    }
+    __syncthreads();
+  }
+  const bool over = carry[0] > caps.pics || carry[1] > caps.slices || carry[2] > caps.rows || carry[3] > caps.segs ||
+                    carry[4] > caps.items || carry[5] > caps.chunks || ccarry > caps.coefs;
+  if (tid == 0) {
+    PlanTotals T;
+    T.overflow = over ? 1u : 0u;
+    T.pics = carry[0];
+    T.slices = carry[1];
+    T.rows = carry[2];
+    T.segs = carry[3];
+    T.items = carry[4];
+    T.chunks = carry[5];
+    T.max_count = maxc[0];
+    T.coefs = ccarry;
+    T.wanted_count = maxc[1];
+    T.pad = 0;
+    *totals = T;
+    DecBatch D;
+    memset(&D, 0, sizeof D);
+    if (!over) {
+      D.n_pics = carry[0];
+      D.n_slices = carry[1];
+      D.n_entropy_items = 3 * ((carry[1] + 63) / 64);
+      D.n_idct_items = carry[4];
+      D.n_chunks = carry[5];
+    }
+    D.chunk = chunk;
+    *B = D;
+  }
+  if (over) return;
+  // The entropy item costs start at zero.
+  for (uint32_t i = tid; i < (carry[1] + 63) / 64; i += 256) group_cost[i] = 0;
+  // Each job's frame, pictures, row slots, segments and chunks.
+  for (uint32_t j = tid; j < n_jobs; j += 256) {
+    PlanRec &R = recs[j];
+    const PlanJob P = scratch[j];
+    R.first_pic = P.base_pic;
+    if (R.code != F_NONE) continue;
+    const DevJob J = jobs[j];
+    const GLOBAL uint8_t *d = (const GLOBAL uint8_t *)(uintptr_t)J.packet, *h = d + 8;
+    const uint32_t width = rd16(h + 8), height = rd16(h + 10), chroma = h[12] >> 6, interlace = (h[12] >> 2) & 3;
+    const uint32_t alpha = h[17] & 15;
+    const uint32_t bd = want_depth ? want_depth : (chroma == 2 ? 10u : 12u);
+    DecFrame F;
+    memset(&F, 0, sizeof F);
+    // Frame::layout (src/frame.rs:247-266): chroma width ceil(w/2) for 4:2:2.
+    const uint32_t cw = chroma == 2 ? (width + 1) / 2 : width;
+    const uint64_t off[3] = {0, (uint64_t)width * height, (uint64_t)width * height + (uint64_t)cw * height};
+    for (int i = 0; i < 3; i++) F.out_plane[i] = J.data + 2 * off[i];
+    F.plane_w[0] = width;
+    F.plane_w[1] = F.plane_w[2] = cw;
+    F.out_alpha = J.alpha;
+    F.width = width;
+    F.height = height;
+    F.chroma = chroma;
+    F.alpha_bits = alpha == 1 ? 8u : alpha == 2 ? 16u : 0u;
+    F.bit_depth = bd;
+    F.interlaced = interlace != 0;
+    // Output (src/dsp/mod.rs:88-96): scale 2^b/512, bias 2^(b-1) + 0.5, max 2^b - 1.
+    F.out_scale = (float)(1u << bd) / 512.0f;
+    F.out_bias = (float)(1u << (bd - 1)) + 0.5f;
+    F.out_max = (float)((1u << bd) - 1);
+    if (F.alpha_bits) {
+      // Rescale (src/decode.rs:446-469): (a * ceil(max_out 2^40 / max_in) + 2^39) >> 40.
+      const uint64_t max_in = (1ull << F.alpha_bits) - 1, max_out = (1ull << bd) - 1;
+      F.rescale_m = ((max_out << 40) + max_in - 1) / max_in;
+      F.rescale_identity = F.alpha_bits == bd;
+    }
+    // Weights (src/header.rs:202-210): the loaded matrices, else all 4s;
+    // chroma falls back to the luma weights.
+    uint32_t mpos = 20;
+    const bool lm = (h[19] & 2) != 0, cm = (h[19] & 1) != 0;
+    for (int k = 0; k < 64; k++) F.luma_w[k] = lm ? h[mpos + k] : (uint8_t)4;
+    if (lm) mpos += 64;
+    for (int k = 0; k < 64; k++) F.chroma_w[k] = cm ? h[mpos + k] : F.luma_w[k];
+    frames[j] = F;
+    const uint32_t cpm = P.cpm;
+    uint32_t pic = P.base_pic, slice = P.base_slice, row = P.base_row, seg = P.base_seg, item = P.base_item,
+             ch = P.base_chunk;
+    uint64_t coef = P.base_coef;
+    for (uint32_t i = 0; i < R.n_pics; i++) {
+      const PlanPic &Q = P.pic[i];
+      DecPic D;
+      memset(&D, 0, sizeof D);
+      D.pic_off = J.packet + Q.pos;
+      D.coef_base = coef;
+      D.header_size = Q.header_size;
+      D.picture_size = Q.picture_size;
+      D.n_slices = Q.row_slices * Q.hmb;
+      D.row_slices = Q.row_slices;
+      D.slice_base = slice;
+      D.layout_base = row;
+      D.frame = j;
+      D.height = Q.height;
+      D.height_in_mb = Q.hmb;
+      const bool top = interlace == 0 || ((interlace == 1) == (i == 0));
+      D.first_row = interlace == 0 ? 0u : (top ? 0u : 1u);
+      D.row_step = interlace == 0 ? 1u : 2u;
+      D.seg_base = seg;
+      D.n_segs = Q.n_segs;
+      D.item_base = item;
+      uint32_t prefix = 0;
+      for (uint32_t k = 0; k < Q.row_slices; k++) {
+        uint32_t mb_x, mbs;
+        row_slice(P.wmb, Q.log2, k, mb_x, mbs);
+        rows[row + k] = RowSlot{(uint16_t)mb_x, (uint16_t)mbs, prefix};
+        prefix += mbs * (4 + 2 * cpm) * 64;
+      }
+      D.row_coefs = prefix;
+      uint32_t mc, want;
+      row_segments(P.wmb, Q.log2, cpm, idct_cap, max_slices, segs + seg, mc, want);
+      for (uint32_t k = 0; k == 0 || k < D.n_slices; k += ST_CHUNK) chunks[ch++] = make_uint2(pic, k);
+      pics[pic] = D;
+      coef += (uint64_t)prefix * Q.hmb;
+      slice += D.n_slices;
+      row += Q.row_slices;
+      seg += Q.n_segs;
+      item += Q.hmb * Q.n_segs;
+      pic++;
+    }
   }
 }

@@ -357,9 +357,9 @@ static Err init_device(Device &d, int id) {
   d.id = id;
   if (!(e = check(hipModuleLoadData(&d.dec_mod, prores_decode_co), "hipModuleLoadData(decode)")).ok())
     return e;
-  const char *const names[] = {"pd_slice_table", "pd_entropy", "pd_alpha", "pd_idct", "pd_headers",
+  const char *const names[] = {"pd_slice_table", "pd_entropy", "pd_alpha", "pd_idct", "pd_plan",
                                "pd_entropy_order"};
-  hipFunction_t *const fns[] = {&d.k_slice_table, &d.k_entropy, &d.k_alpha, &d.k_idct, &d.k_headers, &d.k_order};
+  hipFunction_t *const fns[] = {&d.k_slice_table, &d.k_entropy, &d.k_alpha, &d.k_idct, &d.k_plan, &d.k_order};
   if (!(e = load_functions(d.dec_mod, names, fns, 6)).ok()) return e;
   DevTables host;
   build_tables(host);

@@ -99,7 +99,7 @@ struct Device {
   int id = -1;
   hipModule_t dec_mod = nullptr, enc_mod = nullptr;
   hipFunction_t k_slice_table = nullptr, k_entropy = nullptr, k_alpha = nullptr, k_idct = nullptr,
-                k_headers = nullptr, k_order = nullptr;
+                k_plan = nullptr, k_order = nullptr;
   hipFunction_t k_fdct = nullptr, k_alpha_enc = nullptr, k_size = nullptr, k_search = nullptr,
                 k_shareout = nullptr, k_offsets = nullptr, k_pack = nullptr,
                 k_window = nullptr, k_arena = nullptr, k_slices = nullptr;

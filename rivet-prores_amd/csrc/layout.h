@@ -129,7 +129,8 @@ struct DecBatch {
   uint32_t wide;             // nonzero once the entropy decode stores a value outside +-32767
   uint32_t q[4];             // queue heads: entropy, alpha, idct
   uint32_t n_chunks;         // pd_slice_table's chunks of ST_CHUNK slices
-  uint32_t pad[3];
+  uint32_t ordered;          // nonzero once pd_entropy_order has written the order
+  uint32_t pad[2];
 };
 // Slices a pd_slice_table workgroup (one thread each); a picture's chunks
 // start at multiples of it, and a picture without slices has one chunk.
@@ -143,14 +144,88 @@ constexpr uint32_t ST_CHUNK = 256;
 struct IdctItem {
   uint32_t first, count, blocks, pad;
 };
-// Header bytes pd_headers gathers from a packet held only in device memory,
-// one HDR_BLOB record a packet: the packet's first HDR_FRAME_BYTES bytes (a
-// frame header with both quantisation matrices ends at byte 156), the first
-// HDR_PIC_BYTES bytes of each of the two pictures, and the two pictures'
-// byte offsets (u64; ~0 where the packet is too short to locate one).
-constexpr uint32_t HDR_FRAME_BYTES = 160, HDR_PIC_BYTES = 8, HDR_BLOB = 192;
-constexpr uint32_t HDR_PIC_AT = HDR_FRAME_BYTES, HDR_POS_AT = HDR_FRAME_BYTES + 2 * HDR_PIC_BYTES;
-static_assert(HDR_POS_AT + 16 == HDR_BLOB, "header record layout");
+// ---------------------------------------------------------------------------
+// Device-planned batches (pd_plan): the frame and picture headers of packets
+// held in device memory are parsed on the device (FrameHeader::parse,
+// check_plausible_size, PictureHeader::parse: src/header.rs:82-290,
+// src/decode.rs:113-128) and the batch laid out there, so the launch sequence
+// needs no host round trip first.
+
+// One job: device addresses of the packet, the output planes and alpha, and
+// the packet's length (64 readable bytes past it).
+struct DevJob {
+  uint64_t packet, len, data, alpha;
+};
+
+// Header errors, in the order the host's checks make them. A code | the
+// arguments the message needs (PlanRec::arg). F_* stop the frame before any
+// picture; P_* stop it after `n_pics` good pictures, and count only when
+// those decode cleanly (the first error in picture order).
+enum PlanError : uint32_t {
+  F_NONE = 0,
+  F_SHORT = 1,         // "{len} bytes is too short for a ProRes frame header"
+  F_ICPF = 2,          // "no 'icpf' frame identifier"
+  F_SIZE_EXCEEDS = 3,  // "frame_size {a0} exceeds the {a1} bytes given"
+  F_SIZE_SMALL = 4,    // "frame_size leaves no room for the frame header"
+  F_HDR_SIZE = 5,      // "frame_header_size {a0} is out of range"
+  F_VERSION = 6,       // Unsupported "bitstream_version {a0}"
+  F_DIMS = 7,          // "a {a0}x{a1} frame"
+  F_CHROMA = 8,        // "chroma_format {a0} is reserved"
+  F_INTERLACE = 9,     // "interlace_mode 3 is reserved"
+  F_ALPHA = 10,        // "alpha_channel_type {a0} is reserved"
+  F_MATRIX = 11,       // "a quantisation matrix runs past frame_header_size"
+  F_IMPLAUSIBLE = 12,  // "frame_size {a0} is too small for a {a1}x{a2} frame (at least {a3} bytes)"
+  F_NO_PACKET = 13,    // Config: "a decode job has no device packet"
+  F_NO_PLANES = 14,    // Config: "a decode job has no device planes"
+  F_NO_ALPHA = 15,     // Config: "a decode job has no alpha plane for a frame with alpha"
+  P_START_PAST = 16,   // "a picture starts past the end of the frame"
+  P_HDR_SHORT = 17,    // "a picture header runs past the end of the frame"
+  P_HDR_SIZE = 18,     // "picture_header_size {d} is out of range"
+  P_SIZE = 19,         // "picture_size {d} is out of range"
+  P_TABLE = 20,        // "the slice table runs past picture_size"
+};
+
+// pd_plan's record of one job.
+struct PlanRec {
+  uint32_t code;      // F_* or F_NONE
+  uint32_t deferred;  // P_* after the good pictures, or F_NONE
+  uint32_t first_pic, n_pics;  // the good pictures (DecPic indices)
+  uint64_t arg[4];    // the arguments of `code`
+  uint32_t darg;      // the argument of `deferred`
+  uint32_t pad;
+};
+static_assert(sizeof(PlanRec) == 56, "PlanRec layout");
+
+// pd_plan's scratch: what the parse of one job leaves for the layout pass
+// (its good pictures and their counts), and the job's bases in the batch.
+struct PlanPic {
+  uint64_t pos;  // byte offset of the picture in the packet
+  uint32_t header_size, picture_size, log2, row_slices, height, hmb, n_segs, max_count;
+};
+struct PlanJob {
+  uint32_t n_pics, wmb, cpm, pad;
+  uint32_t slices, rows, segs, items, chunks, max_count, wanted;
+  uint64_t coefs;
+  uint32_t base_pic, base_slice, base_row, base_seg, base_item, base_chunk;
+  uint64_t base_coef;
+  PlanPic pic[2];
+};
+
+// The capacities pd_plan lays a batch into (elements), and the batch's own
+// counts it found: when any count is over its capacity the batch is marked
+// (overflow) and does no work, and the host grows its workspace and runs it
+// again.
+struct PlanCaps {
+  uint32_t pics, rows, segs, chunks, slices, items, item_slices, pad;
+  uint64_t coefs;
+};
+struct PlanTotals {
+  uint32_t overflow, pics, rows, segs, chunks, slices, items, max_count;
+  uint64_t coefs;
+  uint32_t wanted_count;  // the most slices an IDCT item would take without the capacity's limit
+  uint32_t pad;
+};
+static_assert(sizeof(PlanTotals) == 48, "PlanTotals layout");
 
 // Entropy work placement: batches of at most ORDER_MAX grid slots have
 // their entropy items ordered by cost (the largest slice of the item's 64,

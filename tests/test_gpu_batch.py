@@ -129,8 +129,8 @@ def test_autotune_then_batch(batch):
 
 @pytest.mark.parametrize("graph", (0, 1))
 def test_device_only_packets(batch, graph):
-    """Jobs without a host copy of their packet: the header bytes are gathered
-    on the device. Beside them, jobs with a host copy, and packets cut short
+    """Jobs without a host copy of their packet: the batch is planned on the
+    device (headers parsed there). Beside them, jobs with a host copy, and packets cut short
     (inside the frame header, inside a picture header, inside the slices),
     whose errors must be the oracle's."""
     packets, want, bufs, _ = batch
@@ -208,3 +208,48 @@ def test_batch_past_the_ordering_limit(persistent):
     assert errs == [None] * n, errs
     for i, o in enumerate(outs):
         np.testing.assert_array_equal(o.download(total * 2, dtype=np.uint16), want.data, err_msg=f"copy {i}")
+
+
+_FRESH_PLAN = r"""
+import sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import prores_amd as P
+packet = open(sys.argv[2], "rb").read()
+h = P.FrameHeader.parse(packet)
+planes = P.layout(h.width, h.height, h.chroma)
+n = planes[2][0] + planes[2][1] * planes[2][2]
+dp = P.DeviceBuffer(len(packet) + 128)
+dp.upload(np.frombuffer(packet, np.uint8))
+outs = [P.DeviceBuffer(n * 2 + 64) for _ in range(3)]
+dec = P.Decoder()
+for k in range(3):  # the first batch outgrows the first capacities; then 3 frames; then 1 again
+    jobs = [(len(packet), dp.ptr, o.ptr, 0) for o in outs[:(1, 3, 1)[k]]]
+    errs = dec.decode_device_batch(jobs)
+    assert errs == [None] * len(jobs), errs
+np.save(sys.argv[3], np.stack([o.download(n * 2, dtype=np.uint16) for o in outs]))
+"""
+
+
+def test_device_plan_grows_its_workspace_in_a_fresh_process(tmp_path):
+    """The first device-planned batch of a process is planned into first
+    capacities; a 1920x1080 Proxy frame of one-macroblock slices (8,160
+    slices) outgrows them, so the batch does no work, the workspace grows to
+    the batch's totals and it runs again. Every frame equals the oracle's."""
+    import os
+    import subprocess
+    import sys
+
+    f = O.test_frame(1920, 1080, 2, 10, 12, 3)
+    packet = O.encode(f, O.config(0, log2_slice_mbs=0))
+    want = O.decode(packet).data
+    path, out = tmp_path / "p.bin", tmp_path / "out.npy"
+    path.write_bytes(packet)
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "rivet-prores_amd")
+    print("case: fresh-process device plan over the first capacities", flush=True)
+    r = subprocess.run([sys.executable, "-c", _FRESH_PLAN, pkg, str(path), str(out)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = np.load(out)
+    for k in range(3):
+        np.testing.assert_array_equal(got[k], want)

@@ -152,3 +152,52 @@ def test_random_frames_round_trip():
         assert got == want
         d = dec.decode(got)
         np.testing.assert_array_equal(d.data, O.decode(want).data)
+
+
+def test_malformed_packets_planned_on_the_device():
+    """Every fuzz case as a job whose packet is only in device memory: the
+    frame and picture headers are parsed and the batch planned on the device
+    (pd_plan), 128 jobs a batch. Each job's Result must be the oracle's,
+    message included; a job whose header the host cannot parse gets no output
+    planes, so a header the device wrongly accepted would fail as a missing
+    plane, not write anywhere."""
+    dec = P.Decoder()
+    items = []
+    for data in cases(seed=31) + cases(seed=47)[:600]:
+        try:
+            h = P.FrameHeader.parse(data)
+        except P.Error:
+            h = None
+        if h is not None and h.width * h.height > 1 << 20:
+            continue
+        items.append((data, h))
+    kinds = {}
+    for b0 in range(0, len(items), 128):
+        part = items[b0:b0 + 128]
+        bufs, jobs = [], []
+        for data, h in part:
+            dp = P.DeviceBuffer(len(data) + 128)
+            if data:
+                dp.upload(np.frombuffer(data, np.uint8))
+            dd = da = None
+            n = 0
+            if h is not None:
+                n = sum(w * hh for _, w, hh in P.layout(h.width, h.height, h.chroma))
+                dd = P.DeviceBuffer(n * 2 + 64)
+                da = P.DeviceBuffer(h.width * h.height * 2 + 64) if h.alpha else None
+            bufs.append((dp, dd, da, n, h))
+            jobs.append((len(data), dp.ptr, dd.ptr if dd else 0, da.ptr if da else 0))
+        print("case: device-planned fuzz batch from", b0, flush=True)
+        errs = dec.decode_device_batch(jobs)
+        for i, ((data, h), e, (dp, dd, da, n, _)) in enumerate(zip(part, errs, bufs)):
+            want = O.decode_result(data)
+            kinds[want[0]] = kinds.get(want[0], 0) + 1
+            if want[0] != "Ok":
+                assert e is not None and (e.kind, e.msg) == (want[0], want[1]), (b0 + i, len(data), e, want[:2])
+                continue
+            assert e is None, (b0 + i, e)
+            np.testing.assert_array_equal(dd.download(n * 2, dtype=np.uint16), want[1].data)
+            if want[1].alpha is not None:
+                np.testing.assert_array_equal(da.download(h.width * h.height * 2, dtype=np.uint16), want[1].alpha)
+    print("results by kind:", kinds, flush=True)
+    assert kinds.get("Ok", 0) > 0 and kinds.get("Invalid", 0) > 0 and kinds.get("Unsupported", 0) > 0
