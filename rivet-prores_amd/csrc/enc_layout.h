@@ -20,7 +20,12 @@ struct EncFrame {
   uint32_t rescale_identity, max_in;
   uint64_t rescale_m;     // frame depth -> alpha bits
   uint32_t first_pic, n_pics;
-  uint32_t hdr_len, hdr_off;  // frame header bytes in the header table
+  uint32_t hdr_len;       // bytes of the frame header (with the frame_size and 'icpf' prefix)
+  // FrameHeader fields pe_pack writes (src/header.rs:174-199).
+  uint8_t version, chroma_interlace, aspect_rate, alpha_type;
+  uint8_t encoder_id[4];
+  uint8_t primaries, transfer, matrix, matrix_flags;
+  uint8_t luma_matrix[64], chroma_matrix[64];
   float in_scale;         // 512 / 2^bit_depth
   uint32_t first_slice;   // global index of the frame's first slice
 };

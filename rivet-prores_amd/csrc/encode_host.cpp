@@ -78,23 +78,6 @@ Err validate_frame(const prores_frame &f) {  // Frame::validate, src/frame.rs:29
   return Err{};
 }
 
-void write_frame_header(const Header &hd, std::vector<uint8_t> &out) {  // src/header.rs:174-199
-  const prores_frame_header &h = hd.h;
-  const uint32_t size = 20 + 64 * ((uint32_t)h.has_luma_matrix + h.has_chroma_matrix);
-  const uint8_t fixed[28] = {0, 0, 0, 0, 'i', 'c', 'p', 'f', (uint8_t)(size >> 8), (uint8_t)size, 0,
-                             h.bitstream_version, h.encoder_identifier[0], h.encoder_identifier[1],
-                             h.encoder_identifier[2], h.encoder_identifier[3], (uint8_t)(h.width >> 8),
-                             (uint8_t)h.width, (uint8_t)(h.height >> 8), (uint8_t)h.height,
-                             (uint8_t)((h.chroma << 6) | (h.interlace << 2)),
-                             (uint8_t)((h.metadata.aspect_ratio << 4) | (h.metadata.frame_rate_code & 15)),
-                             h.metadata.color_primaries, h.metadata.transfer_characteristic,
-                             h.metadata.matrix_coefficients, (uint8_t)h.alpha, 0,
-                             (uint8_t)((h.has_luma_matrix << 1) | h.has_chroma_matrix)};
-  out.insert(out.end(), fixed, fixed + 28);
-  if (h.has_luma_matrix) out.insert(out.end(), h.luma_matrix, h.luma_matrix + 64);
-  if (h.has_chroma_matrix) out.insert(out.end(), h.chroma_matrix, h.chroma_matrix + 64);
-}
-
 struct EncBatch {
   std::vector<EncFrame> frames;
   std::vector<EncPic> pics;
@@ -103,7 +86,6 @@ struct EncBatch {
   uint64_t n_slices = 0;
   uint32_t max_pic_slices = 0;
   std::vector<EncState> states;
-  std::vector<uint8_t> hdrs;
   // The distinct quantiser-table keys of the batch (luma and chroma
   // weights, interlaced); a picture's qtab names one until run_encode maps
   // it to the table's slot in the lane's device table cache.
@@ -210,9 +192,19 @@ Err plan_encode(const prores_config &cfg, const prores_frame &f, uint64_t dev_da
   }
   F.max_in = 0xFFFFu >> (16 - f.bit_depth);
   F.in_scale = 512.0f / (float)(1u << f.bit_depth);
-  F.hdr_off = (uint32_t)b.hdrs.size();
-  write_frame_header(hd, b.hdrs);
-  F.hdr_len = (uint32_t)(b.hdrs.size() - F.hdr_off);
+  // The frame header's fields; pe_pack writes its bytes.
+  F.hdr_len = 28 + 64 * ((uint32_t)h.has_luma_matrix + h.has_chroma_matrix);
+  F.version = h.bitstream_version;
+  memcpy(F.encoder_id, h.encoder_identifier, 4);
+  F.chroma_interlace = (uint8_t)((h.chroma << 6) | (h.interlace << 2));
+  F.aspect_rate = (uint8_t)((h.metadata.aspect_ratio << 4) | (h.metadata.frame_rate_code & 15));
+  F.primaries = h.metadata.color_primaries;
+  F.transfer = h.metadata.transfer_characteristic;
+  F.matrix = h.metadata.matrix_coefficients;
+  F.alpha_type = (uint8_t)h.alpha;
+  F.matrix_flags = (uint8_t)((h.has_luma_matrix << 1) | h.has_chroma_matrix);
+  memcpy(F.luma_matrix, h.luma_matrix, 64);
+  memcpy(F.chroma_matrix, h.chroma_matrix, 64);
   const uint64_t target = cfg.target_frame_bytes ? cfg.target_frame_bytes : target_frame_bytes(cfg.profile, f.width, f.height);
   const uint32_t npics = hd.picture_count();
   const uint64_t per_picture = (target > F.hdr_len ? target - F.hdr_len : 0) / npics;
@@ -307,7 +299,7 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   const size_t o_frames = part(b.frames.size() * sizeof(EncFrame)), o_pics = part(b.pics.size() * sizeof(EncPic)),
                o_slots = part(b.slots.size() * sizeof(EncRowSlot)),
                o_states = part(b.states.size() * sizeof(EncState)),
-               o_hdrs = part(b.hdrs.size()), plan_bytes = at;
+               plan_bytes = at;
   // The batch's quantiser tables in the lane's device cache: keys not there
   // get the next slots (the arena grows by clearing the cache: every table
   // of the batch is then uploaded again) and go up with the plan.
@@ -343,7 +335,6 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
     put(o_pics, b.pics.data(), b.pics.size() * sizeof(EncPic));
     put(o_slots, b.slots.data(), b.slots.size() * sizeof(EncRowSlot));
     put(o_states, b.states.data(), b.states.size() * sizeof(EncState));
-    put(o_hdrs, b.hdrs.data(), b.hdrs.size());
     for (size_t i = 0; i < fresh.size(); i++) quantiser_values(b.qkeys[fresh[i]], (float *)(h + o_fresh) + i * QTAB_FLOATS);
     if (!(e = check(hipMemcpyAsync(ws.plan.p, ws.hplan.p, plan_bytes, hipMemcpyHostToDevice, L.stream),
                     "hipMemcpyAsync(encode plan)")).ok())
@@ -371,7 +362,7 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   uint8_t *const pl = (uint8_t *)ws.plan.p;
   void *frames = pl + o_frames, *pics = pl + o_pics, *slots = pl + o_slots, *states = pl + o_states;
   void *slices = ws.slices.p;
-  void *qtabs = ws.qarena.p, *hdrs = pl + o_hdrs, *coefs = ws.coefs.p, *ascr = ws.alpha_scratch.p;
+  void *qtabs = ws.qarena.p, *coefs = ws.coefs.p, *ascr = ws.alpha_scratch.p;
   void *alen = ws.alpha_len.p, *stab = ws.size_tab.p, *used = ws.used.p, *qidx = ws.qidx.p;
   void *sout = ws.slice_out.p, *fsize = ws.frame_size.p, *tables = d.tables, *deep = ws.deep.p;
   void *psz = ws.probe_sizes.p, *cbnd = ws.chunk_bounds.p;
@@ -434,7 +425,7 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
     void *aargs[] = {&fsize, &nf, &base, &capacity, &oaddr, &oflow};
     if (!(x = launch(L, d.k_arena, 1, 1, 256, aargs)).ok()) return x;
     void *args[] = {&frames, &pics,  &slices, &coefs, &qtabs, &alen,  &ascr,
-                    &used,   &qidx,  &sout,   &fsize, &hdrs,  &oaddr, &oflow};
+                    &used,   &qidx,  &sout,   &fsize, &oaddr, &oflow};
     if (!(x = launch(L, d.k_pack, g32, 1, 64, args)).ok()) return x;
     return check(hipMemcpyAsync(ws.hres.p, fsize, NF * 8, hipMemcpyDeviceToHost, L.stream), "frame sizes");
   };

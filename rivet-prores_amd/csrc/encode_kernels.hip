@@ -932,6 +932,42 @@ __device__ __forceinline__ void store_word(GLOBAL uint8_t *A, uint32_t w, uint32
 }
 }  // namespace
 
+// Byte b of the frame: frame_size and 'icpf', then FrameHeader::write
+// (src/header.rs:174-199): header size, version, encoder identifier, width,
+// height, chroma format and interlace mode, aspect ratio and frame rate,
+// colour primaries, transfer characteristic, matrix coefficients, alpha
+// channel type, the matrix flags, then the loaded luma and chroma matrices.
+__device__ __forceinline__ uint8_t frame_header_byte(const EncFrame &F, uint32_t frame_size, uint32_t b) {
+  if (b < 4) return (uint8_t)(frame_size >> (24 - 8 * b));
+  if (b >= 28) {
+    const uint32_t m = b - 28;
+    return (F.matrix_flags & 2) ? (m < 64 ? F.luma_matrix[m] : F.chroma_matrix[m - 64]) : F.chroma_matrix[m];
+  }
+  const uint32_t header_size = F.hdr_len - 8;
+  switch (b) {
+    case 4: return 'i';
+    case 5: return 'c';
+    case 6: return 'p';
+    case 7: return 'f';
+    case 8: return (uint8_t)(header_size >> 8);
+    case 9: return (uint8_t)header_size;
+    case 11: return F.version;
+    case 12: case 13: case 14: case 15: return F.encoder_id[b - 12];
+    case 16: return (uint8_t)(F.width >> 8);
+    case 17: return (uint8_t)F.width;
+    case 18: return (uint8_t)(F.height >> 8);
+    case 19: return (uint8_t)F.height;
+    case 20: return F.chroma_interlace;
+    case 21: return F.aspect_rate;
+    case 22: return F.primaries;
+    case 23: return F.transfer;
+    case 24: return F.matrix;
+    case 25: return F.alpha_type;
+    case 27: return F.matrix_flags;
+    default: return 0;  // bytes 10 and 26 are reserved
+  }
+}
+
 // One wavefront per slice: slice header, Y', Cb, Cr codewords (src/encode.rs:
 // 413-437, 796-819) and the alpha bytes; the slice-table entry; the picture
 // header for the first slice of a picture and the frame header for the first
@@ -942,8 +978,8 @@ extern "C" __global__ void __launch_bounds__(64)
             const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
             const uint8_t *__restrict__ alpha_scratch, const uint32_t *__restrict__ used,
             const uint32_t *__restrict__ qidx, const uint64_t *__restrict__ slice_out,
-            const uint64_t *__restrict__ frame_size, const uint8_t *__restrict__ hdr_table,
-            const uint64_t *__restrict__ out_addr, const uint32_t *__restrict__ overflow) {
+            const uint64_t *__restrict__ frame_size, const uint64_t *__restrict__ out_addr,
+            const uint32_t *__restrict__ overflow) {
   __shared__ uint32_t ring[RING];
   if (*overflow) return;  // the arena is too small: the host grows it and packs again
   const uint32_t g = blockIdx.x, L = threadIdx.x;
@@ -978,11 +1014,7 @@ extern "C" __global__ void __launch_bounds__(64)
     // The frame header (src/header.rs:174-199) with frame_size patched, and
     // the header of any picture without slices (a one-row frame's empty field).
     const uint32_t fs = (uint32_t)frame_size[P.frame];
-    for (uint32_t b = L; b < F.hdr_len; b += 64) {
-      uint8_t v = hdr_table[F.hdr_off + b];
-      if (b < 4) v = (uint8_t)(fs >> (24 - 8 * b));
-      out[b] = v;
-    }
+    for (uint32_t b = L; b < F.hdr_len; b += 64) out[b] = frame_header_byte(F, fs, b);
     if (L == 0)
       for (uint32_t p = F.first_pic; p < F.first_pic + F.n_pics; p++) {
         if (pics[p].n_slices) continue;
