@@ -200,6 +200,33 @@ typedef struct {
 prores_status prores_decode_device_batch(const prores_decoder *d, size_t n,
                                          const prores_dev_decode_job *jobs, prores_error *errs);
 
+/* ---- Allocation-free decode ----
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory while they run: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.
+ *
+ * prores_noalloc_init: set-up, once per device, with that device current:
+ * loads the library's kernels, constant tables and stream there and looks up
+ * that device's copy of the library's scratch array
+ * (prores_noalloc_scratch_bytes(): 256 MiB, held by every GPU the library
+ * runs on until the process ends).
+ *
+ * prores_decode_device_batch_noalloc: prores_decode_device_batch with the
+ * batch planned on the device (each job's packet is read from dev_packet;
+ * host_packet is not read) and its whole workspace in the scratch array: the
+ * call allocates no device or page-locked memory and captures no graph.
+ * Samples and errors are those of prores_decode_device_batch. A batch whose
+ * workspace does not fit the scratch (for instance more than one 3840x2160
+ * 4:4:4 frame) fails, every job with PRORES_ERR_CONFIG "the batch needs more
+ * than the 268435456 bytes of its scratch memory"; so does one of more than
+ * 2048 jobs, and a call on a device where prores_noalloc_init has not run.
+ * Calls on one device share its scratch and run one at a time; the scratch
+ * keeps the last call's contents. */
+prores_status prores_noalloc_init(prores_error *err);
+size_t prores_noalloc_scratch_bytes(void);
+prores_status prores_decode_device_batch_noalloc(const prores_decoder *d, size_t n,
+                                                 const prores_dev_decode_job *jobs, prores_error *errs);
+
 /* How a decode batch is launched. Every setting gives the same samples and
  * the same errors; only the speed differs. No counterpart in the reference.
  *   persistent   0: one workgroup (entropy: one wave) per work item;

@@ -33,6 +33,11 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wextra", "-Wn
 
 MODULES = [("decode", "decode_kernels.hip"), ("encode", "encode_kernels.hip")]
 HOST_SOURCES = ["host_common.cpp", "decode_host.cpp", "encode_host.cpp", "serve_host.cpp"]
+# Host code with a device variable of its own (the allocation-free entry
+# point's scratch array): compiled by hipcc as one HIP source.
+HIP_HOST_SOURCES = ["noalloc.hip"]
+HIP_HOST_FLAGS = ["-c", "-fPIC", f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-ffp-contract=off",
+                  "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 
 
 def _run(cmd, verbose):
@@ -64,6 +69,10 @@ def build(verbose=False):
     for src in HOST_SOURCES:
         o = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++", *HOST_FLAGS, "-c", "-o", o, os.path.join(CSRC, src)], verbose)
+        objs.append(o)
+    for src in HIP_HOST_SOURCES:
+        o = os.path.join(BUILD, src.replace(".hip", ".o"))
+        _run([f"{ROCM}/bin/hipcc", *HIP_HOST_FLAGS, "-I", CSRC, "-o", o, os.path.join(CSRC, src)], verbose)
         objs.append(o)
     tmp = LIB + ".tmp"
     _run(["g++", "-shared", "-pthread", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"],

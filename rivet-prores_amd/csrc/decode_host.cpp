@@ -350,9 +350,10 @@ Err enqueue_sequence(Device &d, Lane &L, Launch &A, bool timed) {
 }
 
 // Launches the sequence (replaying a captured graph when the tuning asks
-// for one), copies `res_bytes` of results from `res` into the lane's
-// page-locked result buffer, waits, and records the stage times.
-Err run_sequence(Device &d, Lane &L, Launch &A, const prores_decode_tuning &t, const void *res, size_t res_bytes) {
+// for one), copies `res_bytes` of results from `res` to `host`, waits, and
+// records the stage times.
+Err run_sequence(Device &d, Lane &L, Launch &A, const prores_decode_tuning &t, const void *res, size_t res_bytes,
+                 void *host) {
   Workspace &ws = workspace(L);
   Err e;
   // A capture that fails (another thread's work invalidated it, or the
@@ -411,8 +412,7 @@ Err run_sequence(Device &d, Lane &L, Launch &A, const prores_decode_tuning &t, c
     if (!(e = enqueue_sequence(d, L, A, true)).ok()) return e;
     (void)hipEventRecord(L.ev[4], L.stream);
   }
-  if (!(e = ws.hres.reserve(res_bytes + 64)).ok()) return e;
-  if (res_bytes && !(e = check(hipMemcpyAsync(ws.hres.p, res, res_bytes, hipMemcpyDeviceToHost, L.stream),
+  if (res_bytes && !(e = check(hipMemcpyAsync(host, res, res_bytes, hipMemcpyDeviceToHost, L.stream),
                                "hipMemcpyAsync(results)")).ok())
     return e;
   if (!(e = check(hipStreamSynchronize(L.stream), "hipStreamSynchronize")).ok()) return e;
@@ -588,7 +588,8 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   }
   if (np == 0) {
     L.dec_us[0] = L.dec_us[1] = L.dec_us[2] = L.dec_us[3] = L.dec_us[4] = 0;
-  } else if (!(e = run_sequence(d, L, A, t, ws.picres.p, 2 * ws.cap_np)).ok()) {
+  } else if (!(e = ws.hres.reserve(2 * ws.cap_np + 64)).ok() ||
+             !(e = run_sequence(d, L, A, t, ws.picres.p, 2 * ws.cap_np, ws.hres.p)).ok()) {
     return e;
   }
   const uint32_t *st = (const uint32_t *)ws.hres.p, *fe = st + ws.cap_np / 4;
@@ -646,41 +647,76 @@ Err plan_error(uint32_t code, const uint64_t *a, uint32_t darg) {
 // (each job's record, the batch totals, the picture status words) come back
 // in one copy. The workspace is laid out for the capacities of the largest
 // batch seen on the lane; a batch over them does no work and runs again over
-// a workspace grown to its totals.
+// a workspace grown to its totals. With `fixed`, the workspace is carved from
+// that region instead: nothing is allocated, the first capacities split the
+// region, and a batch whose totals do not fit it fails.
 Err run_device_batch(Device &d, Lane &L, uint32_t bit_depth, size_t n, const prores_dev_decode_job *jobs,
-                     const prores_decode_tuning &t, std::vector<Err> &results) {
+                     const prores_decode_tuning &t, std::vector<Err> &results, FixedRegion *fixed = nullptr) {
   Workspace &ws = workspace(L);
   Err e;
   if (!(e = validate_tuning(t)).ok()) return e;
   if (n > (1u << 24)) return invalid("a batch of more than 2^24 frames");
   results.assign(n, Err{});
   if (!n) return Err{};
-  // Jobs up, from the page-locked plan staging.
-  if (!(e = ws.hplan.reserve(n * sizeof(DevJob) + 64)).ok() || !(e = ws.jobs.reserve(n * sizeof(DevJob) + 64)).ok())
+  // Device buffers: the lane's (grown as needed) or carved from the region.
+  auto need = [&](DevBuf &b, size_t bytes) -> Err {
+    if (!fixed) return b.reserve(bytes);
+    if (fixed->take(b, bytes)) return Err{};
+    return Err{PRORES_ERR_CONFIG, fmtstr("the batch needs more than the %zu bytes of its scratch memory",
+                                         fixed->dev_bytes)};
+  };
+  const size_t job_bytes = n * sizeof(DevJob);
+  if (fixed) {
+    fixed->dev_used = 0;
+    if (job_bytes > fixed->host_bytes) return config_error("too many jobs for the scratch memory's host staging");
+  } else if (!(e = ws.hplan.reserve(job_bytes + 64)).ok()) {
     return e;
+  }
+  if (!(e = need(ws.jobs, job_bytes + 64)).ok()) return e;
+  const size_t jobs_used = fixed ? fixed->dev_used : 0;
+  // Jobs up, from the page-locked plan staging (or the region's host part).
   {
-    DevJob *hj = (DevJob *)ws.hplan.p;
+    DevJob *hj = (DevJob *)(fixed ? (void *)fixed->host : ws.hplan.p);
     for (size_t i = 0; i < n; i++)
       hj[i] = DevJob{(uint64_t)jobs[i].dev_packet, jobs[i].len, (uint64_t)jobs[i].dev_data, (uint64_t)jobs[i].dev_alpha};
-    if (!(e = check(hipMemcpyAsync(ws.jobs.p, ws.hplan.p, n * sizeof(DevJob), hipMemcpyHostToDevice, L.stream),
+    if (!(e = check(hipMemcpyAsync(ws.jobs.p, hj, job_bytes, hipMemcpyHostToDevice, L.stream),
                     "hipMemcpyAsync(jobs)")).ok())
       return e;
   }
   // First capacities: a 4:4:4 frame of 2160p's slices at 8 macroblocks a
-  // job; the totals of a batch over them set the next.
+  // job; the totals of a batch over them set the next. In a fixed region
+  // the first capacities fill it: a slice's records and its share of
+  // coefficients (1,024 a slice: one 4:2:2 macroblock) from what is left
+  // after the per-job parts.
   PlanCaps &c = ws.dcap;
-  if (!c.slices) {
+  const uint32_t np = (uint32_t)(2 * n);
+  if (fixed) {
+    const size_t per_job = 2 * sizeof(DecPic) + sizeof(DecFrame) + sizeof(PlanJob) + sizeof(PlanRec) + 64 * 8 +
+                           4 * 256 + 16;
+    const size_t fixed_part = n * per_job + (size_t)ORDER_MAX * 4 + 16 * 4096;
+    const size_t per_slice = sizeof(SliceRec) + 16 + sizeof(AlphaRec) + 2 * sizeof(IdctItem) + sizeof(RowSlot) + 8 + 1 +
+                             1024 * 6;
+    const size_t room = fixed->dev_bytes > fixed_part + jobs_used ? fixed->dev_bytes - fixed_part - jobs_used : 0;
+    c = PlanCaps{};
+    c.slices = (uint32_t)std::min<size_t>(room / per_slice, 0xFFFFFF00u);
+    c.coefs = (uint64_t)c.slices * 1024;
+    c.rows = c.segs = c.items = c.slices;
+    c.chunks = c.slices / ST_CHUNK + np;
+  } else if (!c.slices) {
     c.slices = (uint32_t)std::min<uint64_t>(4096ull * n, 0xFFFFFF00u);
     c.coefs = 32400ull * 768 * n;
   }
-  const uint32_t np = (uint32_t)(2 * n);
   PlanTotals T;
   for (int attempt = 0;; attempt++) {
     c.pics = np;
-    c.rows = std::max<uint32_t>(c.rows, 64 * np);
-    c.segs = std::max<uint32_t>(c.segs, c.rows);
-    c.chunks = std::max<uint32_t>(c.chunks, c.slices / ST_CHUNK + np);
-    c.items = std::max<uint32_t>(c.items, c.slices);
+    if (fixed) {
+      fixed->dev_used = jobs_used;
+    } else {
+      c.rows = std::max<uint32_t>(c.rows, 64 * np);
+      c.segs = std::max<uint32_t>(c.segs, c.rows);
+      c.chunks = std::max<uint32_t>(c.chunks, c.slices / ST_CHUNK + np);
+      c.items = std::max<uint32_t>(c.items, c.slices);
+    }
     c.item_slices = ws.idct_cap_slices;
     // The plan buffer: frames (one a job), pictures, row slots, segments,
     // chunks, item costs, header.
@@ -693,14 +729,17 @@ Err run_device_batch(Device &d, Lane &L, uint32_t bit_depth, size_t n, const pro
     // Results: job records, totals, picture status, first errors.
     const size_t r_tot = al(n * sizeof(PlanRec)), r_st = r_tot + al(sizeof(PlanTotals)), r_fe = r_st + al(np * 4),
                  res_bytes = r_fe + np * 4;
-    if (!(e = ws.plan.reserve(plan_bytes)).ok() || !(e = ws.dres.reserve(res_bytes + 64)).ok() ||
-        !(e = ws.scratch.reserve(n * sizeof(PlanJob) + 64)).ok() ||
-        !(e = ws.items.reserve((size_t)c.items * sizeof(IdctItem) + 64)).ok() ||
-        !(e = ws.slices.reserve((size_t)c.slices * sizeof(SliceRec) + 64)).ok() ||
-        !(e = ws.slice_err.reserve((size_t)c.slices * 16 + 64)).ok() ||
-        !(e = ws.coefs.reserve(c.coefs * 2 + 64)).ok() || !(e = ws.wide.reserve(c.coefs * 4 + 64)).ok() ||
-        !(e = ws.alpha_order.reserve((size_t)c.slices * sizeof(AlphaRec) + 64)).ok())
+    if (fixed && res_bytes > fixed->host_bytes) return config_error("too many jobs for the scratch memory's host staging");
+    if (!(e = need(ws.plan, plan_bytes)).ok() || !(e = need(ws.dres, res_bytes + 64)).ok() ||
+        !(e = need(ws.scratch, n * sizeof(PlanJob) + 64)).ok() ||
+        !(e = need(ws.items, (size_t)c.items * sizeof(IdctItem) + 64)).ok() ||
+        !(e = need(ws.slices, (size_t)c.slices * sizeof(SliceRec) + 64)).ok() ||
+        !(e = need(ws.slice_err, (size_t)c.slices * 16 + 64)).ok() || !(e = need(ws.coefs, c.coefs * 2 + 64)).ok() ||
+        !(e = need(ws.wide, c.coefs * 4 + 64)).ok() ||
+        !(e = need(ws.alpha_order, (size_t)c.slices * sizeof(AlphaRec) + 64)).ok())
       return e;
+    if (!fixed && !(e = ws.hres.reserve(res_bytes + 64)).ok()) return e;
+    void *const hres = fixed ? (void *)fixed->host : ws.hres.p;
     Launch A;
     uint8_t *const pl = (uint8_t *)ws.plan.p, *rs = (uint8_t *)ws.dres.p;
     A.device_plan = true;
@@ -733,17 +772,27 @@ Err run_device_batch(Device &d, Lane &L, uint32_t bit_depth, size_t n, const pro
     launch_shape(d, t, c.slices, c.chunks, c.items, c.item_slices, A);
     const uint32_t slots = t.persistent ? 3 * ((c.slices + 63) / 64) : A.n_slots;
     if (slots <= ORDER_MAX) {
-      if (!(e = ws.order.reserve((size_t)ORDER_MAX * 4 + 64)).ok()) return e;
+      if (!(e = need(ws.order, (size_t)ORDER_MAX * 4 + 64)).ok()) return e;
       A.order = ws.order.p;
     }
-    if (!(e = run_sequence(d, L, A, t, ws.dres.p, res_bytes)).ok()) return e;
-    const uint8_t *h = (const uint8_t *)ws.hres.p;
+    if (!(e = run_sequence(d, L, A, t, ws.dres.p, res_bytes, hres)).ok()) return e;
+    const uint8_t *h = (const uint8_t *)hres;
     memcpy(&T, h + r_tot, sizeof T);
     // The next batches' IDCT items may hold as many slices as this one's
     // wanted (the dynamic LDS is sized for it).
     ws.idct_cap_slices = std::max<uint32_t>(std::min<uint32_t>(T.wanted_count, IDCT_MAX_SLICES), 1);
     if (T.overflow) {
       if (attempt) return Err{PRORES_ERR_DEVICE, "a device-planned batch outgrew the workspace sized for it"};
+      if (fixed) {
+        // The batch's own totals, if they fit the region.
+        c.slices = T.slices;
+        c.rows = T.rows;
+        c.segs = T.segs;
+        c.chunks = T.chunks;
+        c.items = T.items;
+        c.coefs = T.coefs;
+        continue;
+      }
       if (T.slices > 0xFFFFFFF0u || T.chunks > 0xFFFFFFF0u) return invalid("a batch of more than 2^32 slices");
       auto grow = [](uint32_t v) { return (uint32_t)std::min<uint64_t>((uint64_t)v + v / 4 + 64, 0xFFFFFFF0u); };
       c.slices = std::max(c.slices, grow(T.slices));
@@ -1015,6 +1064,22 @@ Err decode_batch_on(Device &d, Lane &L, uint32_t bit_depth, size_t n, const pror
   }
   for (size_t i = 0; i < n; i++) fill_error(&errs[i], results[i]);
   return Err{};
+}
+
+void *decode_workspace(Lane &L) { return &workspace(L); }
+
+Err decode_batch_fixed(Device &d, Lane &L, const prores_decoder *dec, size_t n, const prores_dev_decode_job *jobs,
+                       FixedRegion &region, prores_error *errs) {
+  prores_decode_tuning t;
+  {
+    std::lock_guard<std::mutex> g(d.mu);
+    t = d.tuning;
+  }
+  t.graph = 0;  // instantiating a graph allocates
+  std::vector<Err> results;
+  Err e = run_device_batch(d, L, dec->bit_depth, n, jobs, t, results, &region);
+  for (size_t i = 0; i < n; i++) fill_error(&errs[i], e.ok() ? results[i] : e);
+  return e;
 }
 
 }  // namespace prores_hip

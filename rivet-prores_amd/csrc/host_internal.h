@@ -155,6 +155,36 @@ void build_tables(DevTables &t);
 Err decode_batch_on(Device &d, Lane &L, uint32_t bit_depth, size_t n, const prores_dev_decode_job *jobs,
                     const prores_decode_tuning *t, prores_error *errs);
 Err encode_batch_on(Device &d, Lane &L, size_t n, prores_dev_encode_job *jobs, prores_error *errs);
+// A device-planned decode batch whose workspace is carved from `region`
+// (device memory and host staging the caller owns) instead of the lane's
+// buffers: it allocates no device or page-locked memory, and replays no
+// graph. The caller holds L.mu.
+// Memory a device-planned batch lays its workspace out in instead of the
+// lane's own buffers (the allocation-free entry point): device bytes from
+// `dev`, host staging from `host`, carved afresh for each attempt.
+struct FixedRegion {
+  uint8_t *dev = nullptr;
+  size_t dev_bytes = 0, dev_used = 0;
+  uint8_t *host = nullptr;
+  size_t host_bytes = 0;
+  // Points `b` at the next `bytes` of the region (256-byte aligned); false
+  // when they do not fit.
+  bool take(DevBuf &b, size_t bytes) {
+    const size_t at = (dev_used + 255) & ~(size_t)255;
+    if (at > dev_bytes || bytes > dev_bytes - at) return false;
+    b.p = dev + at;
+    b.cap = bytes;
+    dev_used = at + bytes;
+    return true;
+  }
+};
+Err decode_batch_fixed(Device &d, Lane &L, const prores_decoder *dec, size_t n, const prores_dev_decode_job *jobs,
+                       FixedRegion &region, prores_error *errs);
+// The lane the allocation-free entry point runs on, and its decode
+// workspace record (made on first use).
+constexpr int NOALLOC_LANE = 5;
+void *decode_workspace(Lane &L);
+
 // The Config and Frame checks of Encoder::encode, without device work.
 Err encode_check(const prores_config &cfg, const prores_frame &frame);
 

@@ -153,7 +153,8 @@ EXPORTS = [
     "prores_server_submit_decode", "prores_server_submit_encode", "prores_server_poll",
     "prores_server_release", "prores_server_get_stats", "prores_last_encode_counters",
     "prores_entropy_tables", "prores_memcpy_error", "prores_decode_host_batch", "prores_find_frames",
-    "prores_fnv1a", "prores_graph_counters", "prores_last_encode_stages",
+    "prores_fnv1a", "prores_graph_counters", "prores_last_encode_stages", "prores_noalloc_init",
+    "prores_noalloc_scratch_bytes", "prores_decode_device_batch_noalloc",
 ]
 
 _lib = None
@@ -210,6 +211,9 @@ def lib():
         L.prores_last_encode_counters.argtypes = [P(C.c_uint64)]
         L.prores_graph_counters.argtypes = [P(C.c_uint64)]
         L.prores_last_encode_stages.argtypes = [P(C.c_double)]
+        L.prores_noalloc_init.argtypes = [P(_Err)]
+        L.prores_noalloc_scratch_bytes.restype = C.c_size_t
+        L.prores_decode_device_batch_noalloc.argtypes = [C.c_void_p, C.c_size_t, P(_DevDecodeJob), P(_Err)]
         L.prores_version.restype = C.c_char_p
         L.prores_find_frames.argtypes = [C.c_char_p, C.c_size_t, P(C.c_uint64), P(C.c_uint64), C.c_size_t]
         L.prores_find_frames.restype = C.c_size_t
@@ -501,6 +505,19 @@ class Decoder:
             return [None] * jobs.n
         return [None if e.kind == 0 else _KINDS.get(e.kind, Error)(e.msg.decode()) for e in jobs.errs]
 
+    def decode_device_batch_noalloc(self, jobs):
+        """decode_device_batch through the allocation-free entry point
+        (prores_decode_device_batch_noalloc): the batch is planned on the
+        device and its workspace is the library's scratch array; call
+        noalloc_init() on the device first. Returns one Error or None per
+        job."""
+        if not isinstance(jobs, DecodeJobs):
+            jobs = DecodeJobs(jobs)
+        rc = lib().prores_decode_device_batch_noalloc(self._h, jobs.n, jobs.arr, jobs.errs)
+        if rc == 4:
+            _raise(jobs.errs[0])
+        return [None if e.kind == 0 else _KINDS.get(e.kind, Error)(e.msg.decode()) for e in jobs.errs]
+
     def autotune(self, jobs, reps=3):
         """Times every candidate DecodeTuning on this batch and makes the
         fastest the device's batch tuning; returns (tuning, median us)."""
@@ -554,6 +571,17 @@ class DecodeTuning:
     @classmethod
     def _from(cls, t):
         return cls(t.persistent, t.chunk, t.idct_blocks, t.entropy_wg, t.graph)
+
+
+def noalloc_init():
+    """Set-up of the allocation-free entry point on the current device."""
+    err = _Err()
+    if lib().prores_noalloc_init(C.byref(err)):
+        _raise(err)
+
+
+def noalloc_scratch_bytes():
+    return int(lib().prores_noalloc_scratch_bytes())
 
 
 def set_decode_tuning(tuning):
