@@ -878,6 +878,19 @@ PinnedPool &pinned() {
   return *p;
 }
 
+}  // namespace
+
+void *host_staging_get(size_t bytes) {
+  void *p = pinned().get(bytes);
+  return p ? p : malloc(bytes ? bytes : 1);
+}
+
+void host_staging_put(void *p) {
+  if (!pinned().put(p)) free(p);
+}
+
+namespace {
+
 // A decoded frame's fields from its header, and its planes allocated from
 // the pinned pool (malloc when none can be pinned).
 Err frame_outputs(const prores_decoder *d, const Header &hd, prores_frame *out, bool allocate, size_t &total,

@@ -137,6 +137,12 @@ struct HostBuf {
   Err reserve(size_t bytes);
 };
 
+// Host memory for staging copies: page-locked from the library's reuse pool
+// (so a copy to the device runs at the link's speed), else malloc'd;
+// returned with host_staging_put either way.
+void *host_staging_get(size_t bytes);
+void host_staging_put(void *p);
+
 // `lds`: bytes of dynamic LDS per workgroup.
 Err launch(Lane &L, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args, unsigned lds = 0);
 Err launch_on(hipStream_t s, hipFunction_t f, unsigned gx, unsigned gy, unsigned bx, void **args,

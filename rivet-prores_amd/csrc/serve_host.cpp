@@ -72,15 +72,35 @@ struct Pool {
 
 enum Kind : uint32_t { DECODE = 1, ENCODE = 2 };
 
+// A host copy of a request's input, taken at submit into page-locked
+// memory where the library can pin it, so the worker's upload is one DMA at
+// the link's speed.
+struct HostCopy {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void assign(const void *src, size_t n) {
+    release();
+    p = host_staging_get(n);
+    bytes = n;
+    if (n) memcpy(p, src, n);
+  }
+  void release() {
+    if (p) host_staging_put(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  ~HostCopy() { release(); }
+};
+
 struct Request {
   uint64_t id = 0;
   Kind kind = DECODE;
   // Inputs (host copies taken at submit).
-  std::vector<uint8_t> packet;
+  HostCopy packet;
   prores_frame_header hdr;
   prores_config cfg;
-  prores_frame frame;  // metadata; data / alpha point into the vectors below
-  std::vector<uint16_t> data, alpha;
+  prores_frame frame;  // metadata; data / alpha point into the copies below
+  HostCopy data, alpha;
   // Device memory.
   size_t hold = 0;       // bytes counted against the budget while admitted
   size_t keep = 0;       // of which the outputs, held until release
@@ -177,24 +197,22 @@ void prores_server::worker(int which) {
         prores_plane pl[3];
         r->out_bytes = prores_frame_layout(r->hdr.width, r->hdr.height, r->hdr.chroma, pl) * 2;
         r->out_alpha_bytes = r->hdr.alpha ? (size_t)r->hdr.width * r->hdr.height * 2 : 0;
-        r->in_bytes = r->packet.size() + 128;  // the decoder reads up to 64 bytes past a packet
+        r->in_bytes = r->packet.bytes + 128;  // the decoder reads up to 64 bytes past a packet
         e = pool.get(r->in_bytes + 64, r->dev_in);
         if (e.ok()) e = pool.get(r->out_bytes + 64, r->dev_out);
         if (e.ok() && r->out_alpha_bytes) e = pool.get(r->out_alpha_bytes + 64, r->dev_out_alpha);
         if (e.ok())
-          e = check(hipMemcpyAsync(r->dev_in, r->packet.data(), r->packet.size(), hipMemcpyHostToDevice, L.stream),
+          e = check(hipMemcpyAsync(r->dev_in, r->packet.p, r->packet.bytes, hipMemcpyHostToDevice, L.stream),
                     "H2D packet");
       } else {
-        r->in_bytes = r->data.size() * 2;
-        r->in_alpha_bytes = r->alpha.size() * 2;
+        r->in_bytes = r->data.bytes;
+        r->in_alpha_bytes = r->alpha.bytes;
         e = pool.get(r->in_bytes + 64, r->dev_in);
         if (e.ok() && r->in_alpha_bytes) e = pool.get(r->in_alpha_bytes + 64, r->dev_in_alpha);
         if (e.ok())
-          e = check(hipMemcpyAsync(r->dev_in, r->data.data(), r->in_bytes, hipMemcpyHostToDevice, L.stream),
-                    "H2D frame");
+          e = check(hipMemcpyAsync(r->dev_in, r->data.p, r->in_bytes, hipMemcpyHostToDevice, L.stream), "H2D frame");
         if (e.ok() && r->in_alpha_bytes)
-          e = check(hipMemcpyAsync(r->dev_in_alpha, r->alpha.data(), r->in_alpha_bytes, hipMemcpyHostToDevice,
-                                   L.stream),
+          e = check(hipMemcpyAsync(r->dev_in_alpha, r->alpha.p, r->in_alpha_bytes, hipMemcpyHostToDevice, L.stream),
                     "H2D alpha");
       }
       r->err = e;
@@ -207,8 +225,9 @@ void prores_server::worker(int which) {
       if (which == 0) {
         std::vector<prores_dev_decode_job> jobs(ok.size());
         for (size_t i = 0; i < ok.size(); i++)
-          jobs[i] = prores_dev_decode_job{ok[i]->packet.data(), ok[i]->packet.size(), (const uint8_t *)ok[i]->dev_in,
-                                          (uint16_t *)ok[i]->dev_out, (uint16_t *)ok[i]->dev_out_alpha};
+          jobs[i] = prores_dev_decode_job{(const uint8_t *)ok[i]->packet.p, ok[i]->packet.bytes,
+                                          (const uint8_t *)ok[i]->dev_in, (uint16_t *)ok[i]->dev_out,
+                                          (uint16_t *)ok[i]->dev_out_alpha};
         // The device's batch tuning, launched directly: a capture on this
         // worker's streams can be invalidated by other threads of the
         // process (clients reading their results through the legacy
@@ -256,9 +275,9 @@ void prores_server::worker(int which) {
       pool.put(r->dev_in, r->in_bytes + 64);
       pool.put(r->dev_in_alpha, r->in_alpha_bytes + 64);
       r->dev_in = r->dev_in_alpha = nullptr;
-      std::vector<uint8_t>().swap(r->packet);
-      std::vector<uint16_t>().swap(r->data);
-      std::vector<uint16_t>().swap(r->alpha);
+      r->packet.release();
+      r->data.release();
+      r->alpha.release();
     }
     finish(batch);
   }
@@ -392,8 +411,8 @@ uint64_t prores_server_submit_decode(prores_server *s, const uint8_t *packet, si
   if (r->err.ok()) {
     r->hdr = h.h;
     const size_t n = std::min<size_t>(len, h.h.frame_size);
-    r->packet.assign(packet, packet + n);
-    r->hold = decode_footprint(r->hdr, r->packet.size(), r->keep);
+    r->packet.assign(packet, n);
+    r->hold = decode_footprint(r->hdr, r->packet.bytes, r->keep);
   }
   return enqueue(s, r, 0);
 }
@@ -406,10 +425,10 @@ uint64_t prores_server_submit_encode(prores_server *s, const prores_config *cfg,
   // The checks of Encoder::encode, so a refused request fails at once.
   r->err = encode_check(*cfg, *frame);
   if (r->err.ok()) {
-    r->data.assign(frame->data, frame->data + frame->data_len);
-    if (frame->alpha) r->alpha.assign(frame->alpha, frame->alpha + frame->alpha_len);
-    r->frame.data = r->data.data();
-    r->frame.alpha = frame->alpha ? r->alpha.data() : nullptr;
+    r->data.assign(frame->data, frame->data_len * 2);
+    if (frame->alpha) r->alpha.assign(frame->alpha, frame->alpha_len * 2);
+    r->frame.data = (uint16_t *)r->data.p;
+    r->frame.alpha = frame->alpha ? (uint16_t *)r->alpha.p : nullptr;
     r->hold = encode_footprint(r->cfg, r->frame, r->keep);
   }
   return enqueue(s, r, 1);
