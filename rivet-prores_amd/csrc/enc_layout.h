@@ -5,7 +5,15 @@
 namespace prores_hip {
 
 constexpr uint32_t ENC_MAXK = 8;         // window columns of the size table
-constexpr uint32_t ENC_PROBES = 9;       // qi = 224, then at most 8 bisection steps (224 < 2^8)
+// The search over qi probes 224, then bisects [1, 224] in at most 8 steps
+// (224 < 2^8). The first pass sizes every slice at the first five probes of
+// the path on which each one fits (224, 112, 56, 28, 14); each later pass at
+// the one probe pending. The first pass settles at least two probes (224 and
+// 112, or 224 alone when it does not fit), so 8 passes settle every search,
+// and at most 5 + 7 probe results are kept.
+constexpr uint32_t ENC_PASS_Q = 5;       // quantisers of the first pass
+constexpr uint32_t ENC_PASSES = 8;       // size passes
+constexpr uint32_t ENC_HIST = 12;        // probe results kept per picture
 constexpr uint32_t ENC_DEEP = 224;       // size cache columns per slice (column t: qi = uniform - t)
 constexpr uint32_t SHARE_WAVES = 8;      // waves of a pe_shareout workgroup
 constexpr uint32_t DEEP_STEP = 4;        // quantisers coded per cache fill
@@ -60,8 +68,10 @@ struct EncSlice {
 
 // The picture-level rate-control state (src/encode.rs:685-794).
 struct EncState {
-  unsigned long long sum;  // probe accumulator: total bytes at the probed qi
-  uint32_t notfit;         // probe accumulator: slices that do not fit
+  unsigned long long psum[ENC_PASS_Q];  // pass accumulators: total bytes at each listed qi
+  uint32_t pnotfit[ENC_PASS_Q];         // and the slices that do not fit at it
+  uint32_t nq;                          // quantisers listed for the next pass (0: search done)
+  uint32_t pq[ENC_PASS_Q];              // the list, coarsest first
   uint32_t alpha_total;    // alpha bytes of the picture
   uint64_t budget;         // budget after alpha and overhead
   uint32_t phase;          // 0: probing 224, 1: bisecting, 2: done
@@ -70,9 +80,9 @@ struct EncState {
   uint32_t fits_at_all, uniform;
   uint32_t window_k;       // columns of the size table: uniform, uniform-1, ...
   uint32_t fallback;       // sizes the share-out computed itself (diagnostic)
-  uint32_t round;          // probes made
-  uint32_t probe_qi[ENC_PROBES];          // the qi of each probe
-  unsigned long long probe_sum[ENC_PROBES];  // and the picture's total bytes at it
+  uint32_t round;          // probe results kept
+  uint32_t probe_qi[ENC_HIST];            // the qi of each
+  unsigned long long probe_sum[ENC_HIST];  // and the picture's total bytes at it
 };
 
 }  // namespace prores_hip

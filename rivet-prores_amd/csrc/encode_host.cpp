@@ -256,6 +256,11 @@ Err plan_encode(const prores_config &cfg, const prores_frame &f, uint64_t dev_da
     EncState st;
     memset(&st, 0, sizeof st);
     st.probe = 224;
+    // The probes of the search while every one fits.
+    const uint32_t path[5] = {224, 112, 56, 28, 14};
+    static_assert(ENC_PASS_Q <= 5, "first pass list");
+    st.nq = ENC_PASS_Q;
+    memcpy(st.pq, path, ENC_PASS_Q * sizeof(uint32_t));
     b.states.push_back(st);
   }
   b.frames.push_back(F);
@@ -351,7 +356,7 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
   if (!(e = ws.alpha_scratch.reserve(G * ALPHA_SLICE_CAP + 64)).ok()) return e;
   if (!(e = ws.alpha_len.reserve(G * 4 + 64)).ok()) return e;
   if (!(e = ws.size_tab.reserve(G * ENC_MAXK * 4 + 64)).ok()) return e;
-  if (!(e = ws.probe_sizes.reserve(G * ENC_PROBES * 4 + 64)).ok()) return e;
+  if (!(e = ws.probe_sizes.reserve(G * ENC_HIST * 4 + 64)).ok()) return e;
   if (!(e = ws.chunk_bounds.reserve(b.n_coefs / 64 * 4 + 512)).ok()) return e;
   if (!(e = ws.deep.reserve(G * ENC_DEEP * 4 + 64)).ok()) return e;
   if (!(e = ws.used.reserve(G * 4 + 64)).ok()) return e;
@@ -383,20 +388,21 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
     if (!(e = launch(L, d.k_alpha_enc, g32, 1, 64, args)).ok()) return e;
   }
   (void)hipEventRecord(ev[2], L.stream);
-  // qi = 224, then at most 8 bisection steps (224 < 2^8).
-  for (uint32_t probe = 0; probe < ENC_PROBES; probe++) {
+  // The search's size passes: the first at five quantisers, then one each.
+  for (uint32_t pass = 0; pass < ENC_PASSES; pass++) {
     void *a1[] = {&tables, &frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &psz, &cbnd};
-    if (!(e = launch(L, d.k_size, (g32 + 3) / 4, 1, 256, a1)).ok()) return e;
+    if (!(e = launch(L, pass ? d.k_size : d.k_size_first, (g32 + 3) / 4, 1, 256, a1)).ok()) return e;
     void *a2[] = {&pics, &states, &np32, &window};
     if (!(e = launch(L, d.k_search, (np32 + 63) / 64, 1, 64, a2)).ok()) return e;
   }
   (void)hipEventRecord(ev[3], L.stream);
   {
-    void *args[] = {&tables, &frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &psz, &cbnd, &stab};
+    // The size cache starts empty: the window fills its deep columns.
+    if (G && !(e = check(hipMemsetAsync(ws.deep.p, 0, G * ENC_DEEP * 4, L.stream), "memset")).ok()) return e;
+    void *args[] = {&tables, &frames, &pics, &slices, &g32, &coefs, &qtabs, &alen, &states, &psz, &cbnd, &stab, &deep};
     if (!(e = launch(L, d.k_window, (g32 + 3) / 4, 1, 256, args)).ok()) return e;
   }
   (void)hipEventRecord(ev[4], L.stream);
-  if (G && !(e = check(hipMemsetAsync(ws.deep.p, 0, G * ENC_DEEP * 4, L.stream), "memset")).ok()) return e;
   {
     void *args[] = {&tables, &frames, &pics, &slices, &coefs, &qtabs, &alen, &states, &stab, &deep, &used, &qidx,
                     &cbnd};

@@ -10,17 +10,21 @@
 //                coded in parallel (src/encode.rs:838-868, 457-470).
 //   pe_size      one wavefront per slice: bytes at the picture's pending probe
 //                qi (quantise + exact bit count, src/encode.rs:671-681,
-//                398-409), kept per slice and summed per picture.
-//   pe_search    one lane per picture: one step of the binary search over qi
-//                (src/encode.rs:685-713).
+//                398-409), kept per slice and summed per picture;
+//                pe_size_first the same at the first five probes at once.
+//   pe_search    one lane per picture: the steps of the binary search over
+//                qi (src/encode.rs:685-713) that the pass settles.
 //   pe_window    one wavefront per slice: the share-out's size table, from
 //                the probes at uniform and uniform-1 and, for the slices that
-//                may step further, the next finer quantisers.
+//                may step further, the next finer quantisers (down to qi 1
+//                while the slice stays a candidate, the deeper ones into the
+//                size cache).
 //   pe_shareout  one workgroup per picture: the sequential slice-by-slice
 //                share-out (src/encode.rs:769-793) solved 64 slices at a time
 //                by fixed-point iteration of its triangular recurrence (one
-//                wave); sizes past the table are coded by all the waves, four
-//                quantisers a request, one per wave, and cached.
+//                wave); a size neither the table nor the cache holds is coded
+//                by all the waves, four quantisers a request, one per wave,
+//                and cached.
 //   pe_offsets   one workgroup per frame: slice and picture offsets.
 //   pe_arena     one workgroup: each frame's place in the output arena.
 //   pe_pack      one wavefront per slice: quantise at the chosen qi and place
@@ -130,6 +134,10 @@ __device__ __forceinline__ int64_t excl_scan64(int64_t v, uint32_t lane) {
 }
 
 __device__ __forceinline__ uint32_t ilog2(uint32_t v) { return 31 - __builtin_clz(v); }
+// The wavefront's index in its workgroup, as a wave-uniform value: what
+// follows from it (the slice, its records, its chunk walk) is then kept in
+// scalar registers and worked on by the scalar unit.
+__device__ __forceinline__ uint32_t wave_index() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 // Table 15 of RDD 36 (src/tables.rs:35-38).
 __device__ __forceinline__ uint32_t qscale(uint32_t q) { return q <= 128 ? q : 128 + 4 * (q - 128); }
 
@@ -182,13 +190,15 @@ __device__ __forceinline__ void load_len_tables(const DevTables *__restrict__ T,
 // first chunk of a component (its DC coefficients) is always coded.
 constexpr float ZERO_CHUNK_LIMIT = 0.5798f;
 
-// Bytes of one slice at quantiser qi, one wavefront, and whether every size
-// field can hold them (src/encode.rs:671-681): per component, the exact bit
-// count of coefficient_bits (src/encode.rs:366-409) over the quantised
-// coefficients (src/dsp/scalar.rs:19-39), a 64-coefficient chunk of the
-// scanned array a step, one coefficient a lane.
+// Bytes of one slice at each of the quantisers qis[0..nq), one wavefront,
+// and whether every size field can hold them (src/encode.rs:671-681): per
+// component, the exact bit count of coefficient_bits (src/encode.rs:366-409)
+// over the quantised coefficients (src/dsp/scalar.rs:19-39), a 64-coefficient
+// chunk of the scanned array a step, one coefficient a lane.
 // - Chunks that quantise to zeros (by their bounds `cb`) are skipped: they
-//   add no bits and leave the run/level contexts as they are.
+//   add no bits and leave the run/level contexts as they are. The chunks are
+//   walked once for every quantiser, each chunk loaded once and counted for
+//   the quantisers at which it may hold a nonzero.
 // - The inverse quantiser of a lane's scan position comes from a register
 //   (lane k holds the component's entry k) by a lane permute, not a load.
 // - The next coded chunk's coefficients are loaded while this one is counted.
@@ -200,32 +210,49 @@ constexpr float ZERO_CHUNK_LIMIT = 0.5798f;
 // No conversion here saturates: a coefficient of fdct_load's output is at
 // most 64 * 256 * (1/2)^2 = 4096 in magnitude and an inverse quantiser at
 // most 8 / 2, so |x| + r < 2^15.
-__device__ void slice_bytes(const float *__restrict__ coef, const float *__restrict__ cb, uint32_t mbs,
-                            uint32_t cpm, const float *__restrict__ qt, uint32_t qi, uint32_t hb, uint32_t alen,
-                            const uint8_t *lens, uint32_t &size, bool &fits) {
+template <int NQ>
+__device__ __forceinline__ void slice_bytes_q(const float *__restrict__ coef, const float *__restrict__ cb,
+                                              uint32_t mbs, uint32_t cpm, const float *__restrict__ qt,
+                                              const uint32_t *qis, uint32_t nq, uint32_t hb, uint32_t alen,
+                                              const uint8_t *lens, uint32_t *size, bool *fits) {
   const uint32_t L = threadIdx.x & 63;
   const uint64_t below = (1ull << L) - 1;
   const uint8_t *run_lens = lens, *level_lens = lens + 16 * 64;
-  // Chunks that may hold a nonzero at this quantiser: bit k of live[k / 64]
-  // (a slice has at most 96 chunks, and no component crosses chunk 64:
-  // slices have 1, 2, 4 or 8 macroblocks).
+  // Chunks that may hold a nonzero at qis[k]: bit n of live0[k] / live1[k]
+  // for chunk n / chunk 64 + n (a slice has at most 96 chunks, and no
+  // component crosses chunk 64: slices have 1, 2, 4 or 8 macroblocks).
   const uint32_t TB = (4 + 2 * cpm) * mbs;
-  const float qlim = (float)qscale(qi) * ZERO_CHUNK_LIMIT;
-  const uint64_t live0 = __ballot(L < TB && cb[L < TB ? L : 0] > qlim);
-  const uint64_t live1 = __ballot(L + 64 < TB && cb[L + 64 < TB ? L + 64 : 0] > qlim);
-  size = hb + alen;
-  fits = true;
+  const float b0 = cb[L < TB ? L : 0], b1 = cb[L + 64 < TB ? L + 64 : 0];
+  uint64_t live0[NQ], live1[NQ], any0 = 0, any1 = 0;
+#pragma unroll
+  for (int k = 0; k < NQ; k++) {
+    const float qlim = (float)qscale(qis[k]) * ZERO_CHUNK_LIMIT;
+    live0[k] = (uint32_t)k < nq ? __ballot(L < TB && b0 > qlim) : 0ull;
+    live1[k] = (uint32_t)k < nq ? __ballot(L + 64 < TB && b1 > qlim) : 0ull;
+    any0 |= live0[k];
+    any1 |= live1[k];
+    size[k] = hb + alen;
+    fits[k] = true;
+  }
   uint32_t k0 = 0;  // the component's first chunk in the slice
   for (uint32_t c = 0; c < 3; c++) {
     uint32_t nb, off;
     component(c, mbs, cpm, nb, off);
     const uint32_t lg = ilog2(nb);
-    uint64_t todo = (((k0 >> 6) ? live1 : live0) >> (k0 & 63)) & ((1ull << nb) - 1);
-    todo |= 1;  // the DC chunk
+    const uint64_t in_comp = (1ull << nb) - 1;
+    uint64_t todo = ((((k0 >> 6) ? any1 : any0) >> (k0 & 63)) & in_comp) | 1;  // and the DC chunk
+    uint64_t lv[NQ];
+    float invL[NQ];
+    uint32_t bits[NQ], last_idx[NQ], ctx[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      lv[k] = ((((k0 >> 6) ? live1[k] : live0[k]) >> (k0 & 63)) & in_comp) | 1;
+      invL[k] = (uint32_t)k < nq ? qt[(qis[k] * 2 + (c > 0 ? 1u : 0u)) * 64 + L] : 0.0f;
+      bits[k] = 0;
+      last_idx[k] = nb - 1;
+      ctx[k] = 4u | 1u << 4;  // no nonzero yet: run context 4, level context 1
+    }
     const float *cf = coef + off + L;
-    const float invL = qt[(qi * 2 + (c > 0 ? 1u : 0u)) * 64 + L];
-    uint32_t bits = 0;
-    uint32_t last_idx = nb - 1, ctx = 4u | 1u << 4;  // no nonzero yet: run context 4, level context 1
     uint32_t j = 0;
     float f = cf[0];
     for (;;) {
@@ -233,56 +260,78 @@ __device__ void slice_bytes(const float *__restrict__ coef, const float *__restr
       const uint32_t jn = todo ? (uint32_t)__builtin_ctzll(todo) : j;
       const float fn = cf[jn * 64];
       const uint32_t base = j * 64, i = base + L, s = i >> lg;
-      const float inv = __shfl(invL, (int)s, 64);
-      const float x = f * inv;
-      const uint32_t m = (uint32_t)(int32_t)floorf(fabsf(x) + (s == 0 ? 0.5f : 0.42f));
-      bool nz = m != 0;
-      if (j == 0) {  // the DC coefficients (src/encode.rs:383-395), nb <= 32 of the first chunk
-        const int32_t q = x < 0.0f ? -(int32_t)m : (int32_t)m;
-        const int32_t d1 = __shfl(q, L ? (int)L - 1 : 0, 64);
-        const int32_t d2 = __shfl(q, L > 1 ? (int)L - 2 : 0, 64);
-        if (L < nb) {
-          if (L == 0) {
-            bits += book_len(FIRST_DC, signed_to_symbol(q));
-          } else {
-            const int32_t diff = (int32_t)((uint32_t)q - (uint32_t)d1);
-            const int32_t pd = L == 1 ? 3 : (int32_t)((uint32_t)d1 - (uint32_t)d2);
-            const int32_t nn = pd < 0 ? (int32_t)(0u - (uint32_t)diff) : diff;
-            const uint32_t a = pd < 0 ? 0u - (uint32_t)pd : (uint32_t)pd;
-            bits += book_len(dc_book(a < 3 ? a : 3), signed_to_symbol(nn));
+#pragma unroll
+      for (int k = 0; k < NQ; k++) {
+        if ((uint32_t)k >= nq || !((lv[k] >> j) & 1)) continue;
+        const float inv = __shfl(invL[k], (int)s, 64);
+        const float x = f * inv;
+        const uint32_t m = (uint32_t)(int32_t)floorf(fabsf(x) + (s == 0 ? 0.5f : 0.42f));
+        bool nz = m != 0;
+        if (j == 0) {  // the DC coefficients (src/encode.rs:383-395), nb <= 32 of the first chunk
+          const int32_t q = x < 0.0f ? -(int32_t)m : (int32_t)m;
+          const int32_t d1 = __shfl(q, L ? (int)L - 1 : 0, 64);
+          const int32_t d2 = __shfl(q, L > 1 ? (int)L - 2 : 0, 64);
+          if (L < nb) {
+            if (L == 0) {
+              bits[k] += book_len(FIRST_DC, signed_to_symbol(q));
+            } else {
+              const int32_t diff = (int32_t)((uint32_t)q - (uint32_t)d1);
+              const int32_t pd = L == 1 ? 3 : (int32_t)((uint32_t)d1 - (uint32_t)d2);
+              const int32_t nn = pd < 0 ? (int32_t)(0u - (uint32_t)diff) : diff;
+              const uint32_t a = pd < 0 ? 0u - (uint32_t)pd : (uint32_t)pd;
+              bits[k] += book_len(dc_book(a < 3 ? a : 3), signed_to_symbol(nn));
+            }
           }
+          nz = nz && L >= nb;
         }
-        nz = nz && L >= nb;
-      }
-      const uint64_t mask = __ballot(nz);
-      if (mask) {
-        const uint64_t lower = mask & below;
-        const int P = lower ? 63 - (int)__builtin_clzll(lower) : -1;
-        const uint32_t prev_idx = P >= 0 ? base + (uint32_t)P : last_idx;
-        const uint32_t run = i - prev_idx - 1, lvl = m - 1;
-        const uint32_t mine = (run < 15 ? run : 15u) | (lvl < 8 ? lvl : 8u) << 4;
-        uint32_t pc = __shfl(mine, P >= 0 ? P : (int)L, 64);
-        if (P < 0) pc = ctx;
-        uint32_t pair = run_lens[(pc & 15) << 6 | (run & 63)] + level_lens[(pc >> 4) << 8 | (lvl & 255)] + 1;
-        const bool big = nz && (run >= 64 || lvl >= 256);
-        if (__ballot(big)) {
-          if (big) pair = run_code_len(pc & 15, run) + level_code_len(pc >> 4, lvl) + 1;
+        const uint64_t mask = __ballot(nz);
+        if (mask) {
+          const uint64_t lower = mask & below;
+          const int P = lower ? 63 - (int)__builtin_clzll(lower) : -1;
+          const uint32_t prev_idx = P >= 0 ? base + (uint32_t)P : last_idx[k];
+          const uint32_t run = i - prev_idx - 1, lvl = m - 1;
+          const uint32_t mine = (run < 15 ? run : 15u) | (lvl < 8 ? lvl : 8u) << 4;
+          uint32_t pc = __shfl(mine, P >= 0 ? P : (int)L, 64);
+          if (P < 0) pc = ctx[k];
+          uint32_t pair = run_lens[(pc & 15) << 6 | (run & 63)] + level_lens[(pc >> 4) << 8 | (lvl & 255)] + 1;
+          const bool big = nz && (run >= 64 || lvl >= 256);
+          if (__ballot(big)) {
+            if (big) pair = run_code_len(pc & 15, run) + level_code_len(pc >> 4, lvl) + 1;
+          }
+          if (nz) bits[k] += pair;
+          const uint32_t last = 63 - (uint32_t)__builtin_clzll(mask);
+          last_idx[k] = base + last;
+          ctx[k] = __builtin_amdgcn_readlane(mine, last);
         }
-        if (nz) bits += pair;
-        const uint32_t last = 63 - (uint32_t)__builtin_clzll(mask);
-        last_idx = base + last;
-        ctx = __builtin_amdgcn_readlane(mine, last);
       }
       if (!todo) break;
       j = jn;
       f = fn;
     }
-    const uint32_t bytes = (wave_sum(bits) + 7) / 8;
-    size += bytes;
-    if (bytes > 65535) fits = false;
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      if ((uint32_t)k >= nq) continue;
+      const uint32_t bytes = (wave_sum(bits[k]) + 7) / 8;
+      size[k] += bytes;
+      if (bytes > 65535) fits[k] = false;
+    }
     k0 += nb;
   }
-  if (size > 65535) fits = false;
+#pragma unroll
+  for (int k = 0; k < NQ; k++)
+    if (size[k] > 65535) fits[k] = false;
+}
+
+// Bytes of one slice at quantiser qi (slice_bytes_q for one quantiser).
+__device__ void slice_bytes(const float *__restrict__ coef, const float *__restrict__ cb, uint32_t mbs,
+                            uint32_t cpm, const float *__restrict__ qt, uint32_t qi, uint32_t hb, uint32_t alen,
+                            const uint8_t *lens, uint32_t &size, bool &fits) {
+  const uint32_t q[1] = {qi};
+  uint32_t sz[1];
+  bool ft[1];
+  slice_bytes_q<1>(coef, cb, mbs, cpm, qt, q, 1, hb, alen, lens, sz, ft);
+  size = sz[0];
+  fits = ft[0];
 }
 
 __device__ __forceinline__ uint16_t rescale(uint32_t a, uint32_t identity, uint64_t m) {
@@ -523,7 +572,46 @@ extern "C" __global__ void __launch_bounds__(64)
 }
 
 // ---------------------------------------------------------------------------
-// One wavefront per slice, four per workgroup.
+// One wavefront per slice, four per workgroup: the slice's bytes at each
+// quantiser the picture's pass lists, kept per slice (probe_sizes, one row
+// per probe result) and summed per picture.
+template <int NQ>
+__device__ __forceinline__ void size_pass(const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
+                                          const EncSlice *__restrict__ slices, uint32_t n_slices,
+                                          const float *__restrict__ coefs, const float *__restrict__ qtabs,
+                                          const uint32_t *__restrict__ alpha_len, EncState *__restrict__ state,
+                                          uint32_t *__restrict__ probe_sizes, const float *__restrict__ cbound,
+                                          const uint32_t *lens) {
+  const uint32_t g = blockIdx.x * 4 + wave_index();
+  if (g >= n_slices) return;
+  const EncSlice S = slices[g];
+  EncState &st = state[S.pic];
+  const uint32_t nq = st.nq < (uint32_t)NQ ? st.nq : (uint32_t)NQ;
+  if (nq == 0) return;
+  uint32_t qis[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; k++) qis[k] = (uint32_t)k < nq ? st.pq[k] : 1u;
+  const uint32_t round = st.round;
+  const EncPic &P = pics[S.pic];
+  const EncFrame &F = frames[P.frame];
+  uint32_t size[NQ];
+  bool fits[NQ];
+  slice_bytes_q<NQ>(coefs + S.coef, cbound + S.coef / 64, S.mbs, F.chroma == 2 ? 2u : 4u,
+                    qtabs + (size_t)P.qtab * 225 * 128, qis, nq, P.header_bytes, alpha_len[g], (const uint8_t *)lens,
+                    size, fits);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      if ((uint32_t)k >= nq) continue;
+      // Every probe's sizes are kept: the share-out starts from the uniform
+      // quantiser's and the next finer one's, both of which the search probed.
+      if (round + k < ENC_HIST) probe_sizes[(size_t)(round + k) * n_slices + g] = size[k] | (fits[k] ? 0u : 0x80000000u);
+      atomicAdd(&st.psum[k], (unsigned long long)size[k]);
+      if (!fits[k]) atomicAdd(&st.pnotfit[k], 1u);
+    }
+  }
+}
+
 extern "C" __global__ void __launch_bounds__(256)
     pe_size(const DevTables *__restrict__ T, const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
             const EncSlice *__restrict__ slices, uint32_t n_slices, const float *__restrict__ coefs,
@@ -531,74 +619,86 @@ extern "C" __global__ void __launch_bounds__(256)
             EncState *__restrict__ state, uint32_t *__restrict__ probe_sizes, const float *__restrict__ cbound) {
   __shared__ uint32_t lens[ENC_LEN_BYTES / 4];
   load_len_tables(T, lens);
-  const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (g >= n_slices) return;
-  const EncSlice S = slices[g];
-  const uint32_t qi = state[S.pic].probe;
-  if (qi == 0) return;
-  const uint32_t round = state[S.pic].round;
-  const EncPic &P = pics[S.pic];
-  const EncFrame &F = frames[P.frame];
-  uint32_t size[1];
-  bool fits[1];
-  slice_bytes(coefs + S.coef, cbound + S.coef / 64, S.mbs, F.chroma == 2 ? 2u : 4u,
-              qtabs + (size_t)P.qtab * 225 * 128, qi, P.header_bytes, alpha_len[g], (const uint8_t *)lens, size[0],
-              fits[0]);
-  if ((threadIdx.x & 63) == 0) {
-    // Every probe's sizes are kept: the share-out starts from the uniform
-    // quantiser's and the next finer one's, both of which the search probed.
-    probe_sizes[(size_t)round * n_slices + g] = size[0] | (fits[0] ? 0u : 0x80000000u);
-    atomicAdd(&state[S.pic].sum, (unsigned long long)size[0]);
-    if (!fits[0]) atomicAdd(&state[S.pic].notfit, 1u);
-  }
+  size_pass<1>(frames, pics, slices, n_slices, coefs, qtabs, alpha_len, state, probe_sizes, cbound, lens);
 }
 
+// The first pass: up to ENC_PASS_Q quantisers, each chunk loaded once.
+extern "C" __global__ void __launch_bounds__(256)
+    pe_size_first(const DevTables *__restrict__ T, const EncFrame *__restrict__ frames,
+                  const EncPic *__restrict__ pics, const EncSlice *__restrict__ slices, uint32_t n_slices,
+                  const float *__restrict__ coefs, const float *__restrict__ qtabs,
+                  const uint32_t *__restrict__ alpha_len, EncState *__restrict__ state,
+                  uint32_t *__restrict__ probe_sizes, const float *__restrict__ cbound) {
+  __shared__ uint32_t lens[ENC_LEN_BYTES / 4];
+  load_len_tables(T, lens);
+  size_pass<ENC_PASS_Q>(frames, pics, slices, n_slices, coefs, qtabs, alpha_len, state, probe_sizes, cbound, lens);
+}
+
+// One lane per picture: the binary search over qi (src/encode.rs:685-713),
+// as many of its steps as the pass just made settles, in the search's own
+// order: a step whose probe the pass sized is taken, and the search waits
+// at the first one it did not (the next pass sizes that probe alone).
 extern "C" __global__ void __launch_bounds__(64)
     pe_search(const EncPic *__restrict__ pics, EncState *__restrict__ state, uint32_t n_pics,
               uint32_t window) {
   const uint32_t p = blockIdx.x * 64 + threadIdx.x;
   if (p >= n_pics) return;
   EncState &st = state[p];
-  if (st.phase == 2) return;
+  const uint32_t nq = st.nq;
+  if (nq == 0) return;
   if (st.phase == 0) {
     // (budget + alpha) less the picture header and the slice table (src/encode.rs:686-689).
     const uint64_t b = pics[p].budget + st.alpha_total;
     const uint64_t over = 8 + 2ull * pics[p].n_slices;
     st.budget = b > over ? b - over : 0;
   }
-  const bool ok = st.notfit == 0 && st.sum <= st.budget;
-  st.probe_qi[st.round] = st.probe;
-  st.probe_sum[st.round] = st.sum;
-  st.round++;
-  if (st.phase == 0) {
-    st.fits_at_all = ok;
-    if (ok) {
-      st.lo = 1;
-      st.hi = 224;
-      st.phase = 1;
-    } else {
-      st.uniform = 224;
-      st.phase = 2;
-    }
-  } else if (ok) {
-    st.hi = st.probe;
-  } else {
-    st.lo = st.probe + 1;
+  for (uint32_t k = 0; k < nq && st.round < ENC_HIST; k++) {
+    st.probe_qi[st.round] = st.pq[k];
+    st.probe_sum[st.round] = st.psum[k];
+    st.round++;
   }
-  if (st.phase == 1) {
-    if (st.lo < st.hi) {
-      st.probe = st.lo + (st.hi - st.lo) / 2;
+  for (;;) {
+    uint32_t k = 0;
+    while (k < nq && st.pq[k] != st.probe) k++;
+    if (k == nq) break;
+    const bool ok = st.pnotfit[k] == 0 && st.psum[k] <= st.budget;
+    if (st.phase == 0) {
+      st.fits_at_all = ok;
+      if (ok) {
+        st.lo = 1;
+        st.hi = 224;
+        st.phase = 1;
+      } else {
+        st.uniform = 224;
+        st.phase = 2;
+      }
+    } else if (ok) {
+      st.hi = st.probe;
     } else {
-      st.uniform = st.hi;
-      st.phase = 2;
+      st.lo = st.probe + 1;
     }
+    if (st.phase == 1) {
+      if (st.lo < st.hi) {
+        st.probe = st.lo + (st.hi - st.lo) / 2;
+      } else {
+        st.uniform = st.hi;
+        st.phase = 2;
+      }
+    }
+    if (st.phase == 2) break;
+  }
+  for (uint32_t k = 0; k < ENC_PASS_Q; k++) {
+    st.psum[k] = 0;
+    st.pnotfit[k] = 0;
   }
   if (st.phase == 2) {
     st.probe = 0;
+    st.nq = 0;
     st.window_k = (st.fits_at_all && st.uniform > 1) ? (st.uniform < window ? st.uniform : window) : 1;
+  } else {
+    st.nq = 1;
+    st.pq[0] = st.probe;
   }
-  st.sum = 0;
-  st.notfit = 0;
 }
 
 // One wavefront per slice: the size table's columns, qi = uniform - t.
@@ -608,16 +708,22 @@ extern "C" __global__ void __launch_bounds__(64)
 // the slices the reference precomputes them for (src/encode.rs:741-766): a
 // slice whose previous column fits and is at most `most`, its size at
 // uniform plus the slack shared from it on if no slice before it takes any.
-// A column left 0 is coded by the share-out if it turns out to need it.
+// Past the table's K columns the same candidate chain goes on, down to qi
+// 1, into the size cache `deep` (column t of slice g at g * ENC_DEEP + t,
+// cleared beforehand): every size the share-out can reach is then found
+// there, unless a size finer than uniform came out smaller than the one at
+// uniform (slack then grows past `most`), and the share-out codes that one
+// itself. A column left 0 is coded by the share-out if it turns out to need
+// it.
 extern "C" __global__ void __launch_bounds__(256)
     pe_window(const DevTables *__restrict__ T, const EncFrame *__restrict__ frames, const EncPic *__restrict__ pics,
               const EncSlice *__restrict__ slices, uint32_t n_slices, const float *__restrict__ coefs,
               const float *__restrict__ qtabs, const uint32_t *__restrict__ alpha_len,
               const EncState *__restrict__ state, const uint32_t *__restrict__ probe_sizes,
-              const float *__restrict__ cbound, uint32_t *__restrict__ size_tab) {
+              const float *__restrict__ cbound, uint32_t *__restrict__ size_tab, uint32_t *__restrict__ deep) {
   __shared__ uint32_t lens[ENC_LEN_BYTES / 4];
   load_len_tables(T, lens);
-  const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t g = blockIdx.x * 4 + wave_index();
   if (g >= n_slices) return;
   const uint32_t L = threadIdx.x & 63;
   const EncSlice S = slices[g];
@@ -641,16 +747,32 @@ extern "C" __global__ void __launch_bounds__(256)
     // The search found uniform fitting, so its total is within the budget.
     const uint64_t slack = st.budget - st.probe_sum[ru];
     const uint64_t most = (uint64_t)(col[0] & 0x7FFFFFFFu) + slack * S.mbs / S.mbs_from;
+    // The slice is a candidate for column t while column t - 1 fits and is
+    // at most `most` (src/encode.rs:751-754).
+    uint32_t prev = col[1];
+    bool candidate = true;
 #pragma unroll
     for (uint32_t t = 2; t < ENC_MAXK; t++) {
-      const uint32_t prev = col[t - 1];
-      if (t >= K || prev == 0 || (prev >> 31) || (uint64_t)(prev & 0x7FFFFFFFu) > most) break;
-      const uint32_t q = U - t;
+      if (t >= K) break;
+      if ((prev >> 31) || (uint64_t)(prev & 0x7FFFFFFFu) > most) {
+        candidate = false;
+        break;
+      }
       uint32_t sz[1];
       bool ft[1];
-      slice_bytes(coefs + S.coef, cbound + S.coef / 64, S.mbs, cpm, qt, q, P.header_bytes, alen,
+      slice_bytes(coefs + S.coef, cbound + S.coef / 64, S.mbs, cpm, qt, U - t, P.header_bytes, alen,
                   (const uint8_t *)lens, sz[0], ft[0]);
       col[t] = sz[0] | (ft[0] ? 0u : 0x80000000u);
+      prev = col[t];
+    }
+    for (uint32_t t = K; candidate && t < U; t++) {
+      if ((prev >> 31) || (uint64_t)(prev & 0x7FFFFFFFu) > most) break;
+      uint32_t sz;
+      bool ft;
+      slice_bytes(coefs + S.coef, cbound + S.coef / 64, S.mbs, cpm, qt, U - t, P.header_bytes, alen,
+                  (const uint8_t *)lens, sz, ft);
+      prev = sz | (ft ? 0u : 0x80000000u);
+      if (L == 0) deep[(size_t)g * ENC_DEEP + t] = prev;
     }
   }
   if (L < ENC_MAXK) {
@@ -680,7 +802,7 @@ extern "C" __global__ void __launch_bounds__(64 * SHARE_WAVES)
   __shared__ uint32_t n_req, done;
   __shared__ uint32_t lens[ENC_LEN_BYTES / 4];
   load_len_tables(T, lens);
-  const uint32_t p = blockIdx.x, tid = threadIdx.x, L = tid & 63, wave = tid >> 6;
+  const uint32_t p = blockIdx.x, tid = threadIdx.x, L = tid & 63, wave = wave_index();
   const EncPic &P = pics[p];
   EncState &st = state[p];
   const uint32_t n = P.n_slices, g0 = P.slice_base, U = st.uniform, K = st.window_k;

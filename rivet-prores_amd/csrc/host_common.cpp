@@ -407,11 +407,12 @@ Err ensure_encoder(Device &d) {
   if (d.enc_mod) return Err{};
   Err e = check(hipModuleLoadData(&d.enc_mod, prores_encode_co), "hipModuleLoadData(encode)");
   if (!e.ok()) return e;
-  const char *const names[] = {"pe_fdct",     "pe_alpha",   "pe_size", "pe_search", "pe_window",
-                               "pe_shareout", "pe_offsets", "pe_pack", "pe_arena", "pe_slices"};
-  hipFunction_t *const fns[] = {&d.k_fdct,     &d.k_alpha_enc, &d.k_size, &d.k_search, &d.k_window,
-                                &d.k_shareout, &d.k_offsets,   &d.k_pack, &d.k_arena,  &d.k_slices};
-  e = load_functions(d.enc_mod, names, fns, 10);
+  const char *const names[] = {"pe_fdct",     "pe_alpha",   "pe_size", "pe_search", "pe_window",   "pe_shareout",
+                               "pe_offsets", "pe_pack",    "pe_arena", "pe_slices", "pe_size_first"};
+  hipFunction_t *const fns[] = {&d.k_fdct,    &d.k_alpha_enc, &d.k_size,  &d.k_search, &d.k_window,
+                                &d.k_shareout, &d.k_offsets,  &d.k_pack,  &d.k_arena,  &d.k_slices,
+                                &d.k_size_first};
+  e = load_functions(d.enc_mod, names, fns, 11);
   if (!e.ok()) {
     (void)hipModuleUnload(d.enc_mod);
     d.enc_mod = nullptr;

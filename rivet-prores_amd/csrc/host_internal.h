@@ -102,7 +102,7 @@ struct Device {
                 k_plan = nullptr, k_order = nullptr;
   hipFunction_t k_fdct = nullptr, k_alpha_enc = nullptr, k_size = nullptr, k_search = nullptr,
                 k_shareout = nullptr, k_offsets = nullptr, k_pack = nullptr,
-                k_window = nullptr, k_arena = nullptr, k_slices = nullptr;
+                k_window = nullptr, k_arena = nullptr, k_slices = nullptr, k_size_first = nullptr;
   DevTables *tables = nullptr;
   std::mutex mu;  // lane creation, encoder loading, the batch tuning
   Lane *lanes[MAX_LANES] = {nullptr};
