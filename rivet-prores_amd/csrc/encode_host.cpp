@@ -431,7 +431,7 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
     void *aargs[] = {&fsize, &nf, &base, &capacity, &oaddr, &oflow};
     if (!(x = launch(L, d.k_arena, 1, 1, 256, aargs)).ok()) return x;
     void *args[] = {&frames, &pics,  &slices, &coefs, &qtabs, &alen,  &ascr,
-                    &used,   &qidx,  &sout,   &fsize, &oaddr, &oflow};
+                    &used,   &qidx,  &sout,   &fsize, &oaddr, &oflow, &cbnd};
     if (!(x = launch(L, d.k_pack, g32, 1, 64, args)).ok()) return x;
     return check(hipMemcpyAsync(ws.hres.p, fsize, NF * 8, hipMemcpyDeviceToHost, L.stream), "frame sizes");
   };

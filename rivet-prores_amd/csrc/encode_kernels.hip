@@ -79,16 +79,15 @@ __device__ __forceinline__ uint32_t book_len(Book b, uint32_t n) {
   uint64_t v = (uint64_t)(n - th) + (1ull << b.ke);
   return b.lq + 1 + 2 * bitlen64(v) - b.ke - 1;
 }
-// Codebook::code (src/vlc.rs:131-151): the low `len` bits are the codeword.
-__device__ __forceinline__ uint64_t book_code(Book b, uint32_t n, uint32_t &len) {
-  uint32_t th = (b.lq + 1) << b.kr;
-  if (n < th) {
-    len = (n >> b.kr) + 1 + b.kr;
-    return (1ull << b.kr) | (n & ((1u << b.kr) - 1));
-  }
-  uint64_t v = (uint64_t)(n - th) + (1ull << b.ke);
-  len = b.lq + 1 + 2 * bitlen64(v) - b.ke - 1;
-  return v;
+// Codebook::code (src/vlc.rs:131-151) in 32-bit arithmetic and without branches, for n < 2^30
+// (every symbol pe_pack codes is below 2^17): the codeword's value (its low
+// `len` bits are the codeword; the bits above the value are zeros).
+__device__ __forceinline__ uint32_t book_code32(Book b, uint32_t n, uint32_t &len) {
+  const uint32_t th = (b.lq + 1) << b.kr;
+  const uint32_t v = n - th + (1u << b.ke);
+  const bool rice = n < th;
+  len = rice ? (n >> b.kr) + 1 + b.kr : b.lq + 2 * (32 - (uint32_t)__clz((int)v)) - b.ke;
+  return rice ? (1u << b.kr) | (n & ((1u << b.kr) - 1)) : v;
 }
 
 __device__ __forceinline__ uint32_t signed_to_symbol(int32_t n) {
@@ -115,6 +114,29 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
   }
   return v;
 }
+// Wave-wide inclusive prefix sum over lanes by DPP row shifts (within rows
+// of 16: +1, +2, +3 of the input, then +4 and +8 of the partial sums on the
+// upper banks) and row broadcasts (lane 15 into rows 1 and 3, lane 31 into
+// rows 2 and 3). Every lane of the wave must be active.
+template <int CTRL, int ROWS, int BANKS>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, BANKS, false);
+}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  uint32_t s = v + dpp_move<0x111, 0xf, 0xf>(v);  // row_shr:1
+  s += dpp_move<0x112, 0xf, 0xf>(v);              // row_shr:2
+  s += dpp_move<0x113, 0xf, 0xf>(v);              // row_shr:3
+  s += dpp_move<0x114, 0xf, 0xe>(s);              // row_shr:4, banks 1-3
+  s += dpp_move<0x118, 0xf, 0xc>(s);              // row_shr:8, banks 2-3
+  s += dpp_move<0x142, 0xa, 0xf>(s);              // row_bcast:15, rows 1 and 3
+  s += dpp_move<0x143, 0xc, 0xf>(s);              // row_bcast:31, rows 2 and 3
+  return s;
+}
+// The wave's total (every lane active), a wave-uniform value.
+__device__ __forceinline__ uint32_t wave_total(uint32_t v) {
+  return __builtin_amdgcn_readlane(wave_incl_scan(v), 63);
+}
+
 __device__ __forceinline__ uint32_t excl_scan(uint32_t v, uint32_t lane) {
   uint32_t incl = v;
   for (uint32_t d = 1; d < 64; d <<= 1) {
@@ -148,12 +170,6 @@ __device__ __forceinline__ void component(uint32_t c, uint32_t mbs, uint32_t cpm
   nb = (c == 0 ? 4u : cpm) * mbs;
   off = c == 0 ? 0u : (c == 1 ? 256u * mbs : (256u + 64u * cpm) * mbs);
 }
-
-// The run/level context state pe_pack carries from one 64-coefficient chunk
-// of a component's scanned array to the next (src/encode.rs:413-437).
-struct Ctx {
-  uint32_t idx, run, lvl;  // last nonzero AC index, its run and level symbol
-};
 
 // Length of the combination code (lq, kr, ke) of n (Codebook::len,
 // src/vlc.rs:155-172) without branches: the Rice length below the threshold,
@@ -311,7 +327,7 @@ __device__ __forceinline__ void slice_bytes_q(const float *__restrict__ coef, co
 #pragma unroll
     for (int k = 0; k < NQ; k++) {
       if ((uint32_t)k >= nq) continue;
-      const uint32_t bytes = (wave_sum(bits[k]) + 7) / 8;
+      const uint32_t bytes = (wave_total(bits[k]) + 7) / 8;
       size[k] += bytes;
       if (bytes > 65535) fits[k] = false;
     }
@@ -862,7 +878,7 @@ extern "C" __global__ void __launch_bounds__(64 * SHARE_WAVES)
           m = nm_next;
           if (c0 + 64 < n) prefetch(c0 + 64);
           base = valid ? (w0 & 0x7FFFFFFFu) : 0u;
-          M = total_mbs - mbs_before - excl_scan(m, L);
+          M = total_mbs - mbs_before - (wave_incl_scan(m) - m);
           s_i = slack;
           round = 0;
           phase = 1;
@@ -1101,7 +1117,7 @@ extern "C" __global__ void __launch_bounds__(64)
             const uint8_t *__restrict__ alpha_scratch, const uint32_t *__restrict__ used,
             const uint32_t *__restrict__ qidx, const uint64_t *__restrict__ slice_out,
             const uint64_t *__restrict__ frame_size, const uint64_t *__restrict__ out_addr,
-            const uint32_t *__restrict__ overflow) {
+            const uint32_t *__restrict__ overflow, const float *__restrict__ cbound) {
   __shared__ uint32_t ring[RING];
   if (*overflow) return;  // the arena is too small: the host grows it and packs again
   const uint32_t g = blockIdx.x, L = threadIdx.x;
@@ -1157,33 +1173,57 @@ extern "C" __global__ void __launch_bounds__(64)
   const uint32_t cpm = F.chroma == 2 ? 2u : 4u;
   const float *qt = qtabs + (size_t)P.qtab * 225 * 128;
   const float *cf0 = coefs + S.coef;
+  // Chunks that may hold a nonzero at qi (their bounds, as in slice_bytes_q):
+  // the others add no codeword and leave the contexts as they are.
+  const uint32_t TB = (4 + 2 * cpm) * S.mbs;
+  const float *cb = cbound + S.coef / 64;
+  const float qlim = (float)qscale(qi) * ZERO_CHUNK_LIMIT;
+  const uint64_t live0 = __ballot(L < TB && cb[L < TB ? L : 0] > qlim);
+  const uint64_t live1 = __ballot(L + 64 < TB && cb[L + 64 < TB ? L + 64 : 0] > qlim);
+  // Stores the ring's complete words [flushed, upto) and clears them.
+  auto flush = [&](uint32_t upto) {
+    for (uint32_t w = flushed + L; w < upto; w += 64) {
+      store_word(A, w, ring[w & (RING - 1)], w == 0 ? lead : 0u, 4);
+      ring[w & (RING - 1)] = 0;
+    }
+    flushed = upto;
+  };
   uint32_t comp_bytes[3];
+  uint32_t k0 = 0;  // the component's first chunk in the slice
   for (uint32_t c = 0; c < 3; c++) {
     uint32_t nb, off;
     component(c, S.mbs, cpm, nb, off);
     const uint32_t lg = ilog2(nb);
-    const float *cf = cf0 + off;
-    const float *inv = qt + (qi * 2 + (c > 0 ? 1 : 0)) * 64;
+    const float *cf = cf0 + off + L;
+    const float invL = qt[(qi * 2 + (c > 0 ? 1 : 0)) * 64 + L];
+    uint64_t todo = ((((k0 >> 6) ? live1 : live0) >> (k0 & 63)) & ((1ull << nb) - 1)) | 1;  // and the DC chunk
     const uint32_t start = pos;
-    Ctx cx{nb - 1, 4, 1};
-    for (uint32_t base = 0; base < nb * 64; base += 64) {
-      const uint32_t i = base + L;
+    // The last nonzero's index and its packed run/level contexts
+    // (min(run, 15) | min(level, 8) << 4); none yet: run context 4, level 1.
+    uint32_t last_idx = nb - 1, ctx = 4u | 1u << 4;
+    uint32_t j = 0;
+    float f = cf[0];
+    for (;;) {
+      todo &= todo - 1;  // chunk j is the lowest bit
+      const uint32_t jn = todo ? (uint32_t)__builtin_ctzll(todo) : j;
+      const float fn = cf[jn * 64];  // the next coded chunk, loaded while this one is coded
+      const uint32_t base = j * 64, i = base + L;
       const uint32_t s = i >> lg;
-      const int32_t q = quant(cf[i], inv[s], s == 0 ? 0.5f : 0.42f);
+      const int32_t q = quant(f, __shfl(invL, (int)s, 64), s == 0 ? 0.5f : 0.42f);
       uint64_t v1 = 0, v2 = 0;
       uint32_t n1 = 0, n2 = 0;
-      if (base == 0) {
+      if (j == 0) {
         const int32_t d1 = __shfl(q, L ? L - 1 : 0, 64);
         const int32_t d2 = __shfl(q, L > 1 ? L - 2 : 0, 64);
         if (L < nb) {
           if (L == 0) {
-            v1 = book_code(FIRST_DC, signed_to_symbol(q), n1);
+            v1 = book_code32(FIRST_DC, signed_to_symbol(q), n1);
           } else {
             const int32_t diff = (int32_t)((uint32_t)q - (uint32_t)d1);
             const int32_t pd = L == 1 ? 3 : (int32_t)((uint32_t)d1 - (uint32_t)d2);
             const int32_t nn = pd < 0 ? (int32_t)(0u - (uint32_t)diff) : diff;
             const uint32_t a = pd < 0 ? 0u - (uint32_t)pd : (uint32_t)pd;
-            v1 = book_code(dc_book(a < 3 ? a : 3), signed_to_symbol(nn), n1);
+            v1 = book_code32(dc_book(a < 3 ? a : 3), signed_to_symbol(nn), n1);
           }
         }
       }
@@ -1192,50 +1232,50 @@ extern "C" __global__ void __launch_bounds__(64)
       if (mask) {
         const uint64_t lower = mask & ((1ull << L) - 1);
         const int Pn = lower ? 63 - (int)__builtin_clzll(lower) : -1;
-        const uint32_t prev_idx = Pn >= 0 ? base + (uint32_t)Pn : cx.idx;
+        const uint32_t prev_idx = Pn >= 0 ? base + (uint32_t)Pn : last_idx;
         const uint32_t run = i - prev_idx - 1;
         const uint32_t lvl = (q < 0 ? 0u - (uint32_t)q : (uint32_t)q) - 1;
-        uint32_t prun = __shfl(run, Pn >= 0 ? Pn : (int)L, 64);
-        uint32_t plvl = __shfl(lvl, Pn >= 0 ? Pn : (int)L, 64);
-        if (Pn < 0) {
-          prun = cx.run;
-          plvl = cx.lvl;
-        }
+        const uint32_t mine = (run < 15 ? run : 15u) | (lvl < 8 ? lvl : 8u) << 4;
+        uint32_t pc = __shfl(mine, Pn >= 0 ? Pn : (int)L, 64);
+        if (Pn < 0) pc = ctx;
         if (nz) {
-          // Run, then level and sign (src/encode.rs:419-432).
-          v1 = book_code(run_book(prun < 15 ? prun : 15), run, n1);
+          // Run, then level and sign (src/encode.rs:419-432): one codeword
+          // of both when it fits 64 bits.
+          v1 = book_code32(run_book(pc & 15), run, n1);
           uint32_t ll;
-          const uint64_t lc = book_code(lvl_book(plvl < 8 ? plvl : 8), lvl, ll);
+          const uint64_t lc = book_code32(lvl_book(pc >> 4), lvl, ll);
           v2 = (lc << 1) | (q < 0 ? 1u : 0u);
           n2 = ll + 1;
+          if (n1 + n2 <= 64) {
+            v1 = (v1 << n2) | v2;
+            n1 += n2;
+            n2 = 0;
+          }
         }
-        const int last = 63 - (int)__builtin_clzll(mask);
-        cx.idx = base + (uint32_t)last;
-        cx.run = __shfl(run, last, 64);
-        cx.lvl = __shfl(lvl, last, 64);
+        const uint32_t last = 63 - (uint32_t)__builtin_clzll(mask);
+        last_idx = base + last;
+        ctx = __builtin_amdgcn_readlane(mine, last);
       }
       const uint32_t T = n1 + n2;
-      const uint32_t o = pos + excl_scan(T, L);
-      const uint32_t chunk = __shfl(o + T, 63, 64) - pos;
+      const uint32_t incl = wave_incl_scan(T);
+      const uint32_t o = pos + incl - T;
+      const uint32_t chunk = __builtin_amdgcn_readlane(incl, 63);
       if (n1) ring_put(ring, o, v1, n1);
       if (n2) ring_put(ring, o + n1, v2, n2);
-      __syncthreads();
-      const uint32_t done = (pos + chunk) >> 5;
-      for (uint32_t w = flushed + L; w < done; w += 64) {
-        const uint32_t val = ring[w & (RING - 1)];
-        store_word(A, w, val, w == 0 ? lead : 0u, 4);
-        ring[w & (RING - 1)] = 0;
-      }
-      flushed = done;
       pos += chunk;
-      __syncthreads();
+      // A chunk adds at most 64 x 128 bits (256 words): with fewer than 256
+      // complete words held, the ring's 512 always have room for the next.
+      if ((pos >> 5) - flushed >= 256) flush(pos >> 5);
+      if (!todo) break;
+      j = jn;
+      f = fn;
     }
     pos = (pos + 7) & ~7u;  // zero bits to the byte boundary (src/encode.rs:436)
     comp_bytes[c] = (pos - start) / 8;
+    k0 += nb;
   }
-  // Words completed by the last byte alignment, then the last, partial word.
-  for (uint32_t w = flushed + L; w < (pos >> 5); w += 64)
-    store_word(A, w, ring[w & (RING - 1)], w == 0 ? lead : 0u, 4);
+  // The complete words held, then the last, partial word.
+  flush(pos >> 5);
   if ((pos & 31) && L == 0) {
     const uint32_t w = pos >> 5;
     store_word(A, w, ring[w & (RING - 1)], w == 0 ? lead : 0u, (pos & 31) / 8);
