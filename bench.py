@@ -97,19 +97,23 @@ def alpha_stream_len(packet):
     return total
 
 
-def profiled_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` on `workload`, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), or None."""
-    path = os.path.join(ROOT, "profiles", "decode_c2_hbm_traffic.md")
+def profile_block(name, workload):
+    """The machine-readable block of a committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/<name>) when it covers `workload`, or None."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         text = open(path).read()
         block = text[text.index("```json") + 7:]
         d = json.loads(block[:block.index("```")])
     except (OSError, ValueError):
         return None
-    if d.get("workload") != workload:
-        return None
-    return d.get("traffic_bytes", {}).get(kernel)
+    return d if d.get("workload") == workload else None
+
+
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on the decode `workload`, or None."""
+    d = profile_block("decode_c2_hbm_traffic.md", workload)
+    return None if d is None else d.get("traffic_bytes", {}).get(kernel)
 
 
 def host_cpu():
@@ -325,7 +329,12 @@ def leg_batch_encode(P, synth, steps, warmup, check, keep_for_cpu):
     compared with the oracle's encode of the same frame and Config.
     `roofline` is the whole launch sequence's: algorithmic bytes (planes read,
     packets written) over the batch's kernel time; `stage_us` breaks that
-    time down kernel by kernel (HIP events on the library's stream)."""
+    time down kernel by kernel (HIP events on the library's stream).
+    `traffic` is the batch's HBM bytes from the committed FETCH_SIZE /
+    WRITE_SIZE summary (profiles/encode_c4_hbm_traffic.md); `dominant` is the
+    longest stage, its kernels, their traffic and the rate they move it at
+    (the size passes read the engine's coefficient intermediate: they have no
+    algorithmic bytes of their own)."""
     rng = np.random.default_rng(11)
     frames, items, bufs = [], [], []
     for i in range(32):
@@ -375,17 +384,30 @@ def leg_batch_encode(P, synth, steps, warmup, check, keep_for_cpu):
     kernels_us = sum(stages[k] for k in P.ENCODE_STAGES[:-1])
     dom = max(P.ENCODE_STAGES[:-1], key=lambda k: stages[k])
     gbs = (in_bytes + out_bytes) / (kernels_us * 1e-6) / 1e9
-    out = {"workload": "encode 32 x 3840x2160 422 HQ 10-bit, 8-MB slices, half loaded matrices, "
-                       "a quarter with hard-edged rectangles",
+    workload = ("encode 32 x 3840x2160 422 HQ 10-bit, 8-MB slices, half loaded matrices, "
+                "a quarter with hard-edged rectangles")
+    # HBM bytes a batch, whole sequence and dominant stage, from the committed
+    # FETCH_SIZE / WRITE_SIZE summary of this workload.
+    prof = profile_block("encode_c4_hbm_traffic.md", workload)
+    traffic = dom_traffic = None
+    if prof:
+        tb, per = prof["traffic_bytes"], prof["launches_per_batch"]
+        traffic = int(sum(tb[k] * per[k] for k in tb))
+        dom_traffic = int(sum(tb[k] * per[k] for k in prof["stages"].get(dom, []) if k in tb))
+    dominant = {"stage": dom, "us": round(stages[dom], 1),
+                "kernels": prof["stages"].get(dom) if prof else None, "traffic": dom_traffic,
+                "traffic_gbs": round(dom_traffic / (stages[dom] * 1e-6) / 1e9, 1) if dom_traffic else None,
+                "algorithmic_bytes": {"fdct": in_bytes, "pack": out_bytes}.get(dom, 0)}
+    out = {"workload": workload,
            "frames": 32, "batched_fps": round(32 / b_s, 2), "plain_fps": round(32 / p_s, 2),
            "batched_over_plain": round(p_s / b_s, 3), "ms_per_batch": round(b_s * 1e3, 3),
            "stage_us": {k: round(v, 1) for k, v in stages.items()},
            "rate_control_us": round(stages["probes"] + stages["window"] + stages["shareout"] + stages["offsets"], 1),
            "kernels_us": round(kernels_us, 1),
-           "roofline": {"bound": "hbm", "kernel": "the whole launch sequence (dominant: pe_" + dom + ")",
+           "roofline": {"bound": "hbm", "kernel": "the whole launch sequence",
                         "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                        "algorithmic_bytes": int(in_bytes + out_bytes)},
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                        "algorithmic_bytes": int(in_bytes + out_bytes), "dominant": dominant},
            "target_frame_bytes": target, "max_frame_bytes": max(sizes), "min_frame_bytes": min(sizes),
            "all_at_or_under_target": all(s <= target for s in sizes),
            "packets_fnv1a": fnv_packets(P, got)}
