@@ -1529,8 +1529,18 @@ extern "C" __global__ void __launch_bounds__(256)
         s_rstride[tid - 32] = P.row_step * F.plane_w[tid - 32];
       }
       {
+        // The item's coefficients (at most IDCT_MAX_BLOCKS = 96 blocks, three
+        // 16-byte pieces a thread), every load issued before the first store.
+        static_assert(IDCT_MAX_BLOCKS * 8 <= 3 * 256, "three pieces a thread");
         const uint4 *src = (const uint4 *)(coefs + S0.coef);
-        for (uint32_t i = tid; i < I.blocks * 8; i += 256) ((uint4 *)sc)[i] = src[i];
+        const uint32_t n16 = I.blocks * 8;
+        // Indices past the item's end read its last piece again (n16 >= 8).
+        const uint4 v0 = src[tid < n16 ? tid : n16 - 1];
+        const uint4 v1 = src[tid + 256 < n16 ? tid + 256 : n16 - 1];
+        const uint4 v2 = src[tid + 512 < n16 ? tid + 512 : n16 - 1];
+        if (tid < n16) ((uint4 *)sc)[tid] = v0;
+        if (tid + 256 < n16) ((uint4 *)sc)[tid + 256] = v1;
+        if (tid + 512 < n16) ((uint4 *)sc)[tid + 512] = v2;
       }
       __syncthreads();
       // Dequantisation factors, scale[k] = (f32)(W[k] qScale) / 8 for each
