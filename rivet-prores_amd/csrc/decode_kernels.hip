@@ -1153,6 +1153,7 @@ extern "C" __global__ void __launch_bounds__(ST_CHUNK)
       const uint32_t row = i / P.n_segs;
       IdctItem it = segs[P.seg_base + (i - row * P.n_segs)];
       it.first += P.slice_base + row * P.row_slices;
+      it.pic = p;
       items[P.item_base + i] = it;
     }
     __syncthreads();
@@ -1505,9 +1506,9 @@ extern "C" __global__ void __launch_bounds__(256)
     for (uint32_t it = first; it < last; it++) {
       const IdctItem I = items[it];
       const SliceRec S0 = slices[I.first];
-      const DecPic &P = pics[S0.pic];
+      const DecPic &P = pics[I.pic];
       const DecFrame &F = frames[P.frame];
-      const bool pic_ok = pic_status[S0.pic] == PIC_OK;
+      const bool pic_ok = pic_status[I.pic] == PIC_OK;
       const uint32_t chroma = F.chroma, cpm = chroma == 2 ? 2u : 4u;
       // The item's slices: block offsets, qScale, position, error state; the
       // frame's weights, scan and planes.

@@ -140,9 +140,10 @@ constexpr uint32_t ST_CHUNK = 256;
 // (their coefficients are contiguous in the arena). As a row segment,
 // `first` counts from the start of the row; as an item, from the batch's
 // first slice. pd_slice_table expands each picture's row segments into its
-// items (rows x segments).
+// items (rows x segments), and gives each item its picture (`pic`: a row
+// segment leaves it 0), so pd_idct reads the picture beside the slices.
 struct IdctItem {
-  uint32_t first, count, blocks, pad;
+  uint32_t first, count, blocks, pic;
 };
 // ---------------------------------------------------------------------------
 // Device-planned batches (pd_plan): the frame and picture headers of packets
