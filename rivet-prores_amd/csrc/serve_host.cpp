@@ -2,9 +2,14 @@
 // admitted while their device-memory footprint fits the server's budget, and
 // are batched as they arrive: a worker takes every admissible request each
 // time it comes round, so requests that arrive while a batch is on the GPU
-// ride the next one. Decode batches and encode batches run on lanes of their
-// own (separate streams), so their work overlaps on the device. Results stay
-// resident in device memory until the client releases them.
+// ride the next one. Admission is first in, first out. While a worker's
+// batch is on the GPU, a stager thread of its kind admits the requests that
+// arrive and uploads their inputs on a stream of its own, so the next
+// batch's uploads overlap the kernels of the one running; an idle worker
+// admits and uploads requests itself. Decode
+// batches and encode batches run on lanes of their own (separate streams),
+// so their work overlaps on the device. Results stay resident in device
+// memory until the client releases them.
 //
 // Device buffers come from a caching pool: freed buffers wait in size-class
 // lists for reuse and are returned to HIP only when the server is destroyed
@@ -107,8 +112,12 @@ struct Request {
   void *dev_in = nullptr, *dev_in_alpha = nullptr, *dev_out = nullptr, *dev_out_alpha = nullptr;
   size_t in_bytes = 0, in_alpha_bytes = 0, out_bytes = 0, out_alpha_bytes = 0;
   Err err;
+  hipEvent_t uploaded = nullptr;  // recorded after the input's upload on its stager's stream
   double t_submit = 0, t_start = 0, t_done = 0;
   uint32_t batch_frames = 0;
+  ~Request() {
+    if (uploaded) (void)hipEventDestroy(uploaded);
+  }
 };
 
 // Bytes a decode holds while in flight: the packet, the planes, the alpha
@@ -148,75 +157,126 @@ struct prores_server {
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   Pool pool;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_ready, cv_done;
   bool stop = false;
   uint64_t next_id = 1;
-  std::deque<Request *> pend[2];  // decode, encode
+  std::deque<Request *> pend[2];   // decode, encode: waiting for admission
+  std::deque<Request *> ready[2];  // admitted, inputs uploading or uploaded
+  hipStream_t upload[2] = {nullptr, nullptr};
+  bool busy[2] = {false, false};   // the worker has a batch on the GPU
   std::deque<Request *> done;
   std::map<uint64_t, Request *> live;  // admitted and not yet released
   uint64_t held = 0;
   prores_server_stats stats;
-  std::thread workers[2];
+  std::thread workers[2], stagers[2];
 
   bool fits(const Request *r) const { return held + r->hold <= cfg.hbm_budget; }
+  Request *admit(int which);
+  void stage(Request *r, hipStream_t st, bool event);
+  void stager(int which);
   void worker(int which);
   void finish(std::vector<Request *> &batch);
 };
 
-// One batch of one kind: buffers, uploads, the batch call, results.
+// The front request of a kind's queue, admitted (mu held; it fits).
+Request *prores_server::admit(int which) {
+  Request *r = pend[which].front();
+  pend[which].pop_front();
+  held += r->hold;
+  stats.held_bytes = held;
+  stats.peak_held_bytes = std::max(stats.peak_held_bytes, held);
+  live[r->id] = r;
+  return r;
+}
+
+// An admitted request's device buffers and its upload on stream `st`; with
+// `event`, an event recorded after it for the batch to wait on (an upload on
+// the batch's own stream needs none).
+void prores_server::stage(Request *r, hipStream_t st, bool event) {
+  Err e;
+  if (r->kind == DECODE) {
+    prores_plane pl[3];
+    r->out_bytes = prores_frame_layout(r->hdr.width, r->hdr.height, r->hdr.chroma, pl) * 2;
+    r->out_alpha_bytes = r->hdr.alpha ? (size_t)r->hdr.width * r->hdr.height * 2 : 0;
+    r->in_bytes = r->packet.bytes + 128;  // the decoder reads up to 64 bytes past a packet
+    e = pool.get(r->in_bytes + 64, r->dev_in);
+    if (e.ok()) e = pool.get(r->out_bytes + 64, r->dev_out);
+    if (e.ok() && r->out_alpha_bytes) e = pool.get(r->out_alpha_bytes + 64, r->dev_out_alpha);
+    if (e.ok())
+      e = check(hipMemcpyAsync(r->dev_in, r->packet.p, r->packet.bytes, hipMemcpyHostToDevice, st), "H2D packet");
+  } else {
+    r->in_bytes = r->data.bytes;
+    r->in_alpha_bytes = r->alpha.bytes;
+    e = pool.get(r->in_bytes + 64, r->dev_in);
+    if (e.ok() && r->in_alpha_bytes) e = pool.get(r->in_alpha_bytes + 64, r->dev_in_alpha);
+    if (e.ok()) e = check(hipMemcpyAsync(r->dev_in, r->data.p, r->in_bytes, hipMemcpyHostToDevice, st), "H2D frame");
+    if (e.ok() && r->in_alpha_bytes)
+      e = check(hipMemcpyAsync(r->dev_in_alpha, r->alpha.p, r->in_alpha_bytes, hipMemcpyHostToDevice, st),
+                "H2D alpha");
+  }
+  if (e.ok() && event) e = check(hipEventCreateWithFlags(&r->uploaded, hipEventDisableTiming), "hipEventCreate");
+  if (e.ok() && event) e = check(hipEventRecord(r->uploaded, st), "hipEventRecord");
+  // A request that failed here goes back to the pool without a batch: no
+  // copy of it may still be in flight then.
+  if (!e.ok()) (void)hipStreamSynchronize(st);
+  r->err = e;
+}
+
+// While the kind's worker has a batch on the GPU: admission and upload of
+// the requests that arrive, one at a time, for the worker's next batch.
+void prores_server::stager(int which) {
+  for (;;) {
+    Request *r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_work.wait(lk, [&] { return stop || (busy[which] && !pend[which].empty() && fits(pend[which].front())); });
+      if (stop) return;
+      r = admit(which);
+    }
+    stage(r, upload[which], true);
+    {
+      std::lock_guard<std::mutex> g(mu);
+      ready[which].push_back(r);
+    }
+    cv_ready.notify_all();
+  }
+}
+
+// One batch of one kind: the requests the stager uploaded, then any the
+// worker admits and uploads itself, the batch call, results.
 void prores_server::worker(int which) {
   Lane &L = which == 0 ? *dec_lane : *enc_lane;
   for (;;) {
     std::vector<Request *> batch;
+    size_t staged = 0;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] { return stop || (!pend[which].empty() && fits(pend[which].front())); });
+      busy[which] = false;
+      cv_ready.wait(lk, [&] {
+        return stop || !ready[which].empty() || (!pend[which].empty() && fits(pend[which].front()));
+      });
       if (stop) return;
-      // First in, first out: a request that does not fit waits, and so do
-      // the ones behind it.
-      while (!pend[which].empty() && batch.size() < cfg.max_batch_frames && fits(pend[which].front())) {
-        Request *r = pend[which].front();
-        pend[which].pop_front();
-        held += r->hold;
-        stats.held_bytes = held;
-        stats.peak_held_bytes = std::max(stats.peak_held_bytes, held);
-        live[r->id] = r;
-        batch.push_back(r);
+      while (!ready[which].empty() && batch.size() < cfg.max_batch_frames) {
+        batch.push_back(ready[which].front());
+        ready[which].pop_front();
       }
+      staged = batch.size();
+      while (!pend[which].empty() && batch.size() < cfg.max_batch_frames && fits(pend[which].front()))
+        batch.push_back(admit(which));
+      busy[which] = true;
     }
+    cv_work.notify_all();  // the stager takes the arrivals from here on
+    std::lock_guard<std::mutex> lane_lock(L.mu);
+    for (size_t i = staged; i < batch.size(); i++) stage(batch[i], L.stream, false);
     const double start = now_us(t0);
     for (Request *r : batch) {
       r->t_start = start;
       r->batch_frames = (uint32_t)batch.size();
     }
-    std::lock_guard<std::mutex> lane_lock(L.mu);
-    // Device buffers and uploads.
-    for (Request *r : batch) {
-      Err e;
-      if (r->kind == DECODE) {
-        prores_plane pl[3];
-        r->out_bytes = prores_frame_layout(r->hdr.width, r->hdr.height, r->hdr.chroma, pl) * 2;
-        r->out_alpha_bytes = r->hdr.alpha ? (size_t)r->hdr.width * r->hdr.height * 2 : 0;
-        r->in_bytes = r->packet.bytes + 128;  // the decoder reads up to 64 bytes past a packet
-        e = pool.get(r->in_bytes + 64, r->dev_in);
-        if (e.ok()) e = pool.get(r->out_bytes + 64, r->dev_out);
-        if (e.ok() && r->out_alpha_bytes) e = pool.get(r->out_alpha_bytes + 64, r->dev_out_alpha);
-        if (e.ok())
-          e = check(hipMemcpyAsync(r->dev_in, r->packet.p, r->packet.bytes, hipMemcpyHostToDevice, L.stream),
-                    "H2D packet");
-      } else {
-        r->in_bytes = r->data.bytes;
-        r->in_alpha_bytes = r->alpha.bytes;
-        e = pool.get(r->in_bytes + 64, r->dev_in);
-        if (e.ok() && r->in_alpha_bytes) e = pool.get(r->in_alpha_bytes + 64, r->dev_in_alpha);
-        if (e.ok())
-          e = check(hipMemcpyAsync(r->dev_in, r->data.p, r->in_bytes, hipMemcpyHostToDevice, L.stream), "H2D frame");
-        if (e.ok() && r->in_alpha_bytes)
-          e = check(hipMemcpyAsync(r->dev_in_alpha, r->alpha.p, r->in_alpha_bytes, hipMemcpyHostToDevice, L.stream),
-                    "H2D alpha");
-      }
-      r->err = e;
-    }
+    // The batch's kernels run after the stager's uploads have arrived.
+    for (Request *r : batch)
+      if (r->err.ok() && r->uploaded)
+        r->err = check(hipStreamWaitEvent(L.stream, r->uploaded, 0), "hipStreamWaitEvent");
     std::vector<Request *> ok;
     for (Request *r : batch)
       if (r->err.ok()) ok.push_back(r);
@@ -305,6 +365,7 @@ void prores_server::finish(std::vector<Request *> &batch) {
   }
   cv_done.notify_all();
   cv_work.notify_all();
+  cv_ready.notify_all();
 }
 
 extern "C" {
@@ -348,8 +409,20 @@ prores_status prores_server_create(const prores_server_config *cfg, prores_serve
     fill_error(err, e);
     return (prores_status)e.kind;
   }
-  s->workers[0] = std::thread([s] { s->worker(0); });
-  s->workers[1] = std::thread([s] { s->worker(1); });
+  for (int k = 0; k < 2 && e.ok(); k++)
+    e = check(hipStreamCreateWithFlags(&s->upload[k], hipStreamNonBlocking), "hipStreamCreate(serve upload)");
+  if (!e.ok()) {
+    for (hipStream_t st : s->upload)
+      if (st) (void)hipStreamDestroy(st);
+    prores_decoder_destroy(s->decoder);
+    delete s;
+    fill_error(err, e);
+    return (prores_status)e.kind;
+  }
+  for (int k = 0; k < 2; k++) {
+    s->stagers[k] = std::thread([s, k] { s->stager(k); });
+    s->workers[k] = std::thread([s, k] { s->worker(k); });
+  }
   *out = s;
   fill_error(err, Err{});
   return PRORES_OK;
@@ -362,13 +435,23 @@ void prores_server_destroy(prores_server *s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
+  s->cv_ready.notify_all();
   s->cv_done.notify_all();
   for (auto &w : s->workers)
     if (w.joinable()) w.join();
+  for (auto &w : s->stagers)
+    if (w.joinable()) w.join();
+  // Uploads still in flight end before their buffers go.
+  for (hipStream_t st : s->upload)
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
   // Every request once: the queued ones, the admitted ones (live, polled or
   // not) and the ones completed at submit without admission (done only).
   std::set<Request *> all;
   for (auto &q : s->pend) all.insert(q.begin(), q.end());
+  for (auto &q : s->ready) all.insert(q.begin(), q.end());
   for (auto &kv : s->live) all.insert(kv.second);
   all.insert(s->done.begin(), s->done.end());
   for (Request *r : all) delete r;
@@ -398,6 +481,7 @@ static uint64_t enqueue(prores_server *s, Request *r, int which) {
   }
   s->pend[which].push_back(r);
   s->cv_work.notify_all();
+  s->cv_ready.notify_all();
   return r->id;
 }
 
@@ -483,6 +567,7 @@ prores_status prores_server_release(prores_server *s, uint64_t id) {
   s->pool.put(r->dev_out_alpha, r->out_alpha_bytes + 64);
   delete r;
   s->cv_work.notify_all();
+  s->cv_ready.notify_all();
   return PRORES_OK;
 }
 
