@@ -352,6 +352,7 @@ typedef struct {
   uint64_t submitted, completed, failed;
   uint64_t decode_batches, encode_batches, max_batch_frames;
   uint64_t held_bytes, peak_held_bytes, hbm_budget, pending;
+  uint64_t staged_ahead; /* requests uploaded while their kind's previous batch was on the GPU */
 } prores_server_stats;
 
 void prores_server_config_default(prores_server_config *cfg);

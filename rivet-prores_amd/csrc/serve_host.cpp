@@ -237,6 +237,7 @@ void prores_server::stager(int which) {
     {
       std::lock_guard<std::mutex> g(mu);
       ready[which].push_back(r);
+      stats.staged_ahead++;
     }
     cv_ready.notify_all();
   }

@@ -126,7 +126,7 @@ class _ServerResult(C.Structure):
 class _ServerStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "failed", "decode_batches", "encode_batches",
                                            "max_batch_frames", "held_bytes", "peak_held_bytes", "hbm_budget",
-                                           "pending")]
+                                           "pending", "staged_ahead")]
 
 
 class _DevDecodeJob(C.Structure):

@@ -115,6 +115,31 @@ def test_alone_and_in_the_mix_are_the_same(requests_):
         assert st["decode_batches"] >= 1 and st["encode_batches"] >= 1
 
 
+def test_uploads_staged_while_a_batch_runs(requests_):
+    """Batches of at most two frames and a burst of requests: the ones that
+    arrive while a batch is on the GPU are uploaded beside it (on the
+    stager's stream, the next batch waiting on their events), and every
+    result equals the one the request gets alone."""
+    with P.Server() as srv:
+        alone = []
+        for req in requests_:
+            rid = submit(srv, req)
+            alone.append(check(drain(srv, 1)[rid], req))
+            srv.release(rid)
+    with P.Server(max_batch_frames=2) as srv:
+        print("case: serve burst, batches of two", flush=True)
+        ids = [submit(srv, req) for req in requests_ * 4]
+        got = drain(srv, len(ids))
+        for k, rid in enumerate(ids):
+            assert check(got[rid], requests_[k % len(requests_)]) == alone[k % len(requests_)]
+            srv.release(rid)
+        st = srv.stats()
+        print("stats:", st, flush=True)
+        assert st["max_batch_frames"] <= 2
+        assert st["staged_ahead"] >= 1
+        assert st["held_bytes"] == 0 and st["pending"] == 0
+
+
 def test_admission_waits_for_release(requests_):
     """With room for one request, the second waits until the first result is
     released."""
