@@ -165,6 +165,15 @@ __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t
   return E_NONE;
 }
 
+// A 16-byte store marked non-temporal: a 64-byte piece of a 128-byte line
+// written that way goes past the L2 without the L2 first reading the line's
+// other half (which an ordinary store makes it do).
+typedef uint32_t nt_u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_nt(GLOBAL uint4 *p, uint4 v) {
+  const nt_u32x4 x = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(x, (GLOBAL nt_u32x4 *)p);
+}
+
 // Writes one component's scanned coefficients as i16, in increasing index
 // order, through a 64-coefficient ring of LDS per lane: half-line h (the 32
 // coefficients from 32 h) is kept in half h & 1 of the lane's line, and a
@@ -172,8 +181,8 @@ __device__ __forceinline__ uint32_t read_code(Reader &r, uint32_t book, uint32_t
 // later half-line leaves the finished one pending; the wave writes out the
 // pending half-lines of all its lanes together: each lane lists its own in
 // the wave's LDS list, then four lanes copy each listed half-line as one
-// 64-byte store, sixteen a store instruction, clearing it in LDS. Zeros
-// between are written too, so the arena needs no clearing. A value outside
+// 64-byte store (non-temporal), sixteen a store instruction, clearing it in
+// LDS. Zeros between are written too, so the arena needs no clearing. A value outside
 // +-32767 is stored as -32768 with the full value beside it in the i32 side
 // arena. The *_all calls are made by every lane of the wave together.
 struct CoefOut {
@@ -195,7 +204,7 @@ struct CoefOut {
   __device__ __forceinline__ void zero_half(uint32_t h) {
     GLOBAL uint4 *o = (GLOBAL uint4 *)(base + 32 * h);
 #pragma unroll
-    for (uint32_t i = 0; i < 4; i++) o[i] = make_uint4(0, 0, 0, 0);
+    for (uint32_t i = 0; i < 4; i++) store_nt(&o[i], make_uint4(0, 0, 0, 0));
   }
   // Coefficient n, of the half-line being filled or the next one.
   __device__ __forceinline__ void store(uint32_t n, int32_t v) {
@@ -232,7 +241,7 @@ struct CoefOut {
         uint4 *src = (uint4 *)(lds + ent.x) + (L & 3);
         const uint4 v = *src;
         *src = make_uint4(0, 0, 0, 0);
-        ((GLOBAL uint4 *)(uintptr_t)(((uint64_t)ent.w << 32) | ent.z))[L & 3] = v;
+        store_nt(&((GLOBAL uint4 *)(uintptr_t)(((uint64_t)ent.w << 32) | ent.z))[L & 3], v);
       }
     }
   }
@@ -269,7 +278,7 @@ struct CoefOut {
       const uint32_t count = lane_u32(nhalves, j) - first;
       GLOBAL int16_t *const b = lane_base(j);
       for (uint32_t k = L >> 2; k < count; k += 16)
-        ((GLOBAL uint4 *)(b + 32 * (first + k)))[L & 3] = make_uint4(0, 0, 0, 0);
+        store_nt(&((GLOBAL uint4 *)(b + 32 * (first + k)))[L & 3], make_uint4(0, 0, 0, 0));
     }
   }
   // Lane j's value of a 32-bit lane variable, and of `base` (j uniform).
