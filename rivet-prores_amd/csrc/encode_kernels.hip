@@ -422,10 +422,17 @@ extern "C" __global__ void __launch_bounds__(256)
       const GLOBAL uint16_t *src =
           (const GLOBAL uint16_t *)(uintptr_t)F.in_plane[c] + (size_t)(P.first_row + P.row_step * prow) * pw;
       float p[8];
+      if (bx + 8 <= pw && ((uintptr_t)(src + bx) & 15) == 0) {  // the row's 8 samples in one load
+        const uint4 v = *(const GLOBAL uint4 *)(src + bx);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-      for (uint32_t x = 0; x < 8; x++) {
-        const uint32_t col = bx + x < pw - 1 ? bx + x : pw - 1;
-        p[x] = (float)src[col] * scale - 256.0f;
+        for (uint32_t x = 0; x < 8; x++) p[x] = (float)((w[x >> 1] >> (16 * (x & 1))) & 0xFFFFu) * scale - 256.0f;
+      } else {  // at the right edge (the last column repeated) or unaligned
+#pragma unroll
+        for (uint32_t x = 0; x < 8; x++) {
+          const uint32_t col = bx + x < pw - 1 ? bx + x : pw - 1;
+          p[x] = (float)src[col] * scale - 256.0f;
+        }
       }
 #pragma unroll
       for (uint32_t u = 0; u < 8; u++) {
