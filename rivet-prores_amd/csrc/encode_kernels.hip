@@ -384,7 +384,7 @@ extern "C" __global__ void __launch_bounds__(256)
     pe_fdct(const DevTables *__restrict__ T, const EncFrame *__restrict__ frames,
             const EncPic *__restrict__ pics, const EncSlice *__restrict__ slices,
             const float *__restrict__ qtabs, float *__restrict__ coefs, float *__restrict__ cbound) {
-  __shared__ float outc[96 * 64];
+  __shared__ __attribute__((aligned(16))) float outc[96 * 64];
   __shared__ float tmp[32 * 64];
   const uint32_t g = blockIdx.x, tid = threadIdx.x;
   const EncSlice S = slices[g];
@@ -455,7 +455,7 @@ extern "C" __global__ void __launch_bounds__(256)
     __syncthreads();
   }
   float *dst = coefs + S.coef;
-  for (uint32_t i = tid; i < TB * 64; i += 256) dst[i] = outc[i];
+  for (uint32_t i = tid; i < TB * 16; i += 256) ((float4 *)dst)[i] = ((const float4 *)outc)[i];  // 16 bytes a store
   // Each chunk's bound for the size count: the largest |F| * fl(8 / W) of
   // its 64 coefficients (the quantiser table's row for qi = 1), four
   // threads a chunk.
