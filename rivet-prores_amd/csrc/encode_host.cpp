@@ -85,6 +85,7 @@ struct EncBatch {
   std::map<std::pair<uint32_t, uint32_t>, uint32_t> slot_index;  // (width in MBs, log2) -> first slot
   uint64_t n_slices = 0;
   uint32_t max_pic_slices = 0;
+  uint32_t max_slice_blocks = 0;  // the largest slice's blocks: pe_fdct's LDS is sized for it
   std::vector<EncState> states;
   // The distinct quantiser-table keys of the batch (luma and chroma
   // weights, interlaced); a picture's qtab names one until run_encode maps
@@ -252,6 +253,7 @@ Err plan_encode(const prores_config &cfg, const prores_frame &f, uint64_t dev_da
     b.n_coefs += 64ull * (4 + 2 * cpm) * wmb * P.rows;
     b.n_slices += P.n_slices;
     b.max_pic_slices = std::max(b.max_pic_slices, P.n_slices);
+    for (uint32_t m : sizes) b.max_slice_blocks = std::max(b.max_slice_blocks, (4 + 2 * cpm) * m);
     b.pics.push_back(P);
     EncState st;
     memset(&st, 0, sizeof st);
@@ -379,8 +381,10 @@ Err run_encode(Device &d, Lane &L, EncBatch &b, std::vector<uint64_t> &offsets, 
     if (!(e = launch(L, d.k_slices, (b.max_pic_slices + 255) / 256, np32, 256, args)).ok()) return e;
   }
   {
+    // LDS for the batch's largest slice: a batch of slices of at most 64
+    // blocks (4:2:2 up to 8 macroblocks) fits six workgroups a CU.
     void *args[] = {&tables, &frames, &pics, &slices, &qtabs, &coefs, &cbnd};
-    if (!(e = launch(L, d.k_fdct, g32, 1, 256, args)).ok()) return e;
+    if (!(e = launch(L, b.max_slice_blocks <= 64 ? d.k_fdct64 : d.k_fdct, g32, 1, 256, args)).ok()) return e;
   }
   (void)hipEventRecord(ev[1], L.stream);
   {

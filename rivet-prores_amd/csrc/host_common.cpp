@@ -408,11 +408,11 @@ Err ensure_encoder(Device &d) {
   Err e = check(hipModuleLoadData(&d.enc_mod, prores_encode_co), "hipModuleLoadData(encode)");
   if (!e.ok()) return e;
   const char *const names[] = {"pe_fdct",     "pe_alpha",   "pe_size", "pe_search", "pe_window",   "pe_shareout",
-                               "pe_offsets", "pe_pack",    "pe_arena", "pe_slices", "pe_size_first"};
+                               "pe_offsets", "pe_pack",    "pe_arena", "pe_slices", "pe_size_first", "pe_fdct64"};
   hipFunction_t *const fns[] = {&d.k_fdct,    &d.k_alpha_enc, &d.k_size,  &d.k_search, &d.k_window,
                                 &d.k_shareout, &d.k_offsets,  &d.k_pack,  &d.k_arena,  &d.k_slices,
-                                &d.k_size_first};
-  e = load_functions(d.enc_mod, names, fns, 11);
+                                &d.k_size_first, &d.k_fdct64};
+  e = load_functions(d.enc_mod, names, fns, 12);
   if (!e.ok()) {
     (void)hipModuleUnload(d.enc_mod);
     d.enc_mod = nullptr;

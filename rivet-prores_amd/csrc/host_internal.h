@@ -102,7 +102,8 @@ struct Device {
                 k_plan = nullptr, k_order = nullptr;
   hipFunction_t k_fdct = nullptr, k_alpha_enc = nullptr, k_size = nullptr, k_search = nullptr,
                 k_shareout = nullptr, k_offsets = nullptr, k_pack = nullptr,
-                k_window = nullptr, k_arena = nullptr, k_slices = nullptr, k_size_first = nullptr;
+                k_window = nullptr, k_arena = nullptr, k_slices = nullptr, k_size_first = nullptr,
+                k_fdct64 = nullptr;  // pe_fdct for batches whose slices have at most 64 blocks
   DevTables *tables = nullptr;
   std::mutex mu;  // lane creation, encoder loading, the batch tuning
   Lane *lanes[MAX_LANES] = {nullptr};
