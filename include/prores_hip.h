@@ -191,8 +191,11 @@ typedef struct {
   uint16_t *dev_alpha;        /* device output alpha (width*height) or NULL if none */
 } prores_dev_decode_job;
 
-/* Decodes n frames with one launch sequence on the library's stream; waits
- * for completion. errs[i] receives frame i's result (kind 0 = Ok). Returns
+/* Decodes n frames with one launch sequence on one of the library's two
+ * per-call streams; waits for completion. Calls from two threads run at
+ * once, each on its own stream and workspace (a third waits for the first
+ * stream), so one batch's planning and launches overlap the other's
+ * kernels. errs[i] receives frame i's result (kind 0 = Ok). Returns
  * PRORES_ERR_DEVICE only for runtime failures, else PRORES_OK. Uses the
  * device's batch tuning (the defaults until prores_set_decode_tuning or
  * prores_autotune_decode sets it). No counterpart in the reference: the
@@ -290,7 +293,9 @@ prores_status prores_encode_device_batch(size_t n, prores_dev_encode_job *jobs, 
 void prores_graph_counters(uint64_t *out3);
 
 /* Microseconds of the last decode batch's kernels by stage, measured with
- * HIP events on the library's stream: [0] slice table, [1] colour entropy
+ * HIP events on its stream (of the per-call batches, the one that ended
+ * last; with two in flight each also shares the GPU with the other while
+ * it runs): [0] slice table, [1] colour entropy
  * decode, [2] alpha decode, [3] idct, [4] all kernels. No counterpart in the
  * reference. */
 void prores_last_batch_timing(double *us5);

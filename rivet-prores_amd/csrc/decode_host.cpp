@@ -428,6 +428,7 @@ Err run_sequence(Device &d, Lane &L, Launch &A, const prores_decode_tuning &t, c
   (void)hipEventElapsedTime(&all, L.ev[0], L.ev[4]);
   for (int i = 0; i < 4; i++) L.dec_us[i] = ms[i] * 1000.0;
   L.dec_us[4] = all * 1000.0;
+  L.dec_seq = ++d.batches;
   return Err{};
 }
 
@@ -588,6 +589,7 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   }
   if (np == 0) {
     L.dec_us[0] = L.dec_us[1] = L.dec_us[2] = L.dec_us[3] = L.dec_us[4] = 0;
+    L.dec_seq = ++d.batches;
   } else if (!(e = ws.hres.reserve(2 * ws.cap_np + 64)).ok() ||
              !(e = run_sequence(d, L, A, t, ws.picres.p, 2 * ws.cap_np, ws.hres.p)).ok()) {
     return e;
@@ -1102,15 +1104,25 @@ extern "C" {
 static prores_status batch_common(const prores_decoder *d, size_t n, const prores_dev_decode_job *jobs,
                                   const prores_decode_tuning *tuning, prores_error *errs) {
   Device *dev = nullptr;
-  Lane *lane0 = nullptr;
+  Lane *ls[2] = {nullptr, nullptr};
   Err e = device(dev);
-  if (e.ok()) e = lane(*dev, 0, lane0);
+  for (int i = 0; i < 2 && e.ok(); i++) e = lane(*dev, CALL_LANES[i], ls[i]);
   if (!e.ok()) {
     for (size_t i = 0; i < n; i++) fill_error(&errs[i], e);
     return (prores_status)e.kind;
   }
-  std::lock_guard<std::mutex> g(lane0->mu);
-  e = decode_batch_on(*dev, *lane0, d->bit_depth, n, jobs, tuning, errs);
+  // The first free call lane; when both are busy, wait for the first.
+  Lane *L = ls[0];
+  std::unique_lock<std::mutex> g(ls[0]->mu, std::try_to_lock);
+  if (!g.owns_lock()) {
+    g = std::unique_lock<std::mutex>(ls[1]->mu, std::try_to_lock);
+    if (g.owns_lock()) {
+      L = ls[1];
+    } else {
+      g = std::unique_lock<std::mutex>(ls[0]->mu);
+    }
+  }
+  e = decode_batch_on(*dev, *L, d->bit_depth, n, jobs, tuning, errs);
   return e.ok() ? PRORES_OK : (prores_status)e.kind;
 }
 
@@ -1337,11 +1349,24 @@ prores_status prores_autotune_decode(const prores_decoder *d, size_t n, const pr
 
 void prores_last_batch_timing(double *us5) {
   Device *dev = nullptr;
-  Lane *lane0 = nullptr;
   for (int i = 0; i < 5; i++) us5[i] = 0;
-  if (!device(dev).ok() || !lane(*dev, 0, lane0).ok()) return;
-  std::lock_guard<std::mutex> g(lane0->mu);
-  for (int i = 0; i < 5; i++) us5[i] = lane0->dec_us[i];
+  if (!device(dev).ok()) return;
+  // The call lane whose batch ended last.
+  uint64_t last = 0;
+  bool any = false;
+  for (int k : CALL_LANES) {
+    Lane *L = nullptr;
+    {
+      std::lock_guard<std::mutex> g(dev->mu);
+      L = dev->lanes[k];
+    }
+    if (!L) continue;
+    std::lock_guard<std::mutex> g(L->mu);
+    if (any && L->dec_seq <= last) continue;
+    any = true;
+    last = L->dec_seq;
+    for (int i = 0; i < 5; i++) us5[i] = L->dec_us[i];
+  }
 }
 
 void prores_graph_counters(uint64_t *out3) {

@@ -6,6 +6,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -88,11 +89,15 @@ struct Lane {
   void *dec = nullptr;  // decode workspace (decode_host.cpp)
   void *enc = nullptr;  // encode workspace (encode_host.cpp)
   double dec_us[5] = {0, 0, 0, 0, 0};  // last decode batch by stage
+  uint64_t dec_seq = 0;                // when it ended, in the device's batch count
   double enc_us[4] = {0, 0, 0, 0};     // last encode batch by stage
   double enc_stage_us[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // see prores_last_encode_stages
   uint64_t enc_counters[4] = {0, 0, 0, 0};  // last encode batch: see prores_last_encode_counters
 };
 constexpr int MAX_LANES = 8;
+// The lanes of the per-call device batch API: a call takes the first that
+// is free, so two callers' batches are in flight at once.
+constexpr int CALL_LANES[2] = {0, 6};
 
 // The per-device context: modules, kernels, constant tables, lanes.
 struct Device {
@@ -109,6 +114,7 @@ struct Device {
   Lane *lanes[MAX_LANES] = {nullptr};
   prores_decode_tuning tuning;
   bool tuned = false;
+  std::atomic<uint64_t> batches{0};  // decode batches ended (Lane::dec_seq)
 };
 // The context of the current device, created on first use.
 Err device(Device *&out);

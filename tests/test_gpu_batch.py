@@ -116,6 +116,42 @@ def test_graph_replays_with_eager_work_between(batch):
         assert dec.decode_device_batch(single, eager) == [None]
 
 
+def test_two_callers_at_once(batch):
+    """Two threads calling the batch entry point at once (each call takes a
+    free per-call stream, so both batches are in flight together), one of
+    them replaying a captured graph, each into planes of its own: every
+    result of every call equals the oracle's."""
+    import threading
+    packets, want, bufs, jobs = batch
+    bufs2, jobs2 = [], []
+    for p, (dp, dd, da, n, h) in zip(packets, bufs):
+        dd2 = P.DeviceBuffer(n * 2 + 64)
+        da2 = P.DeviceBuffer(h.width * h.height * 2 + 64) if h.alpha else None
+        bufs2.append((dp, dd2, da2, n, h))
+        jobs2.append((p, dp.ptr, dd2.ptr, da2.ptr if da2 else 0))
+    jobs2 = P.DecodeJobs(jobs2)
+    dec = P.Decoder()
+    failed = []
+
+    def run(j, b, tuning, tag):
+        try:
+            for rep in range(6):
+                for dp, dd, da, n, h in b:
+                    dd.upload(np.full(n, 0xABCD, np.uint16))
+                check_batch(dec.decode_device_batch(j, tuning), want, b)
+        except Exception as e:  # reported after the join
+            failed.append((tag, e))
+    print("case: two callers at once", flush=True)
+    ts = [threading.Thread(target=run, args=(jobs, bufs, P.DecodeTuning(), "eager")),
+          threading.Thread(target=run, args=(jobs2, bufs2, P.DecodeTuning(0, 1, 96, 256, 1), "graph"))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not failed, failed
+    assert all(v >= 0 for v in P.last_batch_timing())
+
+
 def test_autotune_then_batch(batch):
     packets, want, bufs, jobs = batch
     dec = P.Decoder()
