@@ -206,6 +206,15 @@ __device__ __forceinline__ void load_len_tables(const DevTables *__restrict__ T,
 // first chunk of a component (its DC coefficients) is always coded.
 constexpr float ZERO_CHUNK_LIMIT = 0.5798f;
 
+// Which chunk slice_bytes_q counts: the DC chunk (the first of a component)
+// or another.
+struct DcChunk {
+  static constexpr bool value = true;
+};
+struct AcChunk {
+  static constexpr bool value = false;
+};
+
 // Bytes of one slice at each of the quantisers qis[0..nq), one wavefront,
 // and whether every size field can hold them (src/encode.rs:671-681): per
 // component, the exact bit count of coefficient_bits (src/encode.rs:366-409)
@@ -251,6 +260,7 @@ __device__ __forceinline__ void slice_bytes_q(const float *__restrict__ coef, co
     fits[k] = true;
   }
   uint32_t k0 = 0;  // the component's first chunk in the slice
+#pragma unroll 1
   for (uint32_t c = 0; c < 3; c++) {
     uint32_t nb, off;
     component(c, mbs, cpm, nb, off);
@@ -269,21 +279,22 @@ __device__ __forceinline__ void slice_bytes_q(const float *__restrict__ coef, co
       ctx[k] = 4u | 1u << 4;  // no nonzero yet: run context 4, level context 1
     }
     const float *cf = coef + off + L;
-    uint32_t j = 0;
-    float f = cf[0];
-    for (;;) {
-      todo &= todo - 1;  // chunk j is the lowest bit
-      const uint32_t jn = todo ? (uint32_t)__builtin_ctzll(todo) : j;
-      const float fn = cf[jn * 64];
+    // One chunk, counted for every quantiser it may hold a nonzero at. The
+    // DC chunk (j = 0) also codes the DC coefficients, its lanes below nb;
+    // in every other chunk each lane's scan position is past 0, so each
+    // coefficient rounds with 0.42 and the nonzero test is the quantised
+    // value's alone.
+    auto count_chunk = [&](auto dc, uint32_t j, float f) {
+      constexpr bool DC = decltype(dc)::value;
       const uint32_t base = j * 64, i = base + L, s = i >> lg;
 #pragma unroll
       for (int k = 0; k < NQ; k++) {
         if ((uint32_t)k >= nq || !((lv[k] >> j) & 1)) continue;
         const float inv = __shfl(invL[k], (int)s, 64);
         const float x = f * inv;
-        const uint32_t m = (uint32_t)(int32_t)floorf(fabsf(x) + (s == 0 ? 0.5f : 0.42f));
+        const uint32_t m = (uint32_t)(int32_t)floorf(fabsf(x) + (DC && s == 0 ? 0.5f : 0.42f));
         bool nz = m != 0;
-        if (j == 0) {  // the DC coefficients (src/encode.rs:383-395), nb <= 32 of the first chunk
+        if (DC) {  // the DC coefficients (src/encode.rs:383-395), nb <= 32 of the first chunk
           const int32_t q = x < 0.0f ? -(int32_t)m : (int32_t)m;
           const int32_t d1 = __shfl(q, L ? (int)L - 1 : 0, 64);
           const int32_t d2 = __shfl(q, L > 1 ? (int)L - 2 : 0, 64);
@@ -300,7 +311,7 @@ __device__ __forceinline__ void slice_bytes_q(const float *__restrict__ coef, co
           }
           nz = nz && L >= nb;
         }
-        const uint64_t mask = __ballot(nz);
+        const uint64_t mask = __builtin_amdgcn_ballot_w64(nz);
         if (mask) {
           const uint64_t lower = mask & below;
           const int P = lower ? 63 - (int)__builtin_clzll(lower) : -1;
@@ -310,9 +321,11 @@ __device__ __forceinline__ void slice_bytes_q(const float *__restrict__ coef, co
           uint32_t pc = __shfl(mine, P >= 0 ? P : (int)L, 64);
           if (P < 0) pc = ctx[k];
           uint32_t pair = run_lens[(pc & 15) << 6 | (run & 63)] + level_lens[(pc >> 4) << 8 | (lvl & 255)] + 1;
-          const bool big = nz && (run >= 64 || lvl >= 256);
-          if (__ballot(big)) {
-            if (big) pair = run_code_len(pc & 15, run) + level_code_len(pc >> 4, lvl) + 1;
+          // Symbols past the tables (each ballot of a plain compare is the
+          // compare's own mask).
+          const uint64_t bigs = (__builtin_amdgcn_ballot_w64(run >= 64) | __builtin_amdgcn_ballot_w64(lvl >= 256)) & mask;
+          if (bigs) {
+            if ((bigs >> L) & 1) pair = run_code_len(pc & 15, run) + level_code_len(pc >> 4, lvl) + 1;
           }
           if (nz) bits[k] += pair;
           const uint32_t last = 63 - (uint32_t)__builtin_clzll(mask);
@@ -320,9 +333,21 @@ __device__ __forceinline__ void slice_bytes_q(const float *__restrict__ coef, co
           ctx[k] = __builtin_amdgcn_readlane(mine, last);
         }
       }
-      if (!todo) break;
-      j = jn;
+    };
+    // The chunks in order, the next coded one loaded while this one is
+    // counted (past the last, the last is loaded again).
+    float f = cf[0];
+    todo &= todo - 1;  // the DC chunk, bit 0
+    uint32_t jn = todo ? (uint32_t)__builtin_ctzll(todo) : 0u;
+    float fn = cf[jn * 64];
+    count_chunk(DcChunk{}, 0u, f);
+    while (todo) {
+      const uint32_t j = jn;
       f = fn;
+      todo &= todo - 1;
+      jn = todo ? (uint32_t)__builtin_ctzll(todo) : j;
+      fn = cf[jn * 64];
+      count_chunk(AcChunk{}, j, f);
     }
 #pragma unroll
     for (int k = 0; k < NQ; k++) {
