@@ -532,8 +532,9 @@ Err run_batch(Device &d, Lane &L, Batch &b, const prores_decode_tuning &t, std::
   fit(ws.cap_rows, b.rows.size() * sizeof(RowSlot));
   fit(ws.cap_segs, segs.size() * sizeof(IdctItem));
   fit(ws.cap_chunks, chunks.size() * sizeof(uint32_t));
-  // Entropy item costs, one word a group of 64 slices, zero at upload.
-  const size_t cost_bytes = (size_t)((G + 63) / 64) * 4;
+  // Entropy item costs, one word an item (three a group of 64 slices),
+  // zero at upload.
+  const size_t cost_bytes = (size_t)((G + 63) / 64) * 12;
   fit(ws.cap_cost, cost_bytes);
   const size_t o_pics = ws.cap_frames, o_rows = o_pics + ws.cap_pics, o_segs = o_rows + ws.cap_rows,
                o_chunks = o_segs + ws.cap_segs, o_cost = o_chunks + ws.cap_chunks, o_hdr = o_cost + ws.cap_cost,
@@ -726,7 +727,7 @@ Err run_device_batch(Device &d, Lane &L, uint32_t bit_depth, size_t n, const pro
     const size_t o_pics = al(n * sizeof(DecFrame)), o_rows = o_pics + al((size_t)c.pics * sizeof(DecPic)),
                  o_segs = o_rows + al((size_t)c.rows * sizeof(RowSlot)),
                  o_chunks = o_segs + al((size_t)c.segs * sizeof(IdctItem)),
-                 o_cost = o_chunks + al((size_t)c.chunks * 8), o_hdr = o_cost + al(((size_t)c.slices + 63) / 64 * 4),
+                 o_cost = o_chunks + al((size_t)c.chunks * 8), o_hdr = o_cost + al(((size_t)c.slices + 63) / 64 * 12),
                  plan_bytes = o_hdr + sizeof(DecBatch);
     // Results: job records, totals, picture status, first errors.
     const size_t r_tot = al(n * sizeof(PlanRec)), r_st = r_tot + al(sizeof(PlanTotals)), r_fe = r_st + al(np * 4),

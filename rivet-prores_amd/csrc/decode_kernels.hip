@@ -1080,14 +1080,22 @@ extern "C" __global__ void __launch_bounds__(ST_CHUNK)
     bool listed = false, heavy = false;
     uint32_t alen = 0, ahead[4] = {0, 0, 0, 0};
     uint64_t astream = 0;
-    if (has_alpha && valid && sz >= 8 && data0 + at + sz <= P.picture_size) {
+    // The colour streams' sizes (the entropy items' costs below): the
+    // header's size fields, each at most the slice's size; a slice whose
+    // first 8 bytes are not inside the picture counts its whole size.
+    uint32_t csz[3] = {sz, sz, sz};
+    if (valid && sz >= 8 && data0 + at + sz <= P.picture_size) {
       const GLOBAL uint8_t *sl = (const GLOBAL uint8_t *)(uintptr_t)(P.pic_off + data0 + at);
       const uint32_t hs = sl[0] >> 3, q = sl[1];
-      const uint32_t c = ((uint32_t)sl[2] << 8 | sl[3]) + ((uint32_t)sl[4] << 8 | sl[5]) +
-                         ((uint32_t)sl[6] << 8 | sl[7]);
+      const uint32_t y = (uint32_t)sl[2] << 8 | sl[3], u = (uint32_t)sl[4] << 8 | sl[5];
+      const uint32_t v = has_alpha ? ((uint32_t)sl[6] << 8 | sl[7]) : (hs + y + u < sz ? sz - hs - y - u : 0u);
+      csz[0] = y < sz ? y : sz;
+      csz[1] = u < sz ? u : sz;
+      csz[2] = v < sz ? v : sz;
+      const uint32_t c = y + u + v;
       // slice_header with alpha: 8 <= hs <= size, 1 <= q <= 224, the
       // component sizes inside the body.
-      listed = hs >= 8 && hs <= sz && q >= 1 && q <= 224 && c <= sz - hs;
+      listed = has_alpha && hs >= 8 && hs <= sz && q >= 1 && q <= 224 && c <= sz - hs;
       alen = listed ? sz - hs - c : 0u;
       heavy = listed && alen > ALPHA_HEAVY_BYTES;
       astream = P.pic_off + data0 + at + hs + c;
@@ -1114,20 +1122,25 @@ extern "C" __global__ void __launch_bounds__(ST_CHUNK)
       slice_err[P.slice_base + k] = make_uint4(0, 0, 0, 0);
     }
     {
-      // The entropy items' cost (pd_entropy_order): the largest slice of
-      // each group of 64 consecutive slices of the batch, zero at upload.
-      // A wave's slices fall in at most two groups: one atomic each.
+      // The entropy items' costs (pd_entropy_order): for each group of 64
+      // consecutive slices of the batch, each colour component's largest
+      // stream (item 3 g + c), zero at upload. A wave's slices fall in at
+      // most two groups: one atomic each a component.
       const uint32_t g0 = (P.slice_base + k0 + 64 * wave) / 64, gk = (P.slice_base + k) / 64;
-      uint32_t m0 = valid && gk == g0 ? sz : 0u, m1 = valid && gk != g0 ? sz : 0u;
-      const uint64_t in0 = __ballot(valid && gk == g0), in1 = __ballot(valid && gk != g0);
-      for (uint32_t d = 1; d < 64; d <<= 1) {
-        const uint32_t a = __shfl_xor(m0, d, 64), b = __shfl_xor(m1, d, 64);
-        m0 = a > m0 ? a : m0;
-        m1 = b > m1 ? b : m1;
-      }
-      if (lane == 0) {
-        if (in0) atomicMax(&group_cost[g0], m0);
-        if (in1) atomicMax(&group_cost[g0 + 1], m1);
+      const bool first = valid && gk == g0, second = valid && gk != g0;
+      const uint64_t in0 = __ballot(first), in1 = __ballot(second);
+#pragma unroll
+      for (uint32_t c = 0; c < 3; c++) {
+        uint32_t m0 = first ? csz[c] : 0u, m1 = second ? csz[c] : 0u;
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+          const uint32_t a = __shfl_xor(m0, d, 64), b = __shfl_xor(m1, d, 64);
+          m0 = a > m0 ? a : m0;
+          m1 = b > m1 ? b : m1;
+        }
+        if (lane == 0) {
+          if (in0) atomicMax(&group_cost[3 * g0 + c], m0);
+          if (in1) atomicMax(&group_cost[3 * (g0 + 1) + c], m1);
+        }
       }
     }
     if (has_alpha) {
@@ -1193,12 +1206,19 @@ extern "C" __global__ void __launch_bounds__(ENTROPY_MAX_WG)
     const uint32_t b = blockIdx.x, w = threadIdx.x >> 6, ew = blockDim.x >> 6, nb = gridDim.x;
     uint32_t item = b * ew + w;
     if (order && B->ordered) {
-      // `order` holds the items longest first. Tier k of them (nb items)
-      // goes one to a workgroup, odd tiers in reverse, so every workgroup
-      // (one to a CU) gets one item of each tier; wave w takes tier
-      // (w mod 4) * ew/4 + w/4 (of the mappings tried, the fastest).
-      const uint32_t tier = (ew & 3) == 0 ? (w & 3) * (ew >> 2) + (w >> 2) : w;
-      item = order[tier * nb + ((tier & 1) ? nb - 1 - b : b)];
+      // `order` holds the items longest first. Waves w, w + 4, w + 8, ... of
+      // a workgroup run on one SIMD (the hardware deals a workgroup's waves
+      // to its CU's four SIMDs in turn), and they share its issue slots.
+      // With m of them a SIMD and n = 4 nb SIMD slots in all, the waves of
+      // SIMD slot s take ranks s, n + s, ..., and the last one rank
+      // m n - 1 - s: the longest items share a SIMD with the shortest, so
+      // they run alone at the end, and the SIMDs' loads come out even.
+      if ((ew & 3) == 0) {
+        const uint32_t m = ew >> 2, n = 4 * nb, s = 4 * b + (w & 3), k = w >> 2;
+        item = order[(m > 1 && k == m - 1) ? m * n - 1 - s : k * n + s];
+      } else {
+        item = order[w * nb + ((w & 1) ? nb - 1 - b : b)];
+      }
     }
     const uint32_t g = (item / 3) * 64 + lane;
     if (item < n_items)  // the same for the whole wave
@@ -1223,14 +1243,14 @@ extern "C" __global__ void __launch_bounds__(ENTROPY_MAX_WG)
 // ---------------------------------------------------------------------------
 // One workgroup: the entropy items of the batch in order of cost, longest
 // first, into order[0, n_slots) (slots past the items hold indices past
-// them; n_slots 0: the batch's item count). An item's cost is its largest
-// slice in bytes (pd_slice_table's group costs): a wave's time follows its
+// them; n_slots 0: the batch's item count). An item's cost is its longest
+// stream in bytes (pd_slice_table's costs): a wave's time follows its
 // longest stream. A counting sort on cost / ORDER_BUCKET_BYTES (items of
 // one bucket in any order); the host launches this only for
 // n_slots <= ORDER_MAX.
 constexpr uint32_t ORDER_BUCKETS = 2048, ORDER_BUCKET_BYTES = 32;
 extern "C" __global__ void __launch_bounds__(1024)
-    pd_entropy_order(DecBatch *__restrict__ B, const uint32_t *__restrict__ group_cost,
+    pd_entropy_order(DecBatch *__restrict__ B, const uint32_t *__restrict__ item_cost,
                      uint32_t *__restrict__ order, uint32_t n_slots) {
   __shared__ uint32_t cnt[ORDER_BUCKETS];
   __shared__ uint16_t bucket[ORDER_MAX];
@@ -1243,15 +1263,14 @@ extern "C" __global__ void __launch_bounds__(1024)
   if (n_slots > ORDER_MAX || n_items > n_slots) return;
   for (uint32_t i = tid; i < ORDER_BUCKETS; i += blockDim.x) cnt[i] = 0;
   __syncthreads();
-  // Costs, one a group of 64 slices (its largest slice, from
-  // pd_slice_table); the group's three items share it.
-  for (uint32_t grp = tid; grp < groups; grp += blockDim.x) {
-    const uint32_t v = group_cost[grp];
+  // Costs, one an item (its longest stream, from pd_slice_table).
+  for (uint32_t i = tid; i < n_items; i += blockDim.x) {
+    const uint32_t v = item_cost[i];
     // Descending cost: bucket 0 holds the longest items.
     const uint32_t b = ORDER_BUCKETS - 1 - (v / ORDER_BUCKET_BYTES < ORDER_BUCKETS - 1
                                                 ? v / ORDER_BUCKET_BYTES : ORDER_BUCKETS - 1);
-    bucket[3 * grp] = bucket[3 * grp + 1] = bucket[3 * grp + 2] = (uint16_t)b;
-    atomicAdd(&cnt[b], 3u);
+    bucket[i] = (uint16_t)b;
+    atomicAdd(&cnt[b], 1u);
   }
   __syncthreads();
   // Exclusive scan of the counts, two buckets a thread.
@@ -1971,8 +1990,8 @@ extern "C" __global__ void __launch_bounds__(256)
     *B = D;
   }
   if (over) return;
-  // The entropy item costs start at zero.
-  for (uint32_t i = tid; i < (carry[1] + 63) / 64; i += 256) group_cost[i] = 0;
+  // The entropy item costs (three a group of 64 slices) start at zero.
+  for (uint32_t i = tid; i < 3 * ((carry[1] + 63) / 64); i += 256) group_cost[i] = 0;
   // Each job's frame, pictures, row slots, segments and chunks.
   for (uint32_t j = tid; j < n_jobs; j += 256) {
     PlanRec &R = recs[j];

@@ -229,8 +229,9 @@ struct PlanTotals {
 static_assert(sizeof(PlanTotals) == 48, "PlanTotals layout");
 
 // Entropy work placement: batches of at most ORDER_MAX grid slots have
-// their entropy items ordered by cost (the largest slice of the item's 64,
-// in bytes) so that every workgroup gets a mix of long and short streams.
+// their entropy items ordered by cost (the longest of the item's 64 streams,
+// in bytes), and the waves that share a SIMD get a long, a middle and a
+// short item.
 constexpr uint32_t ORDER_MAX = 8192;
 
 constexpr uint32_t IDCT_MAX_BLOCKS = 96;  // blocks one workgroup holds in LDS
