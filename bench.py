@@ -629,6 +629,9 @@ def main():
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="after the timed steps, one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, for scaling profiler byte counters)")
+    ap.add_argument("--in-flight", type=int, default=2, choices=(1, 2),
+                    help="headline batches in flight: 2 (the value; two host threads) or 1 (one call at a time "
+                         "only: for profiling, so every launch runs without another batch beside it)")
     ap.add_argument("--legs-only", action="store_true",
                     help="run only the --legs (no headline decode, no CPU baseline): for profiling one leg")
     args = ap.parse_args()
@@ -708,7 +711,8 @@ def main():
     for _ in range(args.warmup):
         batched()
         plain()
-    two_in_flight(max(2, 2 * args.warmup))  # the second stream's workspace is made here
+    if args.in_flight == 2:
+        two_in_flight(max(2, 2 * args.warmup))  # the second stream's workspace is made here
     # One batch at a time: the stage times (HIP events, no other batch on
     # the GPU) and the one-in-flight rate.
     stage = np.zeros(5)
@@ -719,13 +723,15 @@ def main():
     one_s = (time.perf_counter() - t0) / args.steps
     stage /= args.steps
     log(f"bench: batched, one in flight {one_s * 1e3:.3f} ms/step")
-    batched_s = two_in_flight(args.steps) / args.steps
-    log(f"bench: batched, two in flight {batched_s * 1e3:.3f} ms/step")
+    batched_s = one_s
+    if args.in_flight == 2:
+        batched_s = two_in_flight(args.steps) / args.steps
+        log(f"bench: batched, two in flight {batched_s * 1e3:.3f} ms/step")
     # Every frame of the last timed batches, before the plain loop reuses
     # the output buffers; the two sets of planes must be equal.
     batched_out = [(outs[i][0].download(samples * 2, dtype=np.uint16),
                     outs[i][1].download(w * h * 2, dtype=np.uint16)) for i in range(n)]
-    for i in range(n):
+    for i in range(n if args.in_flight == 2 else 0):
         if not (np.array_equal(batched_out[i][0], outs2[i][0].download(samples * 2, dtype=np.uint16)) and
                 np.array_equal(batched_out[i][1], outs2[i][1].download(w * h * 2, dtype=np.uint16))):
             raise RuntimeError(f"frame {i}: the two batches in flight decoded different planes")
@@ -770,7 +776,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload,
                    "frames": n, "profile": "ap4x", "bit_depth": 12, "alpha_bits": 16, "log2_slice_mbs": 3},
-        "batches_in_flight": 2,
+        "batches_in_flight": args.in_flight,
         "batched_fps": round(n / batched_s, 3),
         "batched_one_in_flight_fps": round(n / one_s, 3),
         "ms_per_step_one_in_flight": round(one_s * 1e3, 4),
