@@ -116,19 +116,24 @@ def test_graph_replays_with_eager_work_between(batch):
         assert dec.decode_device_batch(single, eager) == [None]
 
 
-def test_two_callers_at_once(batch):
+@pytest.mark.parametrize("device_planned", (False, True), ids=("host-planned", "device-planned"))
+def test_two_callers_at_once(batch, device_planned):
     """Two threads calling the batch entry point at once (each call takes a
     free per-call stream, so both batches are in flight together), one of
     them replaying a captured graph, each into planes of its own: every
-    result of every call equals the oracle's."""
+    result of every call equals the oracle's. Device-planned: the packets'
+    headers are read on the device (each stream's workspace plans them)."""
     import threading
     packets, want, bufs, jobs = batch
+    if device_planned:
+        jobs = P.DecodeJobs([(len(p), dp.ptr, dd.ptr, da.ptr if da else 0)
+                             for p, (dp, dd, da, n, h) in zip(packets, bufs)])
     bufs2, jobs2 = [], []
     for p, (dp, dd, da, n, h) in zip(packets, bufs):
         dd2 = P.DeviceBuffer(n * 2 + 64)
         da2 = P.DeviceBuffer(h.width * h.height * 2 + 64) if h.alpha else None
         bufs2.append((dp, dd2, da2, n, h))
-        jobs2.append((p, dp.ptr, dd2.ptr, da2.ptr if da2 else 0))
+        jobs2.append((len(p) if device_planned else p, dp.ptr, dd2.ptr, da2.ptr if da2 else 0))
     jobs2 = P.DecodeJobs(jobs2)
     dec = P.Decoder()
     failed = []
@@ -141,7 +146,7 @@ def test_two_callers_at_once(batch):
                 check_batch(dec.decode_device_batch(j, tuning), want, b)
         except Exception as e:  # reported after the join
             failed.append((tag, e))
-    print("case: two callers at once", flush=True)
+    print("case: two callers at once, device planned" if device_planned else "case: two callers at once", flush=True)
     ts = [threading.Thread(target=run, args=(jobs, bufs, P.DecodeTuning(), "eager")),
           threading.Thread(target=run, args=(jobs2, bufs2, P.DecodeTuning(0, 1, 96, 256, 1), "graph"))]
     for t in ts:
