@@ -255,6 +255,8 @@ typedef struct {
 } prores_decode_tuning;
 
 void prores_decode_tuning_default(prores_decode_tuning *t);
+/* prores_decode_device_batch with this tuning instead of the device's (the
+ * same per-call streams, so two threads' calls run at once). */
 prores_status prores_decode_device_batch_tuned(const prores_decoder *d, size_t n,
                                                const prores_dev_decode_job *jobs,
                                                const prores_decode_tuning *t, prores_error *errs);
