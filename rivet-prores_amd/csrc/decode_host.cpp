@@ -449,6 +449,13 @@ void launch_shape(Device &d, const prores_decode_tuning &t, uint64_t slices, uin
   } else {
     A.grids[0] = (unsigned)chunks;
     A.grids[1] = (unsigned)((entropy_items + ewaves - 1) / ewaves);
+    // Workgroups of several waves (one a CU) fill every CU: the slots past
+    // the items are empty, and the order deals them to the SIMDs that hold
+    // the longest items.
+    int cus = 0;
+    if (ewaves >= 4 && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d.id) == hipSuccess &&
+        A.grids[1] < (unsigned)cus && (uint64_t)cus * ewaves <= ORDER_MAX)
+      A.grids[1] = (unsigned)cus;
     A.grids[2] = (unsigned)slices;
     A.grids[3] = (unsigned)idct_items;
   }
